@@ -1,0 +1,238 @@
+"""geneva_amd: MI355X-native embedding-extension mining (subgraph matching on
+labelled graphs), with the reference's loader / planner interfaces on the host
+and hand-written gfx950 kernels behind a C ABI.
+
+Python binds the host library (libgeneva_host.so, include/geneva_host.h) with
+ctypes.  Mining always runs the HIP kernels; there is no CPU fallback, and an
+Engine cannot be created when the native libraries or a GPU are missing.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_DIR = os.path.join(_HERE, "lib")
+HOST_LIB = os.path.join(LIB_DIR, "libgeneva_host.so")
+HIP_LIB = os.path.join(LIB_DIR, "libgeneva_hip.so")
+
+GV_OK = 0
+STATUS = {0: "ok", -1: "invalid argument", -2: "HIP runtime error", -3: "device allocation failed",
+          -4: "output capacity exceeded", -5: "malformed partition block", -6: "no HIP device"}
+
+_lib = None
+
+
+class GenevaError(RuntimeError):
+    pass
+
+
+def _u32p():
+    return C.POINTER(C.c_uint32)
+
+
+def lib():
+    """Load libgeneva_host.so (and with it libgeneva_hip.so); raise if absent."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(HOST_LIB) or not os.path.exists(HIP_LIB):
+        raise GenevaError(f"native libraries missing under {LIB_DIR}; run `python -m geneva_amd.build`")
+    L = C.CDLL(HOST_LIB)
+    vp, u32, u64, i32, dbl, cs = C.c_void_p, C.c_uint32, C.c_uint64, C.c_int, C.c_double, C.c_char_p
+    sig = {
+        "gh_last_error": (cs, []),
+        "gh_ds_gen_database": (vp, [u32, u32, u32, dbl, dbl, u32, u32, dbl, u64]),
+        "gh_ds_gen_chung_lu": (vp, [u32, dbl, dbl, dbl, u32, dbl, u64]),
+        "gh_ds_load": (vp, [cs, cs]),
+        "gh_ds_from_arrays": (vp, [u32, _u32p(), u64, _u32p(), _u32p(), u32]),
+        "gh_ds_free": (None, [vp]),
+        "gh_ds_add_walk_queries": (i32, [vp, u32, u32, u32, i32, u32, u64]),
+        "gh_ds_add_shape_queries": (i32, [vp, u64]),
+        "gh_ds_add_query": (i32, [vp, cs]),
+        "gh_ds_nv": (u32, [vp]),
+        "gh_ds_vlabel": (_u32p(), [vp]),
+        "gh_ds_nedges": (u64, [vp]),
+        "gh_ds_edges": (_u32p(), [vp]),
+        "gh_ds_graph_of": (_u32p(), [vp]),
+        "gh_ds_ngraphs": (u32, [vp]),
+        "gh_ds_nqueries": (u32, [vp]),
+        "gh_ds_query": (cs, [vp, u32]),
+        "gh_ds_write": (i32, [vp, cs, i32]),
+        "gh_plan_dump": (u64, [cs, i32, C.c_char_p, u64]),
+        "gh_dv_phase_count": (i32, [cs]),
+        "gh_engine_create": (vp, [vp, i32, u64, C.POINTER(i32)]),
+        "gh_engine_free": (None, [vp]),
+        "gh_engine_graph_bytes": (u64, [vp]),
+        "gh_engine_mine": (i32, [vp, cs, i32, C.POINTER(u64), C.POINTER(u64), C.POINTER(dbl),
+                                 _u32p(), u32, C.POINTER(u64)]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def _err(prefix, rc=None):
+    msg = lib().gh_last_error().decode(errors="replace")
+    if rc is not None:
+        msg = f"{STATUS.get(rc, rc)}: {msg}"
+    return GenevaError(f"{prefix}: {msg}")
+
+
+class Dataset:
+    """A labelled data graph (ids 1..nv), owner graph per vertex and queries."""
+
+    def __init__(self, handle):
+        if not handle:
+            raise _err("dataset")
+        self._h = handle
+
+    def __del__(self):
+        h, self._h = getattr(self, "_h", None), None
+        if h and _lib is not None:
+            _lib.gh_ds_free(h)
+
+    @staticmethod
+    def database(ngraphs, nmin, nmax, avg_deg, ba_fraction, nlabels, zipf_s, seed, ba_m=2):
+        return Dataset(lib().gh_ds_gen_database(ngraphs, nmin, nmax, avg_deg, ba_fraction, ba_m,
+                                                nlabels, zipf_s, seed))
+
+    @staticmethod
+    def chung_lu(nv, gamma, mean_deg, wmax, nlabels, zipf_s, seed):
+        return Dataset(lib().gh_ds_gen_chung_lu(nv, gamma, mean_deg, wmax, nlabels, zipf_s, seed))
+
+    @staticmethod
+    def load(mygraph, metadata):
+        return Dataset(lib().gh_ds_load(mygraph.encode(), metadata.encode()))
+
+    @staticmethod
+    def from_arrays(vlabel, edges, graph_of=None, ngraphs=0):
+        vlabel = np.ascontiguousarray(vlabel, dtype=np.uint32)
+        edges = np.ascontiguousarray(edges, dtype=np.uint32).reshape(-1, 3)
+        go = None if graph_of is None else np.ascontiguousarray(graph_of, dtype=np.uint32)
+        p = lambda a: a.ctypes.data_as(_u32p()) if a is not None else None
+        return Dataset(lib().gh_ds_from_arrays(len(vlabel) - 1, p(vlabel), len(edges), p(edges),
+                                               p(go), ngraphs if go is not None else 0))
+
+    def add_walk_queries(self, nq, kmin, kmax, induced=False, min_extra=0, seed=0):
+        rc = lib().gh_ds_add_walk_queries(self._h, nq, kmin, kmax, int(induced), min_extra, seed)
+        if rc:
+            raise _err("add_walk_queries", rc)
+        return self
+
+    def add_shape_queries(self, seed):
+        lib().gh_ds_add_shape_queries(self._h, seed)
+        return self
+
+    def add_query(self, text):
+        lib().gh_ds_add_query(self._h, text.encode())
+        return self
+
+    @property
+    def nv(self):
+        return lib().gh_ds_nv(self._h)
+
+    @property
+    def vlabel(self):
+        return np.ctypeslib.as_array(lib().gh_ds_vlabel(self._h), shape=(self.nv + 1,)).copy()
+
+    @property
+    def edges(self):
+        m = lib().gh_ds_nedges(self._h)
+        if m == 0:
+            return np.zeros((0, 3), dtype=np.uint32)
+        return np.ctypeslib.as_array(lib().gh_ds_edges(self._h), shape=(m, 3)).copy()
+
+    @property
+    def graph_of(self):
+        return np.ctypeslib.as_array(lib().gh_ds_graph_of(self._h), shape=(self.nv + 1,)).copy()
+
+    @property
+    def ngraphs(self):
+        return lib().gh_ds_ngraphs(self._h)
+
+    @property
+    def queries(self):
+        L = lib()
+        return [L.gh_ds_query(self._h, i).decode() for i in range(L.gh_ds_nqueries(self._h))]
+
+    def write(self, prefix, skip_index=False):
+        rc = lib().gh_ds_write(self._h, prefix.encode(), int(skip_index))
+        if rc:
+            raise _err("write", rc)
+
+
+def plan_dump(query_text, single_v=False):
+    """Planner output in the reference cpumine print layout."""
+    L = lib()
+    n = L.gh_plan_dump(query_text.encode(), int(single_v), None, 0)
+    if n == 0:
+        raise _err("plan")
+    buf = C.create_string_buffer(n + 1)
+    L.gh_plan_dump(query_text.encode(), int(single_v), buf, n + 1)
+    return buf.value.decode()
+
+
+def dv_phase_count(query_text):
+    n = lib().gh_dv_phase_count(query_text.encode())
+    if n < 0:
+        raise _err("dv_phase_count", n)
+    return n
+
+
+class MineResult:
+    __slots__ = ("count", "alg_bytes", "ms", "graph_ids", "launches")
+
+    def __init__(self, count, alg_bytes, ms, graph_ids, launches):
+        self.count, self.alg_bytes, self.ms = count, alg_bytes, ms
+        self.graph_ids, self.launches = graph_ids, launches
+
+    def __repr__(self):
+        return (f"MineResult(count={self.count}, alg_bytes={self.alg_bytes}, ms={self.ms:.3f}, "
+                f"graphs={len(self.graph_ids)}, launches={self.launches})")
+
+
+def _bits_to_ids(words, ngraphs):
+    if ngraphs <= 1:
+        return frozenset()
+    bits = np.unpackbits(np.asarray(words, dtype=np.uint32).view(np.uint8), bitorder="little")
+    return frozenset(np.nonzero(bits[:ngraphs])[0].tolist())
+
+
+class Engine:
+    """A data set's graph resident in HBM on one GPU, with the plain mining path."""
+
+    def __init__(self, ds, device=0, level_bytes=1 << 30):
+        st = C.c_int(0)
+        self._h = lib().gh_engine_create(ds._h, device, level_bytes, C.byref(st))
+        if not self._h:
+            raise _err("engine", st.value)
+        self.ngraphs = ds.ngraphs
+        self._ds = ds
+
+    def close(self):
+        h, self._h = getattr(self, "_h", None), None
+        if h and _lib is not None:
+            _lib.gh_engine_free(h)
+
+    def __del__(self):
+        self.close()
+
+    @property
+    def graph_bytes(self):
+        return lib().gh_engine_graph_bytes(self._h)
+
+    def mine(self, query_text, single_v=False):
+        count, ab, launches = C.c_uint64(0), C.c_uint64(0), C.c_uint64(0)
+        ms = C.c_double(0)
+        nwords = (self.ngraphs + 31) // 32 if self.ngraphs > 1 else 0
+        bits = (C.c_uint32 * max(nwords, 1))()
+        rc = lib().gh_engine_mine(self._h, query_text.encode(), int(single_v), C.byref(count),
+                                  C.byref(ab), C.byref(ms), bits, nwords, C.byref(launches))
+        if rc:
+            raise _err("mine", rc)
+        return MineResult(count.value, ab.value, ms.value,
+                          _bits_to_ids(list(bits)[:nwords], self.ngraphs), launches.value)

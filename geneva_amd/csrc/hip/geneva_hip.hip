@@ -1,0 +1,1005 @@
+// libgeneva_hip.so: gfx950 kernels and the C ABI declared in include/geneva_hip.h.
+//
+// Execution of one extension step over a table of partial embeddings:
+//   1. k_size   : per row, pick the shortest back-neighbour list (label
+//                 segment of the CSR, O(1) lookup), cut it at the restriction
+//                 bound by binary search, record its length.
+//   2. k_scan_* : grid-level inclusive scan of the lengths (uint64) = the
+//                 candidate index space of the step.
+//   3. k_extend : load-balanced expansion over candidate indices.  Each
+//                 wave64 owns a contiguous tile of candidates, so a hub row's
+//                 candidates are spread over as many waves as it needs;
+//                 consecutive lanes read consecutive neighbour entries
+//                 (coalesced).  Each candidate is checked against the
+//                 other back-neighbour lists by binary search (sorted-set
+//                 intersection), the same-label columns (injectivity); the
+//                 wave compacts survivors with ballot + mbcnt, reserves output
+//                 with one atomic, stages the rows in LDS and writes them as
+//                 one contiguous coalesced block.
+//   4. k_count  : the same walk for the last step, counting instead of
+//                 writing; optionally fused with a last step that has a
+//                 single back edge, whose count is a binary search per row.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../../include/geneva_hip.h"
+
+namespace {
+
+thread_local std::string g_last_error;
+
+#define HIPCHK(call)                                                              \
+    do {                                                                          \
+        hipError_t e_ = (call);                                                   \
+        if (e_ != hipSuccess) {                                                   \
+            g_last_error = std::string(#call) + ": " + hipGetErrorString(e_);     \
+            return GV_ERR_HIP;                                                    \
+        }                                                                         \
+    } while (0)
+
+constexpr int kBlock = 256;             // 4 wave64 per workgroup
+constexpr int kWavesPerBlock = kBlock / 64;
+constexpr int kIPT = 4;                 // candidates per lane per tile
+constexpr uint64_t kTile = 64 * kIPT;   // candidates per wave tile
+constexpr int kScanItems = 16;          // per thread in the scan
+constexpr int kScanTile = kBlock * kScanItems;
+
+struct DevGraph {
+    const uint32_t* loff;   // [npart][nv1][nl1]
+    const uint32_t* nbr;
+    const uint32_t* gof;    // owner graph per vertex (nullable)
+    uint64_t part_stride;   // nv1 * nl1
+    uint32_t nl1;           // dense labels + 1
+};
+
+__device__ __forceinline__ void seg(const DevGraph& g, uint32_t part, uint32_t v, uint32_t l,
+                                    uint32_t& b, uint32_t& e) {
+    const uint64_t base = (uint64_t)part * g.part_stride + (uint64_t)v * g.nl1 + l;
+    b = g.loff[base];
+    e = g.loff[base + 1];
+}
+
+// first index in [b, e) with nbr[i] >= x
+__device__ __forceinline__ uint32_t lower_bound_u32(const uint32_t* a, uint32_t b, uint32_t e,
+                                                    uint32_t x) {
+    uint32_t n = e - b;
+    while (n > 0) {
+        uint32_t h = n >> 1;
+        if (a[b + h] < x) {
+            b += h + 1;
+            n -= h + 1;
+        } else {
+            n = h;
+        }
+    }
+    return b;
+}
+
+__device__ __forceinline__ bool contains_u32(const uint32_t* a, uint32_t b, uint32_t e,
+                                             uint32_t x) {
+    uint32_t p = lower_bound_u32(a, b, e, x);
+    return p < e && a[p] == x;
+}
+
+__device__ __forceinline__ unsigned lane_id() { return __lane_id(); }
+
+__device__ __forceinline__ unsigned lanes_below(uint64_t mask) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
+                                     __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+
+__device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// ---------------------------------------------------------------------------
+// sizing
+struct SizeArgs {
+    DevGraph g;
+    const uint32_t* rows;
+    uint64_t nrows;
+    uint32_t L;
+    uint32_t* anc_beg;
+    uint32_t* cnt;
+    uint8_t* anc_i;
+    unsigned long long* bytes;
+    gv_step_t st;
+};
+
+__global__ __launch_bounds__(kBlock) void k_size(SizeArgs a) {
+    uint64_t local = 0;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r < a.nrows; r += stride) {
+        const uint32_t* row = a.rows + r * a.L;
+        uint32_t best_b = 0, best_len = 0xffffffffu, best_i = 0;
+        local += 4ull * a.L;
+        for (uint32_t i = 0; i < a.st.nback; ++i) {
+            uint32_t b, e;
+            seg(a.g, a.st.back_part[i], row[a.st.back_col[i]], a.st.label, b, e);
+            uint32_t len = e - b;
+            local += 8ull + 4ull * len;
+            if (len < best_len) {
+                best_len = len;
+                best_b = b;
+                best_i = i;
+            }
+        }
+        uint32_t bound = a.st.restrict_col != GV_NONE ? row[a.st.restrict_col] : 0xffffffffu;
+        uint32_t cut = lower_bound_u32(a.g.nbr, best_b, best_b + best_len, bound) - best_b;
+        a.anc_beg[r] = best_b;
+        a.cnt[r] = cut;
+        a.anc_i[r] = (uint8_t)best_i;
+    }
+    local = wave_sum_u64(local);
+    if (lane_id() == 0 && local) atomicAdd(a.bytes, (unsigned long long)local);
+}
+
+// ---------------------------------------------------------------------------
+// grid-level inclusive scan of uint32 counts into uint64 sums (reduce, scan of
+// tile sums, scan + add)
+
+__device__ __forceinline__ uint64_t block_exclusive_scan(uint64_t v, uint64_t* wsum,
+                                                         uint64_t& total) {
+    const unsigned lane = lane_id(), w = threadIdx.x >> 6;
+    uint64_t x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        uint64_t y = __shfl_up(x, o, 64);
+        if (lane >= (unsigned)o) x += y;
+    }
+    if (lane == 63) wsum[w] = x;
+    __syncthreads();
+    uint64_t off = 0;
+    total = 0;
+    for (int i = 0; i < kWavesPerBlock; ++i) {
+        if (i < (int)w) off += wsum[i];
+        total += wsum[i];
+    }
+    __syncthreads();
+    return off + x - v;
+}
+
+__global__ __launch_bounds__(kBlock) void k_scan_reduce(const uint32_t* in, uint64_t n,
+                                                        uint64_t* tile_sums) {
+    __shared__ uint64_t wsum[kWavesPerBlock];
+    const uint64_t base = (uint64_t)blockIdx.x * kScanTile;
+    uint64_t s = 0;
+    for (int i = 0; i < kScanItems; ++i) {
+        uint64_t idx = base + (uint64_t)i * kBlock + threadIdx.x;
+        if (idx < n) s += in[idx];
+    }
+    s = wave_sum_u64(s);
+    if (lane_id() == 0) wsum[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint64_t t = 0;
+        for (int i = 0; i < kWavesPerBlock; ++i) t += wsum[i];
+        tile_sums[blockIdx.x] = t;
+    }
+}
+
+// exclusive scan of tile sums in place, one workgroup
+__global__ __launch_bounds__(kBlock) void k_scan_tiles(uint64_t* tile_sums, uint64_t ntiles) {
+    __shared__ uint64_t wsum[kWavesPerBlock];
+    uint64_t carry = 0;
+    for (uint64_t b = 0; b < ntiles; b += kBlock) {
+        uint64_t idx = b + threadIdx.x;
+        uint64_t v = idx < ntiles ? tile_sums[idx] : 0;
+        uint64_t tot;
+        uint64_t ex = block_exclusive_scan(v, wsum, tot);
+        if (idx < ntiles) tile_sums[idx] = carry + ex;
+        carry += tot;
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_scan_final(const uint32_t* in, uint64_t n,
+                                                       const uint64_t* tile_off, uint64_t* out) {
+    __shared__ uint32_t vals[kScanTile];
+    __shared__ uint64_t wsum[kWavesPerBlock];
+    const uint64_t base = (uint64_t)blockIdx.x * kScanTile;
+    for (int i = 0; i < kScanItems; ++i) {
+        uint64_t idx = base + (uint64_t)i * kBlock + threadIdx.x;
+        vals[i * kBlock + threadIdx.x] = idx < n ? in[idx] : 0u;
+    }
+    __syncthreads();
+    uint64_t loc[kScanItems];
+    uint64_t s = 0;
+#pragma unroll
+    for (int i = 0; i < kScanItems; ++i) {
+        s += vals[threadIdx.x * kScanItems + i];
+        loc[i] = s;
+    }
+    uint64_t tot;
+    uint64_t ex = block_exclusive_scan(s, wsum, tot) + tile_off[blockIdx.x];
+    // write back through LDS in two halves to keep coalesced stores
+    __shared__ uint64_t outv[kScanTile / 2];
+    for (int half = 0; half < 2; ++half) {
+        __syncthreads();
+        const int lo = half * (kScanTile / 2);
+#pragma unroll
+        for (int i = 0; i < kScanItems; ++i) {
+            int p = threadIdx.x * kScanItems + i - lo;
+            if (p >= 0 && p < kScanTile / 2) outv[p] = ex + loc[i];
+        }
+        __syncthreads();
+        for (int i = threadIdx.x; i < kScanTile / 2; i += kBlock) {
+            uint64_t idx = base + lo + i;
+            if (idx < n) out[idx] = outv[i];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// extension and counting over candidate tiles
+
+struct WalkArgs {
+    DevGraph g;
+    const uint32_t* rows;
+    uint64_t nrows;
+    uint32_t L;
+    const uint32_t* anc_beg;
+    const uint32_t* cnt;
+    const uint64_t* csum;     // inclusive
+    const uint8_t* anc_i;
+    uint64_t c0, c1;
+    // extension output
+    uint32_t* out;
+    uint64_t out_cap;
+    unsigned long long* out_n;
+    unsigned long long* overflow;
+    // counting
+    unsigned long long* count;
+    uint32_t* gbits;
+    unsigned long long* bytes;
+    gv_step_t st;
+    gv_step_t tail;
+    int has_tail;
+};
+
+// first r in [lo, hi) with csum[r] > c
+__device__ __forceinline__ uint64_t find_row(const uint64_t* csum, uint64_t lo, uint64_t hi,
+                                             uint64_t c) {
+    uint64_t n = hi - lo;
+    while (n > 0) {
+        uint64_t h = n >> 1;
+        if (csum[lo + h] <= c) {
+            lo += h + 1;
+            n -= h + 1;
+        } else {
+            n = h;
+        }
+    }
+    return lo;
+}
+
+__device__ __forceinline__ bool check_candidate(const WalkArgs& a, const uint32_t* row,
+                                                uint32_t v, uint32_t skip) {
+    for (uint32_t j = 0; j < a.st.ncoll; ++j)
+        if (row[a.st.coll_col[j]] == v) return false;
+    for (uint32_t i = 0; i < a.st.nback; ++i) {
+        if (i == skip) continue;
+        uint32_t b, e;
+        seg(a.g, a.st.back_part[i], row[a.st.back_col[i]], a.st.label, b, e);
+        if (!contains_u32(a.g.nbr, b, e, v)) return false;
+    }
+    return true;
+}
+
+template <bool kCount>
+__global__ __launch_bounds__(kBlock) void k_walk(WalkArgs a) {
+    __shared__ uint32_t stage[kWavesPerBlock][64 * GV_MAXK];
+    const unsigned lane = lane_id();
+    const unsigned wib = threadIdx.x >> 6;
+    const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+    const uint32_t W = a.L + 1;
+    uint64_t my_count = 0, my_bytes = 0;
+    uint32_t last_gid = 0xffffffffu;
+    for (uint64_t t0 = a.c0 + wave * kTile; t0 < a.c1; t0 += nwaves * kTile) {
+        const uint64_t t1 = t0 + kTile < a.c1 ? t0 + kTile : a.c1;
+        // rows holding the first and last candidate of the tile
+        uint64_t r_lo = 0, r_hi = 0;
+        if (lane == 0) r_lo = find_row(a.csum, 0, a.nrows, t0);
+        if (lane == 1) r_hi = find_row(a.csum, 0, a.nrows, t1 - 1);
+        r_lo = __shfl(r_lo, 0, 64);
+        r_hi = __shfl(r_hi, 1, 64);
+        for (int it = 0; it < kIPT; ++it) {
+            const uint64_t c = t0 + (uint64_t)it * 64 + lane;
+            bool ok = false;
+            uint32_t v = 0;
+            uint64_t r = 0;
+            if (c < t1) {
+                r = find_row(a.csum, r_lo, r_hi + 1, c);
+                const uint64_t first = a.csum[r] - a.cnt[r];
+                v = a.g.nbr[a.anc_beg[r] + (uint32_t)(c - first)];
+                ok = check_candidate(a, a.rows + r * a.L, v, a.anc_i[r]);
+            }
+            const uint32_t* row = a.rows + r * a.L;
+            if (kCount) {
+                uint64_t n = ok ? 1 : 0;
+                if (ok && a.has_tail) {
+                    // last vertex with a single back edge: count it directly
+                    const gv_step_t& t = a.tail;
+                    const uint32_t bc = t.back_col[0];
+                    const uint32_t w = bc == a.L ? v : row[bc];
+                    uint32_t b, e;
+                    seg(a.g, t.back_part[0], w, t.label, b, e);
+                    uint32_t bound = 0xffffffffu;
+                    if (t.restrict_col != GV_NONE)
+                        bound = t.restrict_col == a.L ? v : row[t.restrict_col];
+                    const uint32_t cut = lower_bound_u32(a.g.nbr, b, e, bound);
+                    n = cut - b;
+                    for (uint32_t j = 0; j < t.ncoll; ++j) {
+                        const uint32_t x = t.coll_col[j] == a.L ? v : row[t.coll_col[j]];
+                        if (x < bound && contains_u32(a.g.nbr, b, cut, x)) --n;
+                    }
+                    my_bytes += 4ull * W + 4ull * W + 8ull + 4ull * (e - b);
+                }
+                my_count += n;
+                if (n && a.gbits) {
+                    const uint32_t gid = a.g.gof[row[0]];
+                    if (gid != last_gid) {
+                        atomicOr(&a.gbits[gid >> 5], 1u << (gid & 31));
+                        last_gid = gid;
+                    }
+                }
+            } else {
+                const uint64_t mask = __ballot(ok);
+                const uint32_t n = (uint32_t)__popcll(mask);
+                if (n == 0) continue;
+                uint64_t base = 0;
+                if (lane == 0) base = atomicAdd(a.out_n, (unsigned long long)n);
+                base = __shfl(base, 0, 64);
+                my_bytes += ok ? 4ull * W : 0;
+                if (base + n > a.out_cap) {
+                    if (lane == 0) atomicOr(a.overflow, 1ull);
+                    continue;
+                }
+                uint32_t* st = stage[wib];
+                if (ok) {
+                    const uint32_t p = lanes_below(mask) * W;
+                    for (uint32_t j = 0; j < a.L; ++j) st[p + j] = row[j];
+                    st[p + a.L] = v;
+                }
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                uint32_t* dst = a.out + base * W;
+                for (uint32_t j = lane; j < n * W; j += 64) dst[j] = st[j];
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+            }
+        }
+    }
+    if (kCount) {
+        my_count = wave_sum_u64(my_count);
+        if (lane == 0 && my_count) atomicAdd(a.count, (unsigned long long)my_count);
+    }
+    my_bytes = wave_sum_u64(my_bytes);
+    if (lane == 0 && my_bytes) atomicAdd(a.bytes, (unsigned long long)my_bytes);
+}
+
+__global__ void k_copy_back(const uint32_t* src, uint32_t* dst, uint64_t n) {
+    // overlap-safe when dst >= src: each element is moved by the thread that
+    // owns it, highest indices first within the grid-stride sweep
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    const uint64_t tid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n == 0) return;
+    for (uint64_t k = tid; k < n; k += stride) {
+        uint64_t i = n - 1 - k;
+        dst[i] = src[i];
+    }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// host side
+
+struct gv_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[8] = {};
+    // accumulators: [0] count, [1] bytes, [2] rows written, [3] overflow
+    unsigned long long* acc = nullptr;
+    uint32_t* gbits = nullptr;
+    uint32_t gbits_words = 0;
+    // scan scratch
+    uint64_t* tiles = nullptr;
+    uint64_t tiles_cap = 0;
+    int num_cu = 256;
+};
+
+struct gv_graph {
+    gv_ctx* ctx = nullptr;
+    uint64_t nv1 = 0;
+    uint32_t nlabels = 0;
+    uint32_t nparts = 0;
+    uint64_t nnbr = 0;
+    uint64_t dev_bytes = 0;
+    std::vector<uint32_t> label_values;       // dense -> original
+    std::vector<uint32_t> part_edge_label;    // partition -> edge label
+    std::vector<uint64_t> lvoff;              // host copy of start-table offsets
+    uint32_t* d_loff = nullptr;
+    uint32_t* d_nbr = nullptr;
+    uint32_t* d_lverts = nullptr;
+    uint32_t* d_gof = nullptr;
+    uint32_t ngraphs = 0;
+};
+
+struct gv_frontier {
+    gv_ctx* ctx = nullptr;
+    uint64_t cap = 0;
+    uint32_t* anc_beg = nullptr;
+    uint32_t* cnt = nullptr;
+    uint8_t* anc_i = nullptr;
+    uint64_t* csum = nullptr;
+    uint64_t nrows = 0;
+    uint64_t total = 0;   // candidates of the last sizing
+};
+
+namespace {
+
+DevGraph dev_view(const gv_graph* g) {
+    DevGraph d;
+    d.loff = g->d_loff;
+    d.nbr = g->d_nbr;
+    d.gof = g->d_gof;
+    d.nl1 = g->nlabels + 1;
+    d.part_stride = g->nv1 * (uint64_t)(g->nlabels + 1);
+    return d;
+}
+
+int grid_for(uint64_t items, uint64_t per_block, int cap_blocks) {
+    uint64_t b = (items + per_block - 1) / per_block;
+    if (b < 1) b = 1;
+    if (b > (uint64_t)cap_blocks) b = cap_blocks;
+    return (int)b;
+}
+
+bool valid_step(const gv_step_t* s, uint32_t row_len, const gv_graph* g) {
+    if (!s || s->nback < 1 || s->nback >= GV_MAXK || s->ncoll >= GV_MAXK) return false;
+    if (s->label >= g->nlabels) return false;
+    if (row_len + 1 > GV_MAXK) return false;
+    for (int i = 0; i < s->nback; ++i)
+        if (s->back_col[i] >= row_len || s->back_part[i] >= g->nparts) return false;
+    for (int i = 0; i < s->ncoll; ++i)
+        if (s->coll_col[i] >= row_len) return false;
+    if (s->restrict_col != GV_NONE && s->restrict_col >= row_len) return false;
+    return true;
+}
+
+// the tail step reads columns 0..row_len (row_len = the fused survivor)
+bool valid_tail(const gv_step_t* s, uint32_t row_len, const gv_graph* g) {
+    if (!s || s->nback != 1 || s->ncoll >= GV_MAXK) return false;
+    if (s->label >= g->nlabels || row_len + 2 > GV_MAXK) return false;
+    if (s->back_col[0] > row_len || s->back_part[0] >= g->nparts) return false;
+    for (int i = 0; i < s->ncoll; ++i)
+        if (s->coll_col[i] > row_len) return false;
+    if (s->restrict_col != GV_NONE && s->restrict_col > row_len) return false;
+    return true;
+}
+
+int scan_counts(gv_ctx* c, const uint32_t* cnt, uint64_t n, uint64_t* csum) {
+    const uint64_t ntiles = (n + kScanTile - 1) / kScanTile;
+    if (ntiles > c->tiles_cap) {
+        if (c->tiles) HIPCHK(hipFree(c->tiles));
+        c->tiles = nullptr;
+        uint64_t cap = std::max<uint64_t>(ntiles, 1024);
+        if (hipMalloc(&c->tiles, cap * sizeof(uint64_t)) != hipSuccess) {
+            c->tiles_cap = 0;
+            return GV_ERR_NOMEM;
+        }
+        c->tiles_cap = cap;
+    }
+    hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)ntiles), dim3(kBlock), 0, c->stream, cnt, n,
+                       c->tiles);
+    hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(kBlock), 0, c->stream, c->tiles, ntiles);
+    hipLaunchKernelGGL(k_scan_final, dim3((unsigned)ntiles), dim3(kBlock), 0, c->stream, cnt, n,
+                       c->tiles, csum);
+    HIPCHK(hipGetLastError());
+    return GV_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* gv_status_string(int s) {
+    switch (s) {
+        case GV_OK: return "ok";
+        case GV_ERR_ARG: return "invalid argument";
+        case GV_ERR_HIP: return "HIP runtime error";
+        case GV_ERR_NOMEM: return "device allocation failed";
+        case GV_ERR_CAPACITY: return "output capacity exceeded";
+        case GV_ERR_FORMAT: return "malformed partition block";
+        case GV_ERR_NODEV: return "no HIP device";
+        default: return "unknown status";
+    }
+}
+
+const char* gv_last_error(void) { return g_last_error.c_str(); }
+
+int gv_device_count(int* n) {
+    if (!n) return GV_ERR_ARG;
+    int c = 0;
+    if (hipGetDeviceCount(&c) != hipSuccess) c = 0;
+    *n = c;
+    return GV_OK;
+}
+
+int gv_ctx_create(int device, gv_ctx_t* out) {
+    if (!out) return GV_ERR_ARG;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return GV_ERR_NODEV;
+    if (device < 0 || device >= n) return GV_ERR_ARG;
+    HIPCHK(hipSetDevice(device));
+    gv_ctx* c = new gv_ctx();
+    c->device = device;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return GV_ERR_HIP;
+    }
+    for (auto& e : c->ev) HIPCHK(hipEventCreate(&e));
+    if (hipMalloc(&c->acc, 8 * sizeof(unsigned long long)) != hipSuccess) {
+        delete c;
+        return GV_ERR_NOMEM;
+    }
+    HIPCHK(hipMemset(c->acc, 0, 8 * sizeof(unsigned long long)));
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->num_cu = prop.multiProcessorCount;
+    *out = c;
+    return GV_OK;
+}
+
+int gv_ctx_destroy(gv_ctx_t c) {
+    if (!c) return GV_ERR_ARG;
+    hipSetDevice(c->device);
+    hipStreamSynchronize(c->stream);
+    if (c->acc) hipFree(c->acc);
+    if (c->gbits) hipFree(c->gbits);
+    if (c->tiles) hipFree(c->tiles);
+    for (auto& e : c->ev)
+        if (e) hipEventDestroy(e);
+    hipStreamDestroy(c->stream);
+    delete c;
+    return GV_OK;
+}
+
+int gv_ctx_synchronize(gv_ctx_t c) {
+    if (!c) return GV_ERR_ARG;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return GV_OK;
+}
+
+int gv_mem_get_info(gv_ctx_t c, uint64_t* f, uint64_t* t) {
+    if (!c || !f || !t) return GV_ERR_ARG;
+    size_t a = 0, b = 0;
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(hipMemGetInfo(&a, &b));
+    *f = a;
+    *t = b;
+    return GV_OK;
+}
+
+int gv_malloc(gv_ctx_t c, uint64_t bytes, void** p) {
+    if (!c || !p) return GV_ERR_ARG;
+    *p = nullptr;
+    if (bytes == 0) return GV_OK;
+    if (hipSetDevice(c->device) != hipSuccess) return GV_ERR_HIP;
+    if (hipMalloc(p, bytes) != hipSuccess) {
+        *p = nullptr;
+        g_last_error = "hipMalloc failed";
+        return GV_ERR_NOMEM;
+    }
+    return GV_OK;
+}
+
+int gv_free(gv_ctx_t c, void* p) {
+    if (!c) return GV_ERR_ARG;
+    if (p) HIPCHK(hipFree(p));
+    return GV_OK;
+}
+
+int gv_memcpy_h2d(gv_ctx_t c, void* dst, const void* src, uint64_t bytes) {
+    if (!c || (bytes && (!dst || !src))) return GV_ERR_ARG;
+    if (bytes) HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
+    return GV_OK;
+}
+
+int gv_memcpy_d2h(gv_ctx_t c, void* dst, const void* src, uint64_t bytes) {
+    if (!c || (bytes && (!dst || !src))) return GV_ERR_ARG;
+    if (bytes) HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return GV_OK;
+}
+
+// Decode the reference partition blocks into the label-segmented CSR.
+int gv_graph_create(gv_ctx_t c, const uint32_t* vLabel, uint32_t nblocks,
+                    const uint32_t* const* blocks, const uint32_t* block_edge_labels,
+                    gv_graph_t* out) {
+    if (!c || !vLabel || !out || (nblocks && (!blocks || !block_edge_labels)) || nblocks > 255)
+        return GV_ERR_ARG;
+    *out = nullptr;
+    const uint32_t maxid = vLabel[0];
+    const uint64_t nv1 = (uint64_t)maxid + 1;
+    // dense labels, ascending original value
+    std::vector<uint32_t> vals;
+    vals.reserve(nv1);
+    for (uint64_t v = 1; v < nv1; ++v)
+        if (vLabel[v] != 0xffffffffu) vals.push_back(vLabel[v]);
+    std::sort(vals.begin(), vals.end());
+    vals.erase(std::unique(vals.begin(), vals.end()), vals.end());
+    const uint32_t NL = (uint32_t)vals.size();
+    if (NL == 0 || NL > 65535) return GV_ERR_FORMAT;
+    std::vector<uint32_t> dl(nv1, 0xffffffffu);
+    for (uint64_t v = 1; v < nv1; ++v)
+        if (vLabel[v] != 0xffffffffu)
+            dl[v] = (uint32_t)(std::lower_bound(vals.begin(), vals.end(), vLabel[v]) - vals.begin());
+    const uint64_t nl1 = NL + 1;
+    std::vector<uint32_t> loff(nv1 * nl1 * (uint64_t)std::max<uint32_t>(nblocks, 1), 0u);
+    std::vector<uint32_t> nbr;
+    for (uint32_t p = 0; p < nblocks; ++p) {
+        const uint32_t* blk = blocks[p];
+        const uint32_t tot = blk[0], S = blk[1], n = blk[2], u4 = blk[4];
+        if (tot < 5 || n == 0 || n > 256 || 5ull + 2 * n + 2 + S + 1 > tot || 5ull + u4 > tot)
+            return GV_ERR_FORMAT;
+        const uint32_t* a0 = blk + 5;
+        const uint32_t* a1 = blk + 5 + 2 * n + 2;
+        const uint32_t* a3 = blk + 5 + u4;
+        const uint64_t nn = tot - 5 - u4;
+        uint32_t* lo = loff.data() + (uint64_t)p * nv1 * nl1;
+        for (uint32_t j = 0; j < n; ++j) {
+            const uint32_t start = a0[j], plo = a0[n + j], phi = a0[n + 1 + j];
+            if (phi < plo || phi > S) return GV_ERR_FORMAT;
+            for (uint32_t k = plo; k < phi; ++k) {
+                const uint64_t src = (uint64_t)start + (k - plo);
+                const uint32_t b = a1[k], e = a1[k + 1];
+                if (src == 0 || src >= nv1 || e < b || e > nn) return GV_ERR_FORMAT;
+                uint32_t* row = lo + src * nl1;
+                uint32_t next_label = 0;
+                uint32_t q = b;
+                while (q < e) {
+                    const uint32_t x = a3[q];
+                    if (x == 0 || x >= nv1 || dl[x] == 0xffffffffu) return GV_ERR_FORMAT;
+                    const uint32_t lab = dl[x];
+                    uint32_t r = q;
+                    while (r < e && a3[r] < nv1 && dl[a3[r]] == lab) ++r;
+                    if (lab < next_label) return GV_ERR_FORMAT;  // groups must ascend
+                    const uint32_t L = r - q;
+                    uint32_t pre = 0;
+                    if (L > 130 && L <= 4128) pre = (L + 128) / 129;
+                    else if (L > 4128) pre = 32;
+                    for (uint32_t l = next_label; l <= lab; ++l) row[l] = (uint32_t)nbr.size();
+                    for (uint32_t s = q + pre; s < r; ++s) {
+                        if (a3[s] == 0 || a3[s] >= nv1) return GV_ERR_FORMAT;
+                        nbr.push_back(a3[s]);
+                    }
+                    next_label = lab + 1;
+                    q = r;
+                }
+                for (uint32_t l = next_label; l <= NL; ++l) row[l] = (uint32_t)nbr.size();
+                if (nbr.size() >= 0xffffffffull) return GV_ERR_FORMAT;
+            }
+        }
+    }
+    // start tables per label
+    std::vector<uint64_t> lvoff(nl1, 0);
+    for (uint64_t v = 1; v < nv1; ++v)
+        if (dl[v] != 0xffffffffu) lvoff[dl[v] + 1]++;
+    for (uint32_t l = 1; l <= NL; ++l) lvoff[l] += lvoff[l - 1];
+    std::vector<uint32_t> lverts(lvoff[NL]);
+    {
+        std::vector<uint64_t> pos(lvoff.begin(), lvoff.end() - 1);
+        for (uint64_t v = 1; v < nv1; ++v)
+            if (dl[v] != 0xffffffffu) lverts[pos[dl[v]]++] = (uint32_t)v;
+    }
+    gv_graph* g = new gv_graph();
+    g->ctx = c;
+    g->nv1 = nv1;
+    g->nlabels = NL;
+    g->nparts = nblocks;
+    g->nnbr = nbr.size();
+    g->label_values = vals;
+    g->part_edge_label.assign(block_edge_labels, block_edge_labels + nblocks);
+    g->lvoff = lvoff;
+    if (nbr.empty()) nbr.push_back(0);
+    if (lverts.empty()) lverts.push_back(0);
+    int rc = GV_OK;
+    auto up = [&](uint32_t** d, const void* h, uint64_t bytes) {
+        if (rc != GV_OK) return;
+        if (hipMalloc(d, bytes) != hipSuccess) { rc = GV_ERR_NOMEM; return; }
+        if (hipMemcpy(*d, h, bytes, hipMemcpyHostToDevice) != hipSuccess) rc = GV_ERR_HIP;
+        g->dev_bytes += bytes;
+    };
+    HIPCHK(hipSetDevice(c->device));
+    up(&g->d_loff, loff.data(), loff.size() * sizeof(uint32_t));
+    up(&g->d_nbr, nbr.data(), nbr.size() * sizeof(uint32_t));
+    up(&g->d_lverts, lverts.data(), lverts.size() * sizeof(uint32_t));
+    if (rc != GV_OK) {
+        gv_graph_destroy(g);
+        return rc;
+    }
+    *out = g;
+    return GV_OK;
+}
+
+int gv_graph_destroy(gv_graph_t g) {
+    if (!g) return GV_ERR_ARG;
+    if (g->ctx) hipSetDevice(g->ctx->device);
+    if (g->d_loff) hipFree(g->d_loff);
+    if (g->d_nbr) hipFree(g->d_nbr);
+    if (g->d_lverts) hipFree(g->d_lverts);
+    if (g->d_gof) hipFree(g->d_gof);
+    delete g;
+    return GV_OK;
+}
+
+int gv_graph_set_owner(gv_graph_t g, const uint32_t* graph_of, uint32_t ngraphs) {
+    if (!g || !graph_of || ngraphs == 0) return GV_ERR_ARG;
+    for (uint64_t v = 0; v < g->nv1; ++v)
+        if (graph_of[v] >= ngraphs) return GV_ERR_ARG;
+    if (g->d_gof) HIPCHK(hipFree(g->d_gof));
+    g->d_gof = nullptr;
+    if (hipMalloc(&g->d_gof, g->nv1 * sizeof(uint32_t)) != hipSuccess) return GV_ERR_NOMEM;
+    HIPCHK(hipMemcpy(g->d_gof, graph_of, g->nv1 * sizeof(uint32_t), hipMemcpyHostToDevice));
+    g->ngraphs = ngraphs;
+    return GV_OK;
+}
+
+int gv_graph_owners(gv_graph_t g, uint32_t* ngraphs) {
+    if (!g || !ngraphs) return GV_ERR_ARG;
+    *ngraphs = g->d_gof ? g->ngraphs : 0;
+    return GV_OK;
+}
+
+int gv_graph_label_index(gv_graph_t g, uint32_t label, uint32_t* dense) {
+    if (!g || !dense) return GV_ERR_ARG;
+    auto it = std::lower_bound(g->label_values.begin(), g->label_values.end(), label);
+    if (it == g->label_values.end() || *it != label) return GV_ERR_ARG;
+    *dense = (uint32_t)(it - g->label_values.begin());
+    return GV_OK;
+}
+
+int gv_graph_part_index(gv_graph_t g, uint32_t edge_label, uint32_t* part) {
+    if (!g || !part) return GV_ERR_ARG;
+    for (uint32_t p = 0; p < g->nparts; ++p)
+        if (g->part_edge_label[p] == edge_label) {
+            *part = p;
+            return GV_OK;
+        }
+    return GV_ERR_ARG;
+}
+
+int gv_graph_info(gv_graph_t g, uint64_t* nv1, uint32_t* nl, uint32_t* np, uint64_t* nnbr,
+                  uint64_t* bytes) {
+    if (!g) return GV_ERR_ARG;
+    if (nv1) *nv1 = g->nv1;
+    if (nl) *nl = g->nlabels;
+    if (np) *np = g->nparts;
+    if (nnbr) *nnbr = g->nnbr;
+    if (bytes) *bytes = g->dev_bytes;
+    return GV_OK;
+}
+
+int gv_graph_start_rows(gv_graph_t g, uint32_t l, const uint32_t** rows, uint64_t* n) {
+    if (!g || !rows || !n || l >= g->nlabels) return GV_ERR_ARG;
+    *rows = g->d_lverts + g->lvoff[l];
+    *n = g->lvoff[l + 1] - g->lvoff[l];
+    return GV_OK;
+}
+
+int gv_frontier_create(gv_ctx_t c, uint64_t cap, gv_frontier_t* out) {
+    if (!c || !out || cap == 0) return GV_ERR_ARG;
+    *out = nullptr;
+    HIPCHK(hipSetDevice(c->device));
+    gv_frontier* f = new gv_frontier();
+    f->ctx = c;
+    f->cap = cap;
+    bool ok = hipMalloc(&f->anc_beg, cap * 4) == hipSuccess &&
+              hipMalloc(&f->cnt, cap * 4) == hipSuccess &&
+              hipMalloc(&f->anc_i, cap) == hipSuccess &&
+              hipMalloc(&f->csum, cap * 8) == hipSuccess;
+    if (!ok) {
+        gv_frontier_destroy(f);
+        return GV_ERR_NOMEM;
+    }
+    *out = f;
+    return GV_OK;
+}
+
+int gv_frontier_destroy(gv_frontier_t f) {
+    if (!f) return GV_ERR_ARG;
+    if (f->anc_beg) hipFree(f->anc_beg);
+    if (f->cnt) hipFree(f->cnt);
+    if (f->anc_i) hipFree(f->anc_i);
+    if (f->csum) hipFree(f->csum);
+    delete f;
+    return GV_OK;
+}
+
+int gv_size_rows(gv_ctx_t c, gv_graph_t g, const gv_step_t* st, const uint32_t* rows,
+                 uint32_t L, uint64_t nrows, gv_frontier_t f, uint64_t* total) {
+    if (!c || !g || !f || !total || !rows || L == 0 || !valid_step(st, L, g)) return GV_ERR_ARG;
+    if (nrows > f->cap) return GV_ERR_CAPACITY;
+    *total = 0;
+    f->nrows = nrows;
+    f->total = 0;
+    if (nrows == 0) return GV_OK;
+    SizeArgs a;
+    a.g = dev_view(g);
+    a.rows = rows;
+    a.nrows = nrows;
+    a.L = L;
+    a.anc_beg = f->anc_beg;
+    a.cnt = f->cnt;
+    a.anc_i = f->anc_i;
+    a.bytes = c->acc + 1;
+    a.st = *st;
+    hipLaunchKernelGGL(k_size, dim3(grid_for(nrows, kBlock, c->num_cu * 16)), dim3(kBlock), 0,
+                       c->stream, a);
+    HIPCHK(hipGetLastError());
+    int rc = scan_counts(c, f->cnt, nrows, f->csum);
+    if (rc != GV_OK) return rc;
+    HIPCHK(hipMemcpyAsync(total, f->csum + (nrows - 1), sizeof(uint64_t), hipMemcpyDeviceToHost,
+                          c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    f->total = *total;
+    return GV_OK;
+}
+
+static int launch_walk(gv_ctx* c, WalkArgs& a, bool count) {
+    const uint64_t span = a.c1 - a.c0;
+    const uint64_t tiles = (span + kTile - 1) / kTile;
+    const int blocks = grid_for(tiles, kWavesPerBlock, c->num_cu * 32);
+    if (count)
+        hipLaunchKernelGGL(k_walk<true>, dim3(blocks), dim3(kBlock), 0, c->stream, a);
+    else
+        hipLaunchKernelGGL(k_walk<false>, dim3(blocks), dim3(kBlock), 0, c->stream, a);
+    HIPCHK(hipGetLastError());
+    return GV_OK;
+}
+
+static void walk_common(WalkArgs& a, gv_ctx* c, gv_graph* g, const gv_step_t* st,
+                        const uint32_t* rows, uint32_t L, gv_frontier* f, uint64_t c0,
+                        uint64_t c1) {
+    memset(&a, 0, sizeof(a));
+    a.g = dev_view(g);
+    a.rows = rows;
+    a.nrows = f->nrows;
+    a.L = L;
+    a.anc_beg = f->anc_beg;
+    a.cnt = f->cnt;
+    a.csum = f->csum;
+    a.anc_i = f->anc_i;
+    a.c0 = c0;
+    a.c1 = c1;
+    a.count = c->acc + 0;
+    a.bytes = c->acc + 1;
+    a.out_n = c->acc + 2;
+    a.overflow = c->acc + 3;
+    a.st = *st;
+}
+
+int gv_ext_emb(gv_ctx_t c, gv_graph_t g, const gv_step_t* st, const uint32_t* rows, uint32_t L,
+               uint64_t nrows, gv_frontier_t f, uint64_t c0, uint64_t c1, uint32_t* out,
+               uint64_t max_rows, uint64_t* written) {
+    if (!c || !g || !f || !written || !valid_step(st, L, g) || c0 > c1) return GV_ERR_ARG;
+    if (nrows != f->nrows || c1 > f->total) return GV_ERR_ARG;
+    *written = 0;
+    if (c0 == c1 || nrows == 0) return GV_OK;
+    if (!out || !rows) return GV_ERR_ARG;
+    WalkArgs a;
+    walk_common(a, c, g, st, rows, L, f, c0, c1);
+    a.out = out;
+    a.out_cap = max_rows;
+    HIPCHK(hipMemsetAsync(c->acc + 2, 0, 2 * sizeof(unsigned long long), c->stream));
+    int rc = launch_walk(c, a, false);
+    if (rc != GV_OK) return rc;
+    unsigned long long h[2];
+    HIPCHK(hipMemcpyAsync(h, c->acc + 2, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (h[1]) return GV_ERR_CAPACITY;
+    *written = h[0];
+    return GV_OK;
+}
+
+int gv_count_emb(gv_ctx_t c, gv_graph_t g, const gv_step_t* st, const gv_step_t* tail,
+                 const uint32_t* rows, uint32_t L, uint64_t nrows, gv_frontier_t f, uint64_t c0,
+                 uint64_t c1) {
+    if (!c || !g || !f || !valid_step(st, L, g) || c0 > c1) return GV_ERR_ARG;
+    if (tail && !valid_tail(tail, L, g)) return GV_ERR_ARG;
+    if (nrows != f->nrows || c1 > f->total) return GV_ERR_ARG;
+    if (c0 == c1 || nrows == 0) return GV_OK;
+    if (!rows) return GV_ERR_ARG;
+    WalkArgs a;
+    walk_common(a, c, g, st, rows, L, f, c0, c1);
+    if (tail) {
+        a.tail = *tail;
+        a.has_tail = 1;
+    }
+    a.gbits = (g->d_gof && c->gbits && c->gbits_words * 32ull >= g->ngraphs) ? c->gbits : nullptr;
+    return launch_walk(c, a, true);
+}
+
+int gv_counters_reset(gv_ctx_t c, gv_graph_t g) {
+    if (!c) return GV_ERR_ARG;
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(hipMemsetAsync(c->acc, 0, 8 * sizeof(unsigned long long), c->stream));
+    if (g && g->ngraphs) {
+        uint32_t words = (g->ngraphs + 31) / 32;
+        if (words > c->gbits_words) {
+            HIPCHK(hipStreamSynchronize(c->stream));
+            if (c->gbits) HIPCHK(hipFree(c->gbits));
+            c->gbits = nullptr;
+            c->gbits_words = 0;
+            if (hipMalloc(&c->gbits, words * sizeof(uint32_t)) != hipSuccess) return GV_ERR_NOMEM;
+            c->gbits_words = words;
+        }
+        HIPCHK(hipMemsetAsync(c->gbits, 0, c->gbits_words * sizeof(uint32_t), c->stream));
+    }
+    return GV_OK;
+}
+
+int gv_counters_read(gv_ctx_t c, uint64_t* count, uint64_t* bytes, uint32_t* bits,
+                     uint32_t nwords) {
+    if (!c) return GV_ERR_ARG;
+    unsigned long long h[2];
+    HIPCHK(hipMemcpyAsync(h, c->acc, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+    if (bits && nwords) {
+        if (nwords > c->gbits_words) return GV_ERR_ARG;
+        HIPCHK(hipMemcpyAsync(bits, c->gbits, nwords * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                              c->stream));
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (count) *count = h[0];
+    if (bytes) *bytes = h[1];
+    return GV_OK;
+}
+
+int gv_copy_partial_emb(gv_ctx_t c, const uint32_t* src, uint32_t* dst, uint64_t n) {
+    if (!c || (n && (!src || !dst))) return GV_ERR_ARG;
+    if (n == 0 || src == dst) return GV_OK;
+    if (dst < src || dst >= src + n) {
+        HIPCHK(hipMemcpyAsync(dst, src, n * 4, hipMemcpyDeviceToDevice, c->stream));
+        return GV_OK;
+    }
+    // dst overlaps the tail of src: copy in chunks from the end, each chunk
+    // no longer than the shift so no chunk reads what an earlier one wrote
+    const uint64_t shift = (uint64_t)(dst - src);
+    uint64_t end = n;
+    while (end > 0) {
+        uint64_t len = std::min(shift, end);
+        uint64_t beg = end - len;
+        hipLaunchKernelGGL(k_copy_back, dim3(grid_for(len, kBlock, 4096)), dim3(kBlock), 0,
+                           c->stream, src + beg, dst + beg, len);
+        HIPCHK(hipGetLastError());
+        end = beg;
+    }
+    return GV_OK;
+}
+
+int gv_mark(gv_ctx_t c, int slot) {
+    if (!c || slot < 0 || slot >= 8) return GV_ERR_ARG;
+    HIPCHK(hipEventRecord(c->ev[slot], c->stream));
+    return GV_OK;
+}
+
+int gv_elapsed_ms(gv_ctx_t c, int a, int b, float* ms) {
+    if (!c || !ms || a < 0 || a >= 8 || b < 0 || b >= 8) return GV_ERR_ARG;
+    HIPCHK(hipEventSynchronize(c->ev[b]));
+    HIPCHK(hipEventElapsedTime(ms, c->ev[a], c->ev[b]));
+    return GV_OK;
+}
+
+}  // extern "C"

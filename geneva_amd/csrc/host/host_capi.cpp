@@ -1,0 +1,311 @@
+// extern "C" surface of the host library (include/geneva_host.h).
+#include "../../../include/geneva_host.h"
+
+#include <cstring>
+#include <memory>
+#include <sstream>
+#include <string>
+
+#include "../../../include/geneva_hip.h"
+#include "gen.h"
+#include "genMatchOrder.h"
+#include "loadDataBase.h"
+#include "miner.h"
+
+namespace {
+thread_local std::string g_err;
+void setErr(const std::string& s) { g_err = s; }
+}  // namespace
+
+struct gh_dataset {
+    geneva::Dataset ds;
+};
+
+struct gh_engine {
+    gv_ctx_t ctx = nullptr;
+    gv_graph_t graph = nullptr;
+    std::unique_ptr<geneva::PlainMiner> plain;
+    uint32_t ngraphs = 0;
+};
+
+extern "C" {
+
+const char* gh_last_error(void) { return g_err.c_str(); }
+
+gh_dataset_t gh_ds_gen_database(uint32_t ngraphs, uint32_t nmin, uint32_t nmax, double avg_deg,
+                                double ba_fraction, uint32_t ba_m, uint32_t nlabels,
+                                double zipf_s, uint64_t seed) {
+    if (ngraphs == 0 || nmin < 1 || nmax < nmin || nlabels == 0) {
+        setErr("bad database parameters");
+        return nullptr;
+    }
+    gh_dataset* d = new gh_dataset();
+    d->ds = geneva::genDatabase(ngraphs, nmin, nmax, avg_deg, ba_fraction, ba_m, nlabels, zipf_s,
+                                seed);
+    return d;
+}
+
+gh_dataset_t gh_ds_gen_chung_lu(uint32_t nv, double gamma, double mean_deg, double wmax,
+                                uint32_t nlabels, double zipf_s, uint64_t seed) {
+    if (nv < 2 || gamma <= 1.0 || mean_deg <= 0 || nlabels == 0) {
+        setErr("bad Chung-Lu parameters");
+        return nullptr;
+    }
+    gh_dataset* d = new gh_dataset();
+    d->ds = geneva::genChungLu(nv, gamma, mean_deg, wmax, nlabels, zipf_s, seed);
+    return d;
+}
+
+namespace {
+struct DecodeCtx {
+    geneva::DataGraph* g;
+    uint32_t label;
+};
+void collect(void* user, uint32_t src, uint32_t, const uint32_t* nbrs, uint32_t count) {
+    DecodeCtx* c = (DecodeCtx*)user;
+    for (uint32_t i = 0; i < count; ++i)
+        if (src < nbrs[i]) c->g->edges.push_back({src, nbrs[i], c->label});
+}
+}  // namespace
+
+gh_dataset_t gh_ds_load(const char* mygraph, const char* metadata) {
+    std::map<uint, uint*> db = loadDataBase(mygraph, metadata);
+    auto it = db.find(0xffffffffu);
+    if (it == db.end()) {
+        setErr("loadDataBase failed");
+        return nullptr;
+    }
+    gh_dataset* d = new gh_dataset();
+    const uint* vLabel = it->second;
+    geneva::DataGraph& g = d->ds.g;
+    g.nv = vLabel[0];
+    g.vlabel.assign(vLabel, vLabel + g.nv + 1);
+    g.vlabel[0] = 0;
+    for (auto& kv : db) {
+        if (kv.first == 0xffffffffu) continue;
+        DecodeCtx c{&g, kv.first};
+        std::string err;
+        if (!geneva::forEachLabelGroup(kv.second, vLabel, g.nv, collect, &c, &err)) {
+            setErr("partition " + std::to_string(kv.first) + ": " + err);
+            geneva::freeDataBase(db);
+            delete d;
+            return nullptr;
+        }
+    }
+    d->ds.graphOf.assign(g.nv + 1, 0);
+    d->ds.ngraphs = 1;
+    geneva::freeDataBase(db);
+    return d;
+}
+
+gh_dataset_t gh_ds_from_arrays(uint32_t nv, const uint32_t* vlabel, uint64_t m,
+                               const uint32_t* edges, const uint32_t* graph_of,
+                               uint32_t ngraphs) {
+    if (!vlabel || (m && !edges)) {
+        setErr("null arrays");
+        return nullptr;
+    }
+    gh_dataset* d = new gh_dataset();
+    d->ds.g.nv = nv;
+    d->ds.g.vlabel.assign(vlabel, vlabel + nv + 1);
+    for (uint64_t i = 0; i < m; ++i)
+        d->ds.g.edges.push_back({edges[3 * i], edges[3 * i + 1], edges[3 * i + 2]});
+    if (graph_of && ngraphs > 0) {
+        d->ds.graphOf.assign(graph_of, graph_of + nv + 1);
+        d->ds.ngraphs = ngraphs;
+    } else {
+        d->ds.graphOf.assign(nv + 1, 0);
+        d->ds.ngraphs = 1;
+    }
+    return d;
+}
+
+void gh_ds_free(gh_dataset_t ds) { delete ds; }
+
+int gh_ds_add_walk_queries(gh_dataset_t d, uint32_t nq, uint32_t kmin, uint32_t kmax, int induced,
+                           uint32_t min_extra, uint64_t seed) {
+    if (!d || kmin < 2 || kmax < kmin || kmax > GV_MAXK) return GV_ERR_ARG;
+    auto q = geneva::genWalkQueries(d->ds, nq, kmin, kmax, induced != 0, min_extra, seed);
+    d->ds.queries.insert(d->ds.queries.end(), q.begin(), q.end());
+    return GV_OK;
+}
+
+int gh_ds_add_shape_queries(gh_dataset_t d, uint64_t seed) {
+    if (!d) return GV_ERR_ARG;
+    auto q = geneva::genShapeQueries(d->ds, seed);
+    d->ds.queries.insert(d->ds.queries.end(), q.begin(), q.end());
+    return GV_OK;
+}
+
+int gh_ds_add_query(gh_dataset_t d, const char* text) {
+    if (!d || !text) return GV_ERR_ARG;
+    d->ds.queries.emplace_back(text);
+    return GV_OK;
+}
+
+uint32_t gh_ds_nv(gh_dataset_t d) { return d ? d->ds.g.nv : 0; }
+const uint32_t* gh_ds_vlabel(gh_dataset_t d) { return d ? d->ds.g.vlabel.data() : nullptr; }
+uint64_t gh_ds_nedges(gh_dataset_t d) { return d ? d->ds.g.edges.size() : 0; }
+const uint32_t* gh_ds_edges(gh_dataset_t d) {
+    static_assert(sizeof(geneva::Edge) == 12, "Edge must be three packed uint32");
+    return d ? reinterpret_cast<const uint32_t*>(d->ds.g.edges.data()) : nullptr;
+}
+const uint32_t* gh_ds_graph_of(gh_dataset_t d) { return d ? d->ds.graphOf.data() : nullptr; }
+uint32_t gh_ds_ngraphs(gh_dataset_t d) { return d ? d->ds.ngraphs : 0; }
+uint32_t gh_ds_nqueries(gh_dataset_t d) { return d ? (uint32_t)d->ds.queries.size() : 0; }
+const char* gh_ds_query(gh_dataset_t d, uint32_t i) {
+    return (d && i < d->ds.queries.size()) ? d->ds.queries[i].c_str() : nullptr;
+}
+
+int gh_ds_write(gh_dataset_t d, const char* prefix, int skip_index) {
+    if (!d || !prefix) return GV_ERR_ARG;
+    std::string p(prefix), err;
+    if (!geneva::writeGraphText(p + ".g", d->ds.g) || !geneva::writeMetadata(p + ".metadata", d->ds.g) ||
+        !geneva::writeMyGraph(p + ".mygraph", d->ds.g, &err) ||
+        !geneva::writeReorder(p + ".reorder", d->ds.g, skip_index != 0)) {
+        setErr("write failed: " + err);
+        return GV_ERR_ARG;
+    }
+    for (size_t i = 0; i < d->ds.queries.size(); ++i) {
+        FILE* f = fopen((p + "_" + std::to_string(i) + ".query").c_str(), "w");
+        if (!f) return GV_ERR_ARG;
+        fputs(d->ds.queries[i].c_str(), f);
+        fclose(f);
+    }
+    return GV_OK;
+}
+
+uint64_t gh_plan_dump(const char* text, int single_v, char* out, uint64_t cap) {
+    if (!text) return 0;
+    std::map<uint64_t, uint> rf;
+    std::map<uint, uint> restricts, ql;
+    std::map<uint, std::map<uint, bool>> rls;
+    std::ostringstream o;
+    try {
+        std::istringstream in(text);
+        std::vector<std::pair<char, std::pair<uint, std::vector<uint>>>> phases;
+        if (single_v) {
+            std::vector<PhaseSingleV> mo;
+            geneva::planSV(in, false, mo, rf, restricts, rls, ql, nullptr, nullptr, nullptr, nullptr);
+            for (auto& p : mo) phases.push_back({p.phaseType, {p.edgeLabel, p.pairs}});
+        } else {
+            std::vector<Phase> mo;
+            geneva::planDV(in, false, mo, rf, restricts, rls, ql, nullptr, nullptr, nullptr, nullptr);
+            for (auto& p : mo) phases.push_back({p.phaseType, {p.edgeLabel, p.pairs}});
+        }
+        for (auto& kv : restricts) o << "restrict: " << kv.first << "<" << kv.second << "\n";
+        for (size_t i = 0; i < phases.size(); ++i) {
+            o << "phase " << i << " type " << phases[i].first << " edge label:" << phases[i].second.first
+              << "\n";
+            const auto& pr = phases[i].second.second;
+            for (size_t j = 0; j + 1 < pr.size(); j += 2) o << "svid:" << pr[j] << " evid:" << pr[j + 1] << " ";
+            o << "\n";
+        }
+    } catch (const geneva::PlanError& e) {
+        setErr(e.what);
+        return 0;
+    }
+    std::string s = o.str();
+    if (out && cap) {
+        uint64_t n = std::min<uint64_t>(s.size(), cap - 1);
+        memcpy(out, s.data(), n);
+        out[n] = 0;
+    }
+    return s.size();
+}
+
+int gh_dv_phase_count(const char* text) {
+    if (!text) return GV_ERR_ARG;
+    std::map<uint64_t, uint> rf;
+    std::map<uint, uint> restricts, ql;
+    std::map<uint, std::map<uint, bool>> rls;
+    std::vector<Phase> mo;
+    try {
+        std::istringstream in(text);
+        geneva::planDV(in, true, mo, rf, restricts, rls, ql, nullptr, nullptr, nullptr, nullptr);
+    } catch (const geneva::PlanError& e) {
+        setErr(e.what);
+        return GV_ERR_ARG;
+    }
+    return calDVPhaseCount(mo);
+}
+
+gh_engine_t gh_engine_create(gh_dataset_t d, int device, uint64_t level_bytes, int* status) {
+    auto fail = [&](int rc, const std::string& msg) -> gh_engine_t {
+        if (status) *status = rc;
+        setErr(msg);
+        return nullptr;
+    };
+    if (!d) return fail(GV_ERR_ARG, "null data set");
+    std::unique_ptr<gh_engine> e(new gh_engine());
+    int rc = gv_ctx_create(device, &e->ctx);
+    if (rc != GV_OK) return fail(rc, std::string("gv_ctx_create: ") + gv_status_string(rc));
+    std::map<uint, uint> keyOfLabel;
+    std::string err;
+    std::map<uint, uint*> db = geneva::buildDataBase(d->ds.g, &keyOfLabel, &err);
+    if (db.empty()) {
+        gv_ctx_destroy(e->ctx);
+        return fail(GV_ERR_FORMAT, "buildDataBase: " + err);
+    }
+    std::vector<const uint32_t*> blocks;
+    std::vector<uint32_t> labels;
+    for (auto& kv : keyOfLabel) {
+        labels.push_back(kv.first);
+        blocks.push_back(db.at(kv.second));
+    }
+    rc = gv_graph_create(e->ctx, db.at(0xffffffffu), (uint32_t)blocks.size(), blocks.data(),
+                         labels.data(), &e->graph);
+    geneva::freeDataBase(db);
+    if (rc != GV_OK) {
+        gv_ctx_destroy(e->ctx);
+        return fail(rc, std::string("gv_graph_create: ") + gv_status_string(rc) + " " + gv_last_error());
+    }
+    if (d->ds.ngraphs > 1) {
+        rc = gv_graph_set_owner(e->graph, d->ds.graphOf.data(), d->ds.ngraphs);
+        if (rc != GV_OK) {
+            gv_graph_destroy(e->graph);
+            gv_ctx_destroy(e->ctx);
+            return fail(rc, "gv_graph_set_owner failed");
+        }
+        e->ngraphs = d->ds.ngraphs;
+    }
+    e->plain.reset(new geneva::PlainMiner(e->ctx, e->graph, level_bytes ? level_bytes : (1ull << 30)));
+    if (status) *status = GV_OK;
+    return e.release();
+}
+
+void gh_engine_free(gh_engine_t e) {
+    if (!e) return;
+    e->plain.reset();
+    if (e->graph) gv_graph_destroy(e->graph);
+    if (e->ctx) gv_ctx_destroy(e->ctx);
+    delete e;
+}
+
+uint64_t gh_engine_graph_bytes(gh_engine_t e) {
+    uint64_t b = 0;
+    if (e && e->graph) gv_graph_info(e->graph, nullptr, nullptr, nullptr, nullptr, &b);
+    return b;
+}
+
+int gh_engine_mine(gh_engine_t e, const char* text, int single_v, uint64_t* count,
+                   uint64_t* alg_bytes, double* ms, uint32_t* bits, uint32_t nwords,
+                   uint64_t* launches) {
+    if (!e || !text) return GV_ERR_ARG;
+    geneva::MineResult r;
+    int rc = e->plain->mineText(text, single_v != 0, r);
+    if (rc != GV_OK) {
+        setErr(r.error.empty() ? std::string(gv_status_string(rc)) : r.error);
+        return rc;
+    }
+    if (count) *count = r.count;
+    if (alg_bytes) *alg_bytes = r.algBytes;
+    if (ms) *ms = r.ms;
+    if (launches) *launches = r.launches;
+    if (bits && nwords) {
+        for (uint32_t i = 0; i < nwords; ++i) bits[i] = i < r.graphBits.size() ? r.graphBits[i] : 0;
+    }
+    return GV_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,281 @@
+// Plain mining path over the C ABI.
+#include "miner.h"
+
+#include <algorithm>
+#include <sstream>
+
+namespace geneva {
+
+bool parseQueryPattern(const std::string& text, QueryPattern& q, std::string* err) {
+    std::istringstream in(text);
+    std::string line;
+    std::vector<std::pair<uint32_t, uint32_t>> vs;
+    std::vector<uint32_t> es;
+    while (std::getline(in, line)) {
+        if (line.empty()) continue;
+        std::istringstream ls(line.substr(1));
+        if (line[0] == 'v') {
+            uint32_t id, lab;
+            if (ls >> id >> lab) vs.emplace_back(id, lab);
+        } else if (line[0] == 'e') {
+            uint32_t a, b, l;
+            if (ls >> a >> b >> l) {
+                es.push_back(a);
+                es.push_back(b);
+                es.push_back(l);
+            }
+        }
+    }
+    q = QueryPattern();
+    q.vnum = (uint32_t)vs.size();
+    if (q.vnum < 2 || q.vnum > GV_MAXK) {
+        if (err) *err = "query must have 2.." + std::to_string(GV_MAXK) + " vertices";
+        return false;
+    }
+    q.labels.assign(q.vnum, 0);
+    std::vector<bool> seen(q.vnum, false);
+    for (auto& p : vs) {
+        if (p.first >= q.vnum || seen[p.first]) {
+            if (err) *err = "query vertex ids must be 0..n-1, each once";
+            return false;
+        }
+        seen[p.first] = true;
+        q.labels[p.first] = p.second;
+    }
+    q.adj.assign(q.vnum * q.vnum, 0xffffffffu);
+    for (size_t i = 0; i < es.size(); i += 3) {
+        uint32_t a = es[i], b = es[i + 1], l = es[i + 2];
+        if (a >= q.vnum || b >= q.vnum || a == b) {
+            if (err) *err = "bad query edge";
+            return false;
+        }
+        if (q.adj[a * q.vnum + b] == 0xffffffffu) q.adj[a * q.vnum + b] = q.adj[b * q.vnum + a] = l;
+    }
+    return true;
+}
+
+bool compilePlan(const std::vector<Phase>& mo, const std::map<uint, uint>& restricts,
+                 const std::map<uint, uint>& queryLabel, const QueryPattern& qp, gv_graph_t g,
+                 CompiledQuery& out, std::string* err) {
+    out = CompiledQuery();
+    const uint32_t n = qp.vnum;
+    std::vector<uint32_t> col(n, 0xffffffffu);
+    std::vector<uint32_t> covered(n * n, 0);
+    for (auto& ph : mo) {
+        if (ph.phaseType == 'E' && ph.pairs.size() == 4) out.dvPhases++;
+        out.nphases++;
+        for (size_t i = 0; i + 1 < ph.pairs.size(); i += 2) {
+            uint32_t s = ph.pairs[i], x = ph.pairs[i + 1];
+            if (s >= n || x >= n) {
+                if (err) *err = "plan names a vertex outside the query";
+                return false;
+            }
+            covered[s * n + x]++;
+            covered[x * n + s]++;
+            if (ph.phaseType != 'E') continue;
+            if (out.order.empty()) {
+                col[s] = 0;
+                out.order.push_back(s);
+            }
+            if (col[x] == 0xffffffffu) {
+                col[x] = (uint32_t)out.order.size();
+                out.order.push_back(x);
+            }
+        }
+    }
+    if (out.order.size() != n) {
+        if (err) *err = "plan does not place every query vertex";
+        return false;
+    }
+    for (uint32_t a = 0; a < n; ++a)
+        for (uint32_t b = 0; b < n; ++b) {
+            bool edge = qp.adj[a * n + b] != 0xffffffffu;
+            if ((edge && covered[a * n + b] != 1) || (!edge && covered[a * n + b] != 0)) {
+                if (err) *err = "plan does not cover every query edge exactly once";
+                return false;
+            }
+        }
+    auto dense = [&](uint32_t label, uint32_t& d) { return gv_graph_label_index(g, label, &d) == GV_OK; };
+    gv_query_t& q = out.q;
+    q.k = n;
+    if (!dense(queryLabel.at(out.order[0]), q.label0)) out.empty = true;
+    for (uint32_t t = 1; t < n; ++t) {
+        const uint32_t u = out.order[t];
+        gv_step_t& s = q.step[t];
+        s.col = t;
+        s.restrict_col = GV_NONE;
+        if (!dense(queryLabel.at(u), s.label)) out.empty = true;
+        for (uint32_t w = 0; w < n; ++w) {
+            uint32_t el = qp.adj[u * n + w];
+            if (el == 0xffffffffu || col[w] >= t) continue;
+            uint32_t part = 0;
+            if (gv_graph_part_index(g, el, &part) != GV_OK) out.empty = true;
+            s.back_col[s.nback] = (uint8_t)col[w];
+            s.back_part[s.nback] = (uint8_t)part;
+            s.nback++;
+        }
+        if (s.nback == 0) {
+            if (err) *err = "vertex without an earlier neighbour in the plan";
+            return false;
+        }
+        auto it = restricts.find(u);
+        if (it != restricts.end() && it->second != 0xffffffffu) {
+            if (it->second >= n || col[it->second] >= t) {
+                if (err) *err = "restriction refers to a later vertex";
+                return false;
+            }
+            s.restrict_col = (uint8_t)col[it->second];
+        }
+        for (uint32_t j = 0; j < t; ++j)
+            if (queryLabel.at(out.order[j]) == queryLabel.at(u)) s.coll_col[s.ncoll++] = (uint8_t)j;
+    }
+    return true;
+}
+
+bool planAndCompile(const std::string& text, bool singleV, gv_graph_t g, CompiledQuery& out,
+                    std::string* err) {
+    QueryPattern qp;
+    if (!parseQueryPattern(text, qp, err)) return false;
+    std::map<uint64_t, uint> repeatFlag;
+    std::map<uint, uint> restricts, queryLabel;
+    std::map<uint, std::map<uint, bool>> rls;
+    try {
+        std::istringstream in(text);
+        if (singleV) {
+            std::vector<PhaseSingleV> mo;
+            planSV(in, true, mo, repeatFlag, restricts, rls, queryLabel, nullptr, nullptr, nullptr,
+                   nullptr);
+            std::vector<Phase> asPhases;
+            for (auto& p : mo) {
+                Phase q;
+                q.phaseType = p.phaseType;
+                q.edgeLabel = p.edgeLabel;
+                q.pairs = p.pairs;
+                q.reducePos = p.reducePos;
+                asPhases.push_back(q);
+            }
+            return compilePlan(asPhases, restricts, queryLabel, qp, g, out, err);
+        }
+        std::vector<Phase> mo;
+        planDV(in, true, mo, repeatFlag, restricts, rls, queryLabel, nullptr, nullptr, nullptr,
+               nullptr);
+        return compilePlan(mo, restricts, queryLabel, qp, g, out, err);
+    } catch (const PlanError& e) {
+        if (err) *err = e.what;
+        return false;
+    }
+}
+
+PlainMiner::PlainMiner(gv_ctx_t ctx, gv_graph_t g, uint64_t levelBytes)
+    : ctx_(ctx), g_(g), levelBytes_(levelBytes) {}
+
+PlainMiner::~PlainMiner() {
+    for (auto& l : lv_) {
+        if (l.rows) gv_free(ctx_, l.rows);
+        if (l.fr) gv_frontier_destroy(l.fr);
+    }
+}
+
+int PlainMiner::ensureFrontier(uint32_t width, uint64_t n) {
+    Level& l = lv_[width];
+    if (l.fr && l.frCap >= n) return GV_OK;
+    if (l.fr) gv_frontier_destroy(l.fr);
+    l.fr = nullptr;
+    l.frCap = 0;
+    uint64_t cap = std::max<uint64_t>(n, 1u << 16);
+    int rc = gv_frontier_create(ctx_, cap, &l.fr);
+    if (rc == GV_OK) l.frCap = cap;
+    return rc;
+}
+
+int PlainMiner::ensureRows(uint32_t width) {
+    Level& l = lv_[width];
+    if (l.rows) return GV_OK;
+    uint64_t cap = levelBytes_ / (4ull * width + 17ull);
+    if (cap < 1024) cap = 1024;
+    int rc = gv_malloc(ctx_, cap * 4ull * width, (void**)&l.rows);
+    if (rc != GV_OK) return rc;
+    l.cap = cap;
+    return ensureFrontier(width, cap);
+}
+
+int PlainMiner::run(const gv_query_t& q, uint32_t width, const uint32_t* rows, uint64_t n,
+                    MineResult& out) {
+    if (n == 0) return GV_OK;
+    const gv_step_t& st = q.step[width];
+    int rc = ensureFrontier(width, n);
+    if (rc != GV_OK) return rc;
+    gv_frontier_t fr = lv_[width].fr;
+    uint64_t total = 0;
+    rc = gv_size_rows(ctx_, g_, &st, rows, width, n, fr, &total);
+    if (rc != GV_OK || total == 0) return rc;
+    if (width + 1 == q.k) {
+        out.launches++;
+        return gv_count_emb(ctx_, g_, &st, nullptr, rows, width, n, fr, 0, total);
+    }
+    if (width + 2 == q.k && q.step[width + 1].nback == 1) {
+        out.launches++;
+        return gv_count_emb(ctx_, g_, &st, &q.step[width + 1], rows, width, n, fr, 0, total);
+    }
+    rc = ensureRows(width + 1);
+    if (rc != GV_OK) return rc;
+    Level& nx = lv_[width + 1];
+    for (uint64_t c0 = 0; c0 < total; c0 += nx.cap) {
+        uint64_t c1 = std::min(total, c0 + nx.cap);
+        uint64_t written = 0;
+        out.launches++;
+        out.chunks++;
+        rc = gv_ext_emb(ctx_, g_, &st, rows, width, n, fr, c0, c1, nx.rows, nx.cap, &written);
+        if (rc != GV_OK) return rc;
+        out.peakRows = std::max(out.peakRows, written);
+        rc = run(q, width + 1, nx.rows, written, out);
+        if (rc != GV_OK) return rc;
+    }
+    return GV_OK;
+}
+
+int PlainMiner::mine(const CompiledQuery& cq, MineResult& out) {
+    out = MineResult();
+    uint64_t nv1 = 0;
+    gv_graph_info(g_, &nv1, nullptr, nullptr, nullptr, nullptr);
+    int rc = gv_counters_reset(ctx_, g_);
+    if (rc != GV_OK) return out.status = rc;
+    gv_mark(ctx_, 0);
+    if (!cq.empty) {
+        const uint32_t* rows = nullptr;
+        uint64_t n = 0;
+        rc = gv_graph_start_rows(g_, cq.q.label0, &rows, &n);
+        if (rc == GV_OK) rc = run(cq.q, 1, rows, n, out);
+        if (rc != GV_OK) {
+            out.status = rc;
+            out.error = gv_last_error();
+            return rc;
+        }
+    }
+    gv_mark(ctx_, 1);
+    float ms = 0;
+    gv_elapsed_ms(ctx_, 0, 1, &ms);
+    out.ms = ms;
+    uint32_t ngraphs = 0;
+    gv_graph_owners(g_, &ngraphs);
+    const uint32_t words = (ngraphs + 31) / 32;
+    out.graphBits.assign(words, 0);
+    rc = gv_counters_read(ctx_, &out.count, &out.algBytes, words ? out.graphBits.data() : nullptr,
+                          words);
+    out.status = rc;
+    return rc;
+}
+
+int PlainMiner::mineText(const std::string& text, bool singleV, MineResult& out) {
+    CompiledQuery cq;
+    std::string err;
+    if (!planAndCompile(text, singleV, g_, cq, &err)) {
+        out = MineResult();
+        out.status = GV_ERR_ARG;
+        out.error = err;
+        return GV_ERR_ARG;
+    }
+    return mine(cq, out);
+}
+
+}  // namespace geneva

@@ -1,0 +1,172 @@
+/*
+ * geneva_hip.h -- C ABI of the gfx950 embedding-extension library
+ * (libgeneva_hip.so).
+ *
+ * This layer replaces the reference's kernel launchers
+ * (include/miningPattern.h:1-31, include/reduction.h:1-4, implemented in
+ * src/miningPattern.cu:3350-3752 and src/reduction.cu:266-296) and the CUDA
+ * runtime calls its host driver makes (src/gpumine.cpp:868-1072).  Every
+ * argument is a plain pointer, size or POD struct; every entry point returns a
+ * status (0 = GV_OK) instead of void, and never aborts the process.
+ *
+ * Model.  A partial embedding table is row-major uint32[nrows][row_len]; column
+ * j holds the data vertex of the j-th query vertex in match order (data ids
+ * start at 1, as in the reference).  One extension step adds column row_len.
+ * The step descriptor (gv_step_t) carries everything the reference spreads
+ * over auxArray, restriction flags and the following reduction phases: the
+ * label of the new vertex, every earlier query neighbour (its column and edge
+ * label partition), the restriction column and the same-label columns that
+ * enforce injectivity.  The back edges the reference verifies in separate 'R'
+ * phases (src/reduction.cu:115-264) are checked during the extension as
+ * sorted-set membership, so no unverified row is ever written.
+ *
+ * Threading: a context owns one HIP stream on one device; calls on one context
+ * must not overlap.  Calls on different contexts may run concurrently.
+ */
+#ifndef GENEVA_HIP_H
+#define GENEVA_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GV_OK 0
+#define GV_ERR_ARG -1       /* invalid argument */
+#define GV_ERR_HIP -2       /* HIP runtime error (see gv_last_error) */
+#define GV_ERR_NOMEM -3     /* device allocation failed */
+#define GV_ERR_CAPACITY -4  /* output would exceed the caller's capacity */
+#define GV_ERR_FORMAT -5    /* malformed partition block */
+#define GV_ERR_NODEV -6     /* no HIP device */
+
+#define GV_MAXK 16          /* query vertices per pattern */
+#define GV_NONE 0xffu
+
+typedef struct gv_ctx* gv_ctx_t;
+typedef struct gv_graph* gv_graph_t;
+typedef struct gv_frontier* gv_frontier_t;
+
+/* One extension step: the vertex placed in column `col` (= input row_len). */
+typedef struct {
+    uint32_t label;              /* dense vertex label of the new vertex */
+    uint32_t col;                /* column it occupies */
+    uint8_t nback;               /* earlier query neighbours, 1..GV_MAXK-1 */
+    uint8_t back_col[GV_MAXK];   /* their columns */
+    uint8_t back_part[GV_MAXK];  /* edge-label partition of each back edge */
+    uint8_t restrict_col;        /* new vertex id < row[restrict_col]; GV_NONE = none */
+    uint8_t ncoll;               /* earlier columns holding the same vertex label */
+    uint8_t coll_col[GV_MAXK];
+} gv_step_t;
+
+/* A compiled query: k vertices in match order; step[t] adds column t. */
+typedef struct {
+    uint32_t k;
+    uint32_t label0;             /* dense label of column 0 */
+    gv_step_t step[GV_MAXK];
+} gv_query_t;
+
+/* ---- errors --------------------------------------------------------------- */
+const char* gv_status_string(int status);
+/* Text of the last HIP error seen on this thread ("" if none). */
+const char* gv_last_error(void);
+
+/* ---- context and device memory (src/gpumine.cpp:868,885-906,952) ---------- */
+int gv_ctx_create(int device, gv_ctx_t* out);
+int gv_ctx_destroy(gv_ctx_t ctx);
+int gv_ctx_synchronize(gv_ctx_t ctx);
+int gv_device_count(int* n);
+int gv_mem_get_info(gv_ctx_t ctx, uint64_t* free_bytes, uint64_t* total_bytes);
+int gv_malloc(gv_ctx_t ctx, uint64_t bytes, void** dptr);
+int gv_free(gv_ctx_t ctx, void* dptr);
+/* Stream-ordered copies on the context's stream; *_d2h waits for completion.
+ * Replace cudaMemcpyAsync in the planner upload (src/genMatchOrder.cpp:768)
+ * and the counter read-backs (src/gpumine.cpp:395,574,807,952). */
+int gv_memcpy_h2d(gv_ctx_t ctx, void* dst_dev, const void* src_host, uint64_t bytes);
+int gv_memcpy_d2h(gv_ctx_t ctx, void* dst_host, const void* src_dev, uint64_t bytes);
+
+/* ---- resident data graph -----------------------------------------------------
+ * Built from exactly what loadDataBase returns (src/loadDataBase.cpp:309-365):
+ * vLabel (vLabel[0] = max id) and the edge-label partition blocks in the
+ * .mygraph layout (tools/genDatabase.cpp:437-580; skip-index prefixes are
+ * recognised and dropped).  block_edge_labels[i] names the edge label of
+ * blocks[i].  The device copy is a label-segmented CSR: for partition p,
+ * vertex v and dense label l, the neighbours of v with label l are
+ * nbr[seg(p,v,l) .. seg(p,v,l+1)), sorted ascending.  This replaces the
+ * interval lookup and FIND_LABEL_LIMIT search (include/common.h:27-75). */
+int gv_graph_create(gv_ctx_t ctx, const uint32_t* vLabel, uint32_t nblocks,
+                    const uint32_t* const* blocks, const uint32_t* block_edge_labels,
+                    gv_graph_t* out);
+int gv_graph_destroy(gv_graph_t g);
+/* Optional owner table for a database of graphs: graph_of[v] for v in 0..maxid. */
+int gv_graph_set_owner(gv_graph_t g, const uint32_t* graph_of, uint32_t ngraphs);
+/* Number of owner graphs (0 when no owner table is set). */
+int gv_graph_owners(gv_graph_t g, uint32_t* ngraphs);
+/* Dense index of an original vertex label / partition of an edge label;
+ * GV_ERR_ARG when the graph has none. */
+int gv_graph_label_index(gv_graph_t g, uint32_t label, uint32_t* dense);
+int gv_graph_part_index(gv_graph_t g, uint32_t edge_label, uint32_t* part);
+/* Sizes: vertices (max id + 1), dense labels, partitions, neighbour entries,
+ * device bytes held. */
+int gv_graph_info(gv_graph_t g, uint64_t* nv1, uint32_t* nlabels, uint32_t* nparts,
+                  uint64_t* nnbr, uint64_t* dev_bytes);
+/* The one-column start table of a label: every vertex carrying it, ascending. */
+int gv_graph_start_rows(gv_graph_t g, uint32_t dense_label, const uint32_t** rows_dev,
+                        uint64_t* nrows);
+
+/* ---- frontier sizing --------------------------------------------------------
+ * A frontier holds, for up to `capacity` input rows, the chosen anchor list
+ * (the shortest back-neighbour list), its length after the restriction cut,
+ * and the inclusive prefix sum of those lengths: the candidate index space
+ * [0, total) of one extension step. */
+int gv_frontier_create(gv_ctx_t ctx, uint64_t capacity, gv_frontier_t* out);
+int gv_frontier_destroy(gv_frontier_t f);
+
+/* Size the extension of `rows` by `step` (replaces the per-row label search
+ * and restriction cut of src/miningPattern.cu:3249-3324).  *total receives
+ * the number of candidates (stream-synchronising read). */
+int gv_size_rows(gv_ctx_t ctx, gv_graph_t g, const gv_step_t* step, const uint32_t* rows_dev,
+                 uint32_t row_len, uint64_t nrows, gv_frontier_t f, uint64_t* total);
+
+/* Extension (replaces initEmb_2V_NoHash, initEmb_3V_1and2_NoHash, extEmb_1V_NoHash,
+ * the four extEmb_2V_* launchers and the reductionPhase that follows them;
+ * include/miningPattern.h:1-31): candidates [cand_begin, cand_end) of the
+ * sized frontier are checked against every back edge, the restriction and
+ * injectivity, and survivors are appended to new_rows_dev as rows of
+ * row_len+1 uints.  Fails with GV_ERR_CAPACITY (nothing lost: the call can be
+ * repeated on a smaller range) when more than max_rows would be written; since
+ * survivors never exceed candidates, cand_end-cand_begin <= max_rows always
+ * succeeds.  *rows_written is read back before returning. */
+int gv_ext_emb(gv_ctx_t ctx, gv_graph_t g, const gv_step_t* step, const uint32_t* rows_dev,
+               uint32_t row_len, uint64_t nrows, gv_frontier_t f, uint64_t cand_begin,
+               uint64_t cand_end, uint32_t* new_rows_dev, uint64_t max_rows,
+               uint64_t* rows_written);
+
+/* Count-only final phase (the isLastPhase branch of the reference launchers):
+ * candidates [cand_begin, cand_end) that satisfy `step` are counted into the
+ * context's uint64 accumulator; when `tail` is non-null each survivor is
+ * instead extended by the single-back-edge step `tail` and the number of valid
+ * tail vertices is counted without enumerating them.  Matched graph ids (owner
+ * of column 0) are OR-ed into the context's bitset.  Asynchronous. */
+int gv_count_emb(gv_ctx_t ctx, gv_graph_t g, const gv_step_t* step, const gv_step_t* tail,
+                 const uint32_t* rows_dev, uint32_t row_len, uint64_t nrows, gv_frontier_t f,
+                 uint64_t cand_begin, uint64_t cand_end);
+
+/* Accumulators of a context: embedding count, algorithmic bytes and matched
+ * graph bitset (ngraphs bits).  reset zeroes them; read synchronises. */
+int gv_counters_reset(gv_ctx_t ctx, gv_graph_t g);
+int gv_counters_read(gv_ctx_t ctx, uint64_t* count, uint64_t* alg_bytes,
+                     uint32_t* graph_bits, uint32_t nwords);
+
+/* Overlap-safe device copy of n uints (src/reduction.cu:280-296). */
+int gv_copy_partial_emb(gv_ctx_t ctx, const uint32_t* src_dev, uint32_t* dst_dev, uint64_t n);
+
+/* Timing on the context's stream (HIP events): record a mark, and the
+ * milliseconds between two marks (synchronises on the second). */
+int gv_mark(gv_ctx_t ctx, int slot);
+int gv_elapsed_ms(gv_ctx_t ctx, int slot_a, int slot_b, float* ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GENEVA_HIP_H */

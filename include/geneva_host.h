@@ -1,0 +1,79 @@
+/*
+ * geneva_host.h -- C ABI of the host library (libgeneva_host.so): data sets,
+ * the database loader, the match-order planner and the mining drivers built
+ * on libgeneva_hip.so.  Python binds it with ctypes (geneva_amd/__init__.py).
+ *
+ * Handles are opaque.  Functions returning int give 0 on success and a
+ * negative GV_* status otherwise; gh_last_error() describes the last failure
+ * on the calling thread.
+ */
+#ifndef GENEVA_HOST_H
+#define GENEVA_HOST_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct gh_dataset* gh_dataset_t;
+typedef struct gh_engine* gh_engine_t;
+
+const char* gh_last_error(void);
+
+/* ---- data sets -------------------------------------------------------------
+ * A data set is a labelled graph (vertex ids 1..nv), an optional owner graph
+ * per vertex (databases of graphs) and a list of query texts. */
+gh_dataset_t gh_ds_gen_database(uint32_t ngraphs, uint32_t nmin, uint32_t nmax, double avg_deg,
+                                double ba_fraction, uint32_t ba_m, uint32_t nlabels,
+                                double zipf_s, uint64_t seed);
+gh_dataset_t gh_ds_gen_chung_lu(uint32_t nv, double gamma, double mean_deg, double wmax,
+                                uint32_t nlabels, double zipf_s, uint64_t seed);
+/* Load through loadDataBase (src/loadDataBase.cpp:309-365); the edge label of
+ * the k-th stored partition is k. */
+gh_dataset_t gh_ds_load(const char* mygraph_path, const char* metadata_path);
+/* Graph from arrays: vlabel[0..nv] (index 0 unused), m edges (a, b, label). */
+gh_dataset_t gh_ds_from_arrays(uint32_t nv, const uint32_t* vlabel, uint64_t m,
+                               const uint32_t* edges, const uint32_t* graph_of,
+                               uint32_t ngraphs);
+void gh_ds_free(gh_dataset_t ds);
+int gh_ds_add_walk_queries(gh_dataset_t ds, uint32_t nq, uint32_t kmin, uint32_t kmax,
+                           int induced, uint32_t min_extra, uint64_t seed);
+int gh_ds_add_shape_queries(gh_dataset_t ds, uint64_t seed);
+int gh_ds_add_query(gh_dataset_t ds, const char* text);
+uint32_t gh_ds_nv(gh_dataset_t ds);
+const uint32_t* gh_ds_vlabel(gh_dataset_t ds);          /* nv+1 entries */
+uint64_t gh_ds_nedges(gh_dataset_t ds);
+const uint32_t* gh_ds_edges(gh_dataset_t ds);           /* 3 per edge: a, b, label */
+const uint32_t* gh_ds_graph_of(gh_dataset_t ds);        /* nv+1 entries */
+uint32_t gh_ds_ngraphs(gh_dataset_t ds);
+uint32_t gh_ds_nqueries(gh_dataset_t ds);
+const char* gh_ds_query(gh_dataset_t ds, uint32_t i);
+/* Write prefix.g, prefix.metadata, prefix.mygraph, prefix.reorder (prefix-free
+ * groups unless skip_index) and prefix_<i>.query. */
+int gh_ds_write(gh_dataset_t ds, const char* prefix, int skip_index);
+
+/* ---- planner ---------------------------------------------------------------
+ * Plan a query with genMatchOrder (single_v = 0) or genMatchOrderSingleV and
+ * print it in the layout the reference cpumine prints
+ * (src/cpumine.cpp:661-674).  Returns the text length (0 on error). */
+uint64_t gh_plan_dump(const char* query_text, int single_v, char* out, uint64_t cap);
+/* Two-vertex extension phases of the DV plan (tools/classifyExtPhase.cpp:27-38);
+ * negative on error. */
+int gh_dv_phase_count(const char* query_text);
+
+/* ---- engine: resident graph + plain path -------------------------------- */
+gh_engine_t gh_engine_create(gh_dataset_t ds, int device, uint64_t level_bytes, int* status);
+void gh_engine_free(gh_engine_t e);
+/* Device bytes of the resident graph. */
+uint64_t gh_engine_graph_bytes(gh_engine_t e);
+/* Mine one query on the plain path (one query per launch sequence).
+ * graph_bits (nullable) receives ceil(ngraphs/32) words. */
+int gh_engine_mine(gh_engine_t e, const char* query_text, int single_v, uint64_t* count,
+                   uint64_t* alg_bytes, double* device_ms, uint32_t* graph_bits,
+                   uint32_t nwords, uint64_t* launches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GENEVA_HOST_H */

@@ -1,0 +1,191 @@
+"""Write the golden fixtures under tests/golden/ (run in the build container).
+
+    python tests/golden/make_golden.py            # plans, workload 1, Enron, scaled workloads
+    python tests/golden/make_golden.py --full2    # + full-size workload 2 counts (long)
+
+Reference values come from the reference's own cpumine, compiled from its
+sources into oracle/_ref/cpumine (`make -C oracle ref`), run on files written
+by this project's format writers (prefix-free .reorder); oracle
+values come from oracle/liboracle.so.  Only data is written: query texts,
+plan dumps, counts, matched graph ids and algorithmic byte totals.
+"""
+import gzip
+import json
+import os
+import re
+import shutil
+import subprocess
+import sys
+import tempfile
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+import geneva_amd as G  # noqa: E402
+import oracle as O  # noqa: E402
+from geneva_amd import configs  # noqa: E402
+
+TOT = re.compile(r"write to final result cpu, tot=(\d+)")
+
+
+def ref_run(prefix, query_text, timeout=600):
+    """Run the reference cpumine; return (plan dump, count or None)."""
+    wd = tempfile.mkdtemp()
+    qf = os.path.join(wd, "q.query")
+    with open(qf, "w") as f:
+        f.write(query_text)
+    try:
+        p = subprocess.run([O.REF_CPUMINE, prefix + ".reorder", qf, prefix + ".mygraph",
+                            prefix + ".metadata"], cwd=wd, capture_output=True, text=True,
+                           timeout=timeout)
+        out = p.stdout
+    except subprocess.TimeoutExpired as e:
+        out = e.stdout.decode() if isinstance(e.stdout, bytes) else (e.stdout or "")
+    finally:
+        shutil.rmtree(wd, ignore_errors=True)
+    lines = out.splitlines()
+    plan = [l for l in lines if l.startswith(("restrict: ", "phase ", "svid:"))]
+    m = TOT.search(out)
+    return "\n".join(plan), (int(m.group(1)) if m else None)
+
+
+def write(ds, d, name):
+    prefix = os.path.join(d, name)
+    ds.write(prefix)
+    return prefix
+
+
+def enron():
+    d = tempfile.mkdtemp()
+    for f in ("Enron.mygraph", "Enron.metadata"):
+        with gzip.open(os.path.join(HERE, "enron", f + ".gz"), "rb") as s, open(os.path.join(d, f), "wb") as t:
+            shutil.copyfileobj(s, t)
+    return G.Dataset.load(os.path.join(d, "Enron.mygraph"), os.path.join(d, "Enron.metadata"))
+
+
+def plan_corpus():
+    qs = []
+    c1 = configs.workload1()
+    qs += c1.queries
+    c1.add_walk_queries(40, 3, 8, seed=101)
+    qs += c1.queries[10:]
+    c3 = configs.workload3(scale=0.01, nqueries=30)
+    c3.add_walk_queries(30, 4, 9, induced=True, seed=102)
+    qs += c3.queries
+    c2 = configs.workload2(scale=0.01)
+    qs += c2.queries
+    en = enron()
+    en.add_walk_queries(40, 3, 8, seed=103)
+    en.add_walk_queries(20, 4, 7, induced=True, seed=104)
+    qs += en.queries
+    with open(os.path.join(HERE, "enron", "Enron_4_6_0.query")) as f:
+        qs.append(f.read())
+    # symmetric and multi-edge-label patterns for the automorphism and
+    # triangle-score branches of the planner
+    def q(labels, edges):
+        return "".join(f"v {i} {l}\n" for i, l in enumerate(labels)) + \
+               "".join(f"e {a} {b} {l}\n" for a, b, l in edges)
+    for k in range(3, 8):
+        qs.append(q([1] * k, [(i, i + 1, 1) for i in range(k - 1)]))                  # path
+        qs.append(q([1] * k, [(0, i, 1) for i in range(1, k)]))                       # star
+        qs.append(q([1] * k, [(i, (i + 1) % k, 1) for i in range(k)]))                # cycle
+        qs.append(q([1] * k, [(a, b, 1) for a in range(k) for b in range(a + 1, k)]))  # clique
+    for la in ([1, 1, 1], [1, 1, 2], [1, 2, 1], [2, 1, 1], [1, 2, 3]):
+        for el in ([1, 1, 1], [1, 1, 2], [1, 2, 1], [2, 1, 1], [1, 2, 3]):
+            qs.append(q(la + [la[0]], [(0, 1, el[0]), (1, 2, el[1]), (0, 2, el[2]), (2, 3, 1)]))
+    qs.append(q([1, 2, 1, 2, 1, 2], [(0, 1, 1), (1, 2, 1), (2, 3, 1), (3, 4, 1), (4, 5, 1), (5, 0, 1)]))
+    qs.append(q([1, 1, 2, 2], [(0, 2, 1), (1, 2, 1), (0, 3, 1), (1, 3, 1)]))
+    qs.append(q([3, 1, 1, 1, 2], [(0, 1, 2), (0, 2, 2), (0, 3, 2), (3, 4, 3), (1, 2, 1)]))
+    return qs
+
+
+def main():
+    full2 = "--full2" in sys.argv
+    if not os.path.exists(O.REF_CPUMINE):
+        sys.exit("oracle/_ref/cpumine missing: make -C oracle ref")
+    d = tempfile.mkdtemp()
+    # a small graph carrying edge labels 1..3 so cpumine can plan any corpus query
+    small = G.Dataset.from_arrays([0] + [1, 2, 3, 1, 2, 3, 1, 2],
+                                  [(1, 2, 1), (2, 3, 2), (3, 4, 3), (4, 5, 1), (5, 6, 2), (6, 7, 3), (7, 8, 1)])
+    sp = write(small, d, "small")
+    cases = []
+    seen = set()
+    for qt in plan_corpus():
+        if qt in seen:
+            continue
+        seen.add(qt)
+        plan, _ = ref_run(sp, qt, timeout=60)
+        cases.append({"query": qt, "ref_plan": plan})
+    json.dump(cases, open(os.path.join(HERE, "plans.json"), "w"), indent=0)
+    print("plans:", len(cases))
+
+    # workload 1 at full size: reference counts, oracle counts/graphs/bytes
+    c1 = configs.workload1()
+    p1 = write(c1, d, "w1")
+    og = O.OracleGraph.of(c1)
+    out = []
+    for qt in c1.queries:
+        _, ref = ref_run(p1, qt)
+        r = og.mine(qt)
+        out.append({"query": qt, "ref_count": ref, "count": r.count,
+                    "graph_ids": sorted(r.graph_ids), "alg_bytes": r.alg_bytes})
+    json.dump({"workload": 1, "cases": out}, open(os.path.join(HERE, "workload1.json"), "w"), indent=1)
+    print("workload1:", [c["count"] for c in out])
+
+    # Enron: the shipped query plus walk queries (edge labels 1..3)
+    en = enron()
+    pe = write(en, d, "enron")
+    with open(os.path.join(HERE, "enron", "Enron_4_6_0.query")) as f:
+        eq = [f.read()]
+    en.add_walk_queries(12, 3, 5, seed=201)
+    eq += en.queries
+    oge = O.OracleGraph.of(en)
+    out = []
+    for qt in eq:
+        _, ref = ref_run(pe, qt, timeout=900)
+        r = oge.mine(qt)
+        if ref is None:
+            print("reference timed out on", qt.replace("\n", " | "))
+            continue
+        out.append({"query": qt, "ref_count": ref, "count": r.count, "alg_bytes": r.alg_bytes})
+        assert ref == r.count, (ref, r.count, qt)
+    json.dump({"cases": out}, open(os.path.join(HERE, "enron.json"), "w"), indent=1)
+    print("enron:", [c["count"] for c in out])
+
+    # scaled-down workloads 2-4 (oracle; reference where cheap)
+    wl = []
+    for w, scale, kw in ((2, 0.01, {}), (3, 0.01, {"nqueries": 48}), (4, 0.005, {"nqueries": 24})):
+        ds = configs.WORKLOADS[w](scale=scale, **kw)
+        ogw = O.OracleGraph.of(ds)
+        pw = write(ds, d, f"w{w}")
+        cases = []
+        for qt in ds.queries:
+            r = ogw.mine(qt, threads=8)
+            c = {"query": qt, "count": r.count, "alg_bytes": r.alg_bytes,
+                 "graph_ids": sorted(r.graph_ids)}
+            if r.count < 2_000_000 and w == 3:
+                _, ref = ref_run(pw, qt, timeout=120)
+                if ref is not None:
+                    c["ref_count"] = ref
+                    assert ref == r.count, (ref, r.count, qt)
+            cases.append(c)
+        wl.append({"workload": w, "scale": scale, "kw": kw, "cases": cases})
+        print(f"workload{w}@{scale}:", [c["count"] for c in cases][:12])
+    json.dump({"workloads": wl}, open(os.path.join(HERE, "scaled.json"), "w"), indent=1)
+
+    if full2:
+        ds = configs.workload2()
+        ogw = O.OracleGraph.of(ds)
+        cases = []
+        for qt in ds.queries:
+            r = ogw.mine(qt, threads=os.cpu_count())
+            cases.append({"query": qt, "count": r.count, "alg_bytes": r.alg_bytes,
+                          "oracle_seconds": r.seconds, "threads": os.cpu_count()})
+            print("workload2 full:", r.count, r.seconds, flush=True)
+        json.dump({"workload": 2, "scale": 1.0, "cases": cases},
+                  open(os.path.join(HERE, "workload2_full.json"), "w"), indent=1)
+
+
+if __name__ == "__main__":
+    main()

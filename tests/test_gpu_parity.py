@@ -1,0 +1,129 @@
+"""GPU parity: the HIP path (through libgeneva_host -> libgeneva_hip) against
+the oracle and the golden fixtures.  Counts, matched graph ids and the
+algorithmic byte totals must be identical (integer work, no tolerance)."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import geneva_amd as G
+from geneva_amd import configs
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+pytestmark = pytest.mark.gpu
+
+
+def _load(name):
+    with open(os.path.join(GOLDEN, name)) as f:
+        return json.load(f)
+
+
+@pytest.fixture(scope="module")
+def w1(require_gpu):
+    ds = configs.workload1()
+    return ds, G.Engine(ds)
+
+
+def test_workload1_matches_reference_counts(w1):
+    ds, eng = w1
+    gold = _load("workload1.json")
+    for c in gold["cases"]:
+        print("workload1 query", c["query"].replace("\n", " "), flush=True)
+        for sv in (False, True):
+            r = eng.mine(c["query"], single_v=sv)
+            assert r.count == c["ref_count"]
+            assert sorted(r.graph_ids) == c["graph_ids"]
+            assert r.alg_bytes == c["alg_bytes"]
+
+
+def test_enron_multi_edge_label(require_gpu):
+    import gzip
+    import shutil
+    import tempfile
+    d = tempfile.mkdtemp()
+    for f in ("Enron.mygraph", "Enron.metadata"):
+        with gzip.open(os.path.join(GOLDEN, "enron", f + ".gz"), "rb") as s, \
+                open(os.path.join(d, f), "wb") as t:
+            shutil.copyfileobj(s, t)
+    ds = G.Dataset.load(os.path.join(d, "Enron.mygraph"), os.path.join(d, "Enron.metadata"))
+    eng = G.Engine(ds)
+    for c in _load("enron.json")["cases"]:
+        print("enron query", c["query"].replace("\n", " "), flush=True)
+        r = eng.mine(c["query"])
+        assert r.count == c["ref_count"]
+        assert r.alg_bytes == c["alg_bytes"]
+
+
+def test_scaled_workloads(require_gpu):
+    for w in _load("scaled.json")["workloads"]:
+        ds = configs.WORKLOADS[w["workload"]](scale=w["scale"], **w["kw"])
+        eng = G.Engine(ds)
+        for c in w["cases"]:
+            print("workload", w["workload"], "query", c["query"].replace("\n", " "), flush=True)
+            r = eng.mine(c["query"])
+            assert r.count == c["count"]
+            assert r.alg_bytes == c["alg_bytes"]
+            if ds.ngraphs > 1:
+                assert sorted(r.graph_ids) == c["graph_ids"]
+        eng.close()
+
+
+def test_random_databases_against_live_oracle(require_gpu):
+    import oracle as O
+    for seed in range(4):
+        ds = G.Dataset.database(300, 5, 40, 3.0 + seed, 0.5, 3 + seed, 1.0 * (seed % 2), 1000 + seed)
+        ds.add_walk_queries(10, 3, 7, seed=seed)
+        ds.add_walk_queries(5, 4, 7, induced=True, seed=seed + 50)
+        og = O.OracleGraph.of(ds)
+        eng = G.Engine(ds)
+        for q in ds.queries:
+            print("random db", seed, "query", q.replace("\n", " "), flush=True)
+            want = og.mine(q)
+            got = eng.mine(q)
+            assert got.count == want.count
+            assert got.graph_ids == want.graph_ids
+            assert got.alg_bytes == want.alg_bytes
+        eng.close()
+
+
+def test_hub_graph_and_chunked_frontiers(require_gpu):
+    """A hub with ~20k neighbours (one label group > 4096, skip index in the
+    stored block) and a tiny level budget that forces many extension chunks:
+    counts must not depend on the chunking."""
+    import oracle as O
+    rng = np.random.default_rng(7)
+    n = 30000
+    vl = np.concatenate([[0], rng.integers(1, 4, size=n)]).astype(np.uint32)
+    hub = [(1, v, 1) for v in range(2, 20001)]
+    rest = rng.integers(2, n + 1, size=(40000, 2))
+    rest = [(int(a), int(b), 1) for a, b in rest if a != b]
+    ds = G.Dataset.from_arrays(vl, np.array(hub + rest, dtype=np.uint32))
+    qs = ["v 0 1\nv 1 2\nv 2 2\ne 0 1 1\ne 0 2 1\n",
+          "v 0 1\nv 1 1\nv 2 2\nv 3 3\ne 0 1 1\ne 0 2 1\ne 1 2 1\ne 2 3 1\n",
+          "v 0 2\nv 1 1\nv 2 2\nv 3 3\ne 0 1 1\ne 1 2 1\ne 2 3 1\ne 3 0 1\n",
+          "v 0 1\nv 1 2\nv 2 3\nv 3 2\nv 4 1\ne 0 1 1\ne 0 2 1\ne 0 3 1\ne 3 4 1\n"]
+    og = O.OracleGraph.of(ds)
+    big = G.Engine(ds)
+    small = G.Engine(ds, level_bytes=1 << 16)
+    for q in qs:
+        print("hub query", q.replace("\n", " "), flush=True)
+        want = og.mine(q)
+        a, b = big.mine(q), small.mine(q)
+        assert a.count == b.count == want.count
+        assert a.alg_bytes == b.alg_bytes == want.alg_bytes
+        assert b.launches >= a.launches
+
+
+def test_absent_labels_give_zero(w1):
+    ds, eng = w1
+    assert eng.mine("v 0 99\nv 1 1\ne 0 1 1\n").count == 0
+    assert eng.mine("v 0 1\nv 1 2\ne 0 1 7\n").count == 0
+
+
+def test_bad_query_raises(w1):
+    ds, eng = w1
+    with pytest.raises(G.GenevaError):
+        eng.mine("v 0 1\nv 1 2\n")          # no edges
+    with pytest.raises(G.GenevaError):
+        eng.mine("v 0 1\nv 1 2\nv 2 3\ne 0 1 1\n")  # disconnected
