@@ -66,6 +66,16 @@ def lib():
         "gh_engine_graph_bytes": (u64, [vp]),
         "gh_engine_mine": (i32, [vp, cs, i32, C.POINTER(u64), C.POINTER(u64), C.POINTER(dbl),
                                  _u32p(), u32, C.POINTER(u64)]),
+        "gh_engine_batch_config": (i32, [vp, u32, u64, u64, u32, u32]),
+        "gh_engine_batch_mine": (i32, [vp, C.POINTER(cs), u32, i32, C.POINTER(u64), C.POINTER(u64),
+                                       _u32p(), u32, C.POINTER(dbl)]),
+        "gh_engine_autotune": (i32, [vp, C.POINTER(cs), u32, C.POINTER(u32), C.POINTER(u32),
+                                     C.POINTER(dbl)]),
+        "gh_engine_serve": (i32, [vp, C.POINTER(cs), C.POINTER(dbl), u32, i32, u64, C.POINTER(dbl),
+                                  C.POINTER(u64), C.POINTER(dbl)]),
+        "gh_engine_profile": (i32, [vp, i32]),
+        "gh_engine_profile_reset": (i32, [vp]),
+        "gh_engine_profile_read": (i32, [vp, i32, C.POINTER(u64), C.POINTER(dbl), C.POINTER(u64)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -224,6 +234,78 @@ class Engine:
     @property
     def graph_bytes(self):
         return lib().gh_engine_graph_bytes(self._h)
+
+    KERNELS = {0: "k_size", 1: "k_scan", 2: "k_walk<extend>", 3: "k_walk<count>",
+               4: "batch_iteration"}
+
+    def batch_config(self, slots=0, slot_bytes=0, row_cap=0, ipt=0, blocks_per_cu=0):
+        """Configure the batch engine (0 keeps a setting); rebuilt on next use."""
+        rc = lib().gh_engine_batch_config(self._h, slots, slot_bytes, row_cap, ipt, blocks_per_cu)
+        if rc:
+            raise _err("batch_config", rc)
+
+    @staticmethod
+    def _cstrs(queries):
+        arr = (C.c_char_p * len(queries))(*[q.encode() for q in queries])
+        return arr
+
+    def mine_batch(self, queries, single_v=False):
+        """Mine all queries together on the batch engine.  Returns (results, stats)."""
+        n = len(queries)
+        counts = (C.c_uint64 * max(n, 1))()
+        ab = (C.c_uint64 * max(n, 1))()
+        nwords = (self.ngraphs + 31) // 32 if self.ngraphs > 1 else 0
+        bits = (C.c_uint32 * max(n * nwords, 1))()
+        st = (C.c_double * 5)()
+        rc = lib().gh_engine_batch_mine(self._h, self._cstrs(queries), n, int(single_v), counts, ab,
+                                        bits, nwords, st)
+        if rc:
+            raise _err("mine_batch", rc)
+        res = []
+        for i in range(n):
+            w = list(bits)[i * nwords:(i + 1) * nwords]
+            res.append(MineResult(counts[i], ab[i], 0.0, _bits_to_ids(w, self.ngraphs), 0))
+        stats = dict(zip(("batched", "overflowed", "graph_launches", "device_ms", "wall_ms"), list(st)))
+        return res, stats
+
+    def autotune(self, queries):
+        ipt, bpc, ms = C.c_uint32(0), C.c_uint32(0), C.c_double(0)
+        rc = lib().gh_engine_autotune(self._h, self._cstrs(queries), len(queries), C.byref(ipt),
+                                      C.byref(bpc), C.byref(ms))
+        if rc:
+            raise _err("autotune", rc)
+        return {"ipt": ipt.value, "blocks_per_cu": bpc.value, "device_ms": ms.value}
+
+    def serve(self, queries, arrival_s, batched=True, mem_budget=16 << 30):
+        """Serve requests arriving at arrival_s (seconds).  Returns (latency_ms, counts, stats)."""
+        n = len(queries)
+        arr = (C.c_double * max(n, 1))(*arrival_s)
+        lat = (C.c_double * max(n, 1))()
+        cnt = (C.c_uint64 * max(n, 1))()
+        st = (C.c_double * 4)()
+        rc = lib().gh_engine_serve(self._h, self._cstrs(queries), arr, n, 0 if batched else 1,
+                                   mem_budget, lat, cnt, st)
+        if rc:
+            raise _err("serve", rc)
+        stats = dict(zip(("wall_s", "iterations", "max_concurrent", "overflowed"), list(st)))
+        return list(lat)[:n], list(cnt)[:n], stats
+
+    def profile(self, on=True):
+        rc = lib().gh_engine_profile(self._h, int(on))
+        if rc:
+            raise _err("profile", rc)
+        lib().gh_engine_profile_reset(self._h)
+
+    def profile_read(self):
+        """{kernel: (launches, total device ms, requested bytes)} since the last reset."""
+        out = {}
+        for k, name in self.KERNELS.items():
+            n, ms, rq = C.c_uint64(0), C.c_double(0), C.c_uint64(0)
+            rc = lib().gh_engine_profile_read(self._h, k, C.byref(n), C.byref(ms), C.byref(rq))
+            if rc:
+                raise _err("profile_read", rc)
+            out[name] = (n.value, ms.value, rq.value)
+        return out
 
     def mine(self, query_text, single_v=False):
         count, ab, launches = C.c_uint64(0), C.c_uint64(0), C.c_uint64(0)

@@ -16,7 +16,7 @@ import sys
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 PKG = os.path.join(ROOT, "geneva_amd")
 LIB = os.path.join(PKG, "lib")
-HIP_SRC = [os.path.join(PKG, "csrc", "hip", "geneva_hip.hip")]
+HIP_SRC = [os.path.join(PKG, "csrc", "hip", f) for f in ("geneva_hip.hip", "batch.hip")]
 HOST_SRC = [os.path.join(PKG, "csrc", "host", f) for f in
             ("genMatchOrder.cpp", "loadDataBase.cpp", "gen.cpp", "miner.cpp",
              "batch.cpp", "serve.cpp", "host_capi.cpp")]

@@ -1,0 +1,702 @@
+// Batched execution: many queries in flight, one captured graph per iteration.
+//
+// Device state per slot is split in a descriptor (the compiled query, written
+// at admission) and a compact dynamic record (polled by the host).  One
+// iteration is the kernel sequence
+//   k_b_prep   one workgroup: which slots run, their mode (extend / count /
+//              count with a single-edge tail), their offset in the iteration's
+//              row space, deferral when the row space is full;
+//   k_b_size   per row of every running slot: shortest back-neighbour list,
+//              restriction cut, candidate count;
+//   k_b_scan_* grid-level scan of the counts (size read from device memory);
+//   k_b_walk   load-balanced walk over the candidates of all slots at once
+//              (a wave's tile may straddle slots; per-slot groups are formed
+//              with ballots), writing each slot's next table or counting;
+//   k_b_post   one workgroup: advance, finish or flag overflow.
+// Every kernel reads its extent from device memory, so the sequence is
+// captured once and replayed without host synchronisation.
+#include "common.h"
+
+namespace {
+
+constexpr uint32_t kMaxSlots = 1024;
+
+struct BDesc {               // written by the host at admission
+    gv_query_t q;
+    uint32_t start_off;      // offset of label0's start rows in lverts
+    uint32_t start_n;
+};
+
+struct BState {              // dynamic, polled by the host (mirrors gv_slot_info_t)
+    uint64_t tag;
+    uint32_t state;
+    uint32_t width;          // columns of the current input table
+    uint64_t rows;           // rows of the current input table
+    unsigned long long count;
+    unsigned long long bytes;
+    // internal
+    uint32_t sel;            // half holding the input table
+    uint32_t mode;           // this iteration: 0 idle, 1 extend, 2 count, 3 count + tail
+    unsigned long long out_rows;
+    unsigned long long overflow;
+    uint64_t row_end;        // end of this slot's rows in the iteration row space
+};
+
+struct BArgs {
+    DevGraph g;
+    const uint32_t* lverts;
+    const BDesc* desc;
+    BState* st;
+    uint32_t nslots;
+    uint32_t* half0;
+    uint32_t* half1;
+    uint64_t slot_uints;
+    uint32_t* anc_beg;
+    uint32_t* cnt;
+    uint8_t* anc_i;
+    uint64_t* csum;
+    uint64_t row_cap;
+    uint64_t* tiles;
+    unsigned long long* glob;   // [0] rows, [1] candidates
+    uint32_t* gbits;            // nslots * gwords
+    uint32_t gwords;
+};
+
+__device__ __forceinline__ const uint32_t* slot_rows(const BArgs& a, uint32_t s, const BState& S,
+                                                     const BDesc& D) {
+    if (S.width == 1) return a.lverts + D.start_off;
+    return (S.sel ? a.half1 : a.half0) + (uint64_t)s * a.slot_uints;
+}
+
+__device__ __forceinline__ uint32_t* slot_out(const BArgs& a, uint32_t s, const BState& S) {
+    return (S.sel ? a.half0 : a.half1) + (uint64_t)s * a.slot_uints;
+}
+
+// slot owning global row r: first s with row_end[s] > r (row_end in LDS)
+__device__ __forceinline__ uint32_t slot_of(const uint64_t* row_end, uint32_t n, uint64_t r) {
+    uint32_t lo = 0;
+    while (n > 0) {
+        uint32_t h = n >> 1;
+        if (row_end[lo + h] <= r) {
+            lo += h + 1;
+            n -= h + 1;
+        } else {
+            n = h;
+        }
+    }
+    return lo;
+}
+
+// Block-wide exclusive scan over up to 1024 values held one per thread.
+__device__ uint64_t block_scan_1024(uint64_t v, uint64_t* wsum, uint64_t& total) {
+    const unsigned lane = __lane_id(), w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    uint64_t x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        uint64_t y = __shfl_up(x, o, 64);
+        if (lane >= (unsigned)o) x += y;
+    }
+    if (lane == 63) wsum[w] = x;
+    __syncthreads();
+    uint64_t off = 0;
+    total = 0;
+    for (unsigned i = 0; i < nw; ++i) {
+        if (i < w) off += wsum[i];
+        total += wsum[i];
+    }
+    __syncthreads();
+    return off + x - v;
+}
+
+__global__ __launch_bounds__(1024) void k_b_prep(BArgs a) {
+    __shared__ uint64_t wsum[16];
+    const uint32_t s = threadIdx.x;
+    uint64_t rows = 0;
+    uint32_t mode = 0;
+    if (s < a.nslots) {
+        BState& S = a.st[s];
+        const BDesc& D = a.desc[s];
+        S.out_rows = 0;
+        if (S.state == GV_SLOT_RUNNING) {
+            const uint64_t n = S.width == 1 ? D.start_n : S.rows;
+            if (n == 0 || S.width >= D.q.k) {
+                S.state = GV_SLOT_DONE;
+            } else {
+                const bool last = S.width + 1 == D.q.k;
+                const bool tail = S.width + 2 == D.q.k && D.q.step[S.width + 1].nback == 1;
+                mode = last ? 2 : (tail ? 3 : 1);
+                rows = n;
+                if (n > a.row_cap) {  // can never fit one iteration
+                    S.state = GV_SLOT_OVERFLOW;
+                    mode = 0;
+                    rows = 0;
+                }
+            }
+        }
+    }
+    // first pass: running prefix; slots whose rows end past row_cap wait
+    uint64_t tot;
+    uint64_t ex = block_scan_1024(rows, wsum, tot);
+    if (ex + rows > a.row_cap) {
+        rows = 0;
+        mode = 0;
+    }
+    ex = block_scan_1024(rows, wsum, tot);
+    if (s < a.nslots) {
+        a.st[s].mode = mode;
+        a.st[s].row_end = ex + rows;
+    }
+    if (s == 0) {
+        a.glob[0] = tot;
+        a.glob[1] = 0;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_b_size(BArgs a) {
+    __shared__ uint64_t row_end[kMaxSlots];
+    for (uint32_t i = threadIdx.x; i < a.nslots; i += blockDim.x) row_end[i] = a.st[i].row_end;
+    __syncthreads();
+    const uint64_t nrows = a.glob[0];
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    uint32_t cur = 0xffffffffu;
+    uint64_t acc = 0;
+    for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r < nrows; r += stride) {
+        const uint32_t s = slot_of(row_end, a.nslots, r);
+        const BState& S = a.st[s];
+        const BDesc& D = a.desc[s];
+        const uint32_t L = S.width;
+        const gv_step_t& st = D.q.step[L];
+        const uint64_t local = r - (row_end[s] - (S.width == 1 ? D.start_n : S.rows));
+        const uint32_t* row = slot_rows(a, s, S, D) + local * L;
+        uint32_t best_b = 0, best_len = 0xffffffffu, best_i = 0;
+        uint64_t by = 4ull * L;
+        for (uint32_t i = 0; i < st.nback; ++i) {
+            uint32_t b, e;
+            seg(a.g, st.back_part[i], row[st.back_col[i]], st.label, b, e);
+            const uint32_t len = e - b;
+            by += 8ull + 4ull * len;
+            if (len < best_len) {
+                best_len = len;
+                best_b = b;
+                best_i = i;
+            }
+        }
+        const uint32_t bound = st.restrict_col != GV_NONE ? row[st.restrict_col] : 0xffffffffu;
+        a.anc_beg[r] = best_b;
+        a.cnt[r] = lower_bound_u32(a.g.nbr, best_b, best_b + best_len, bound) - best_b;
+        a.anc_i[r] = (uint8_t)best_i;
+        if (s != cur) {
+            if (acc) atomicAdd(&a.st[cur].bytes, (unsigned long long)acc);
+            cur = s;
+            acc = 0;
+        }
+        acc += by;
+    }
+    if (acc) atomicAdd(&a.st[cur].bytes, (unsigned long long)acc);
+}
+
+// ---- scan with the extent in device memory --------------------------------
+
+__global__ __launch_bounds__(kBlock) void k_b_scan_reduce(BArgs a) {
+    __shared__ uint64_t wsum[kWavesPerBlock];
+    const uint64_t n = a.glob[0];
+    const uint64_t ntiles = (n + kScanTile - 1) / kScanTile;
+    for (uint64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const uint64_t base = t * kScanTile;
+        uint64_t s = 0;
+        for (int i = 0; i < kScanItems; ++i) {
+            const uint64_t idx = base + (uint64_t)i * kBlock + threadIdx.x;
+            if (idx < n) s += a.cnt[idx];
+        }
+        s = wave_sum_u64(s);
+        if (__lane_id() == 0) wsum[threadIdx.x >> 6] = s;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint64_t x = 0;
+            for (int i = 0; i < kWavesPerBlock; ++i) x += wsum[i];
+            a.tiles[t] = x;
+        }
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_b_scan_tiles(BArgs a) {
+    __shared__ uint64_t wsum[kWavesPerBlock];
+    const uint64_t n = a.glob[0];
+    const uint64_t ntiles = (n + kScanTile - 1) / kScanTile;
+    uint64_t carry = 0;
+    for (uint64_t b = 0; b < ntiles; b += kBlock) {
+        const uint64_t idx = b + threadIdx.x;
+        const uint64_t v = idx < ntiles ? a.tiles[idx] : 0;
+        uint64_t tot;
+        const uint64_t ex = block_scan_1024(v, wsum, tot);
+        if (idx < ntiles) a.tiles[idx] = carry + ex;
+        carry += tot;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) a.glob[1] = carry;
+}
+
+__global__ __launch_bounds__(kBlock) void k_b_scan_final(BArgs a) {
+    __shared__ uint32_t vals[kScanTile];
+    __shared__ uint64_t wsum[kWavesPerBlock];
+    __shared__ uint64_t outv[kScanTile / 2];
+    const uint64_t n = a.glob[0];
+    const uint64_t ntiles = (n + kScanTile - 1) / kScanTile;
+    for (uint64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const uint64_t base = t * kScanTile;
+        for (int i = 0; i < kScanItems; ++i) {
+            const uint64_t idx = base + (uint64_t)i * kBlock + threadIdx.x;
+            vals[i * kBlock + threadIdx.x] = idx < n ? a.cnt[idx] : 0u;
+        }
+        __syncthreads();
+        uint64_t loc[kScanItems];
+        uint64_t s = 0;
+#pragma unroll
+        for (int i = 0; i < kScanItems; ++i) {
+            s += vals[threadIdx.x * kScanItems + i];
+            loc[i] = s;
+        }
+        uint64_t tot;
+        const uint64_t ex = block_scan_1024(s, wsum, tot) + a.tiles[t];
+        for (int half = 0; half < 2; ++half) {
+            __syncthreads();
+            const int lo = half * (kScanTile / 2);
+#pragma unroll
+            for (int i = 0; i < kScanItems; ++i) {
+                const int p = threadIdx.x * kScanItems + i - lo;
+                if (p >= 0 && p < kScanTile / 2) outv[p] = ex + loc[i];
+            }
+            __syncthreads();
+            for (int i = threadIdx.x; i < kScanTile / 2; i += kBlock) {
+                const uint64_t idx = base + lo + i;
+                if (idx < n) a.csum[idx] = outv[i];
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ---- walk -------------------------------------------------------------------
+
+__device__ __forceinline__ bool b_check(const DevGraph& g, const gv_step_t& st, const uint32_t* row,
+                                        uint32_t v, uint32_t skip) {
+    for (uint32_t j = 0; j < st.ncoll; ++j)
+        if (row[st.coll_col[j]] == v) return false;
+    for (uint32_t i = 0; i < st.nback; ++i) {
+        if (i == skip) continue;
+        uint32_t b, e;
+        seg(g, st.back_part[i], row[st.back_col[i]], st.label, b, e);
+        if (!contains_u32(g.nbr, b, e, v)) return false;
+    }
+    return true;
+}
+
+template <int IPT>
+__global__ __launch_bounds__(kBlock) void k_b_walk(BArgs a) {
+    __shared__ uint64_t row_end[kMaxSlots];
+    __shared__ uint32_t stage[kWavesPerBlock][64 * GV_MAXK];
+    for (uint32_t i = threadIdx.x; i < a.nslots; i += blockDim.x) row_end[i] = a.st[i].row_end;
+    __syncthreads();
+    const uint64_t nrows = a.glob[0];
+    const uint64_t total = a.glob[1];
+    const uint64_t tile = 64ull * IPT;
+    const unsigned lane = __lane_id(), wib = threadIdx.x >> 6;
+    const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+    // per-lane accumulators of the slot the lane last worked for, flushed when
+    // the lane moves to another slot and at the end
+    uint32_t cur = 0xffffffffu, last_gid = 0xffffffffu;
+    uint64_t acc_count = 0, acc_bytes = 0;
+    auto switch_to = [&](uint32_t s) {
+        if (s == cur) return;
+        if (cur != 0xffffffffu) {
+            if (acc_count) atomicAdd(&a.st[cur].count, (unsigned long long)acc_count);
+            if (acc_bytes) atomicAdd(&a.st[cur].bytes, (unsigned long long)acc_bytes);
+        }
+        cur = s;
+        acc_count = acc_bytes = 0;
+        last_gid = 0xffffffffu;
+    };
+    for (uint64_t t0 = wave * tile; t0 < total; t0 += nwaves * tile) {
+        const uint64_t t1 = t0 + tile < total ? t0 + tile : total;
+        uint64_t r_lo = 0, r_hi = 0;
+        if (lane == 0) r_lo = find_row(a.csum, 0, nrows, t0);
+        if (lane == 1) r_hi = find_row(a.csum, 0, nrows, t1 - 1);
+        r_lo = __shfl(r_lo, 0, 64);
+        r_hi = __shfl(r_hi, 1, 64);
+        for (int it = 0; it < IPT; ++it) {
+            const uint64_t c = t0 + (uint64_t)it * 64 + lane;
+            const bool valid = c < t1;
+            bool ok = false;
+            uint32_t v = 0, s = 0, mode = 0, W = 0, L = 0;
+            const uint32_t* row = nullptr;
+            const BDesc* D = nullptr;
+            if (valid) {
+                const uint64_t r = find_row(a.csum, r_lo, r_hi + 1, c);
+                s = slot_of(row_end, a.nslots, r);
+                const BState& S = a.st[s];
+                D = &a.desc[s];
+                L = S.width;
+                W = L + 1;
+                mode = S.mode;
+                const uint64_t local = r - (row_end[s] - (L == 1 ? D->start_n : S.rows));
+                row = slot_rows(a, s, S, *D) + local * L;
+                v = a.g.nbr[a.anc_beg[r] + (uint32_t)(c - (a.csum[r] - a.cnt[r]))];
+                ok = b_check(a.g, D->q.step[L], row, v, a.anc_i[r]);
+            }
+            if (valid && mode >= 2) {
+                switch_to(s);
+                uint64_t n = ok ? 1 : 0;
+                if (ok && mode == 3) {
+                    const gv_step_t& tl = D->q.step[L + 1];
+                    const uint32_t bc = tl.back_col[0];
+                    const uint32_t w = bc == L ? v : row[bc];
+                    uint32_t b, e;
+                    seg(a.g, tl.back_part[0], w, tl.label, b, e);
+                    uint32_t bound = 0xffffffffu;
+                    if (tl.restrict_col != GV_NONE) bound = tl.restrict_col == L ? v : row[tl.restrict_col];
+                    const uint32_t cut = lower_bound_u32(a.g.nbr, b, e, bound);
+                    n = cut - b;
+                    for (uint32_t j = 0; j < tl.ncoll; ++j) {
+                        const uint32_t x = tl.coll_col[j] == L ? v : row[tl.coll_col[j]];
+                        if (x < bound && contains_u32(a.g.nbr, b, cut, x)) --n;
+                    }
+                    acc_bytes += 8ull * W + 8ull + 4ull * (e - b);
+                }
+                acc_count += n;
+                if (n && a.g.gof) {
+                    const uint32_t gid = a.g.gof[row[0]];
+                    if (gid != last_gid) {
+                        atomicOr(&a.gbits[(uint64_t)s * a.gwords + (gid >> 5)], 1u << (gid & 31));
+                        last_gid = gid;
+                    }
+                }
+            }
+            // extension: survivors of each slot present in the wave, one group at a time
+            const bool surv = valid && mode == 1 && ok;
+            if (surv) {
+                switch_to(s);
+                acc_bytes += 4ull * W;
+            }
+            uint64_t pend = __ballot(surv);
+            while (pend) {
+                const int leader = __ffsll((long long)pend) - 1;
+                const uint32_t ls = __shfl(s, leader, 64);
+                const uint64_t mask = __ballot(surv && s == ls);
+                pend &= ~mask;
+                const uint32_t n = (uint32_t)__popcll(mask);
+                const uint32_t Wg = __shfl(W, leader, 64);
+                uint64_t base = 0;
+                if (lane == (unsigned)leader) base = atomicAdd(&a.st[ls].out_rows, (unsigned long long)n);
+                base = __shfl(base, leader, 64);
+                const uint64_t cap_rows = a.slot_uints / Wg;
+                if (base + n > cap_rows) {
+                    if (lane == (unsigned)leader) atomicOr(&a.st[ls].overflow, 1ull);
+                    continue;
+                }
+                uint32_t* stg = stage[wib];
+                if ((mask >> lane) & 1ull) {
+                    const uint32_t p = lanes_below(mask) * Wg;
+                    for (uint32_t j = 0; j + 1 < Wg; ++j) stg[p + j] = row[j];
+                    stg[p + Wg - 1] = v;
+                }
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                uint32_t* dst = slot_out(a, ls, a.st[ls]) + base * Wg;
+                for (uint32_t j = lane; j < n * Wg; j += 64) dst[j] = stg[j];
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+            }
+        }
+    }
+    switch_to(0xffffffffu);
+}
+
+__global__ __launch_bounds__(1024) void k_b_post(BArgs a) {
+    const uint32_t s = threadIdx.x;
+    if (s >= a.nslots) return;
+    BState& S = a.st[s];
+    if (S.mode == 0) return;
+    if (S.mode >= 2) {
+        S.state = GV_SLOT_DONE;
+        S.width = a.desc[s].q.k;
+    } else if (S.overflow) {
+        S.state = GV_SLOT_OVERFLOW;
+    } else {
+        S.rows = S.out_rows;
+        S.width += 1;
+        S.sel ^= 1u;
+    }
+    S.mode = 0;
+}
+
+__global__ void k_b_fill(uint32_t* p, uint64_t n) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = 0;
+}
+
+}  // namespace
+
+struct gv_batch {
+    gv_ctx* ctx = nullptr;
+    gv_graph* g = nullptr;
+    uint32_t nslots = 0;
+    uint64_t slot_bytes = 0;
+    uint64_t row_cap = 0;
+    gv_tune_t tune{4, 16};
+    BDesc* d_desc = nullptr;
+    BState* d_st = nullptr;
+    uint32_t* d_half[2] = {nullptr, nullptr};
+    uint32_t* d_anc_beg = nullptr;
+    uint32_t* d_cnt = nullptr;
+    uint8_t* d_anc_i = nullptr;
+    uint64_t* d_csum = nullptr;
+    uint64_t* d_tiles = nullptr;
+    unsigned long long* d_glob = nullptr;
+    uint32_t* d_gbits = nullptr;
+    uint32_t gwords = 0;
+    std::vector<uint8_t> used;          // host view: slot occupied
+    BDesc* h_desc = nullptr;            // pinned staging, one entry per slot
+    BState* h_st = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+namespace {
+
+BArgs bargs(gv_batch* b) {
+    BArgs a;
+    memset(&a, 0, sizeof(a));
+    a.g = dev_view(b->g);
+    a.lverts = b->g->d_lverts;
+    a.desc = b->d_desc;
+    a.st = b->d_st;
+    a.nslots = b->nslots;
+    a.half0 = b->d_half[0];
+    a.half1 = b->d_half[1];
+    a.slot_uints = b->slot_bytes / 4;
+    a.anc_beg = b->d_anc_beg;
+    a.cnt = b->d_cnt;
+    a.anc_i = b->d_anc_i;
+    a.csum = b->d_csum;
+    a.row_cap = b->row_cap;
+    a.tiles = b->d_tiles;
+    a.glob = b->d_glob;
+    a.gbits = b->d_gbits;
+    a.gwords = b->gwords;
+    return a;
+}
+
+int capture(gv_batch* b) {
+    gv_ctx* c = b->ctx;
+    if (b->exec) {
+        (void)hipGraphExecDestroy(b->exec);
+        b->exec = nullptr;
+    }
+    if (b->graph) {
+        (void)hipGraphDestroy(b->graph);
+        b->graph = nullptr;
+    }
+    BArgs a = bargs(b);
+    const int size_blocks = c->num_cu * 8;
+    const int scan_blocks = c->num_cu * 4;
+    const int walk_blocks = c->num_cu * (int)b->tune.walk_blocks_per_cu;
+    HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    hipLaunchKernelGGL(k_b_prep, dim3(1), dim3(1024), 0, c->stream, a);
+    hipLaunchKernelGGL(k_b_size, dim3(size_blocks), dim3(kBlock), 0, c->stream, a);
+    hipLaunchKernelGGL(k_b_scan_reduce, dim3(scan_blocks), dim3(kBlock), 0, c->stream, a);
+    hipLaunchKernelGGL(k_b_scan_tiles, dim3(1), dim3(kBlock), 0, c->stream, a);
+    hipLaunchKernelGGL(k_b_scan_final, dim3(scan_blocks), dim3(kBlock), 0, c->stream, a);
+    switch (b->tune.ipt) {
+        case 1: hipLaunchKernelGGL(k_b_walk<1>, dim3(walk_blocks), dim3(kBlock), 0, c->stream, a); break;
+        case 2: hipLaunchKernelGGL(k_b_walk<2>, dim3(walk_blocks), dim3(kBlock), 0, c->stream, a); break;
+        case 8: hipLaunchKernelGGL(k_b_walk<8>, dim3(walk_blocks), dim3(kBlock), 0, c->stream, a); break;
+        default: hipLaunchKernelGGL(k_b_walk<4>, dim3(walk_blocks), dim3(kBlock), 0, c->stream, a); break;
+    }
+    hipLaunchKernelGGL(k_b_post, dim3(1), dim3(1024), 0, c->stream, a);
+    hipError_t e1 = hipStreamEndCapture(c->stream, &b->graph);
+    if (e1 != hipSuccess) {
+        g_last_error = std::string("capture: ") + hipGetErrorString(e1);
+        return GV_ERR_HIP;
+    }
+    HIPCHK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    return GV_OK;
+}
+
+bool tune_ok(const gv_tune_t* t) {
+    return t && (t->ipt == 1 || t->ipt == 2 || t->ipt == 4 || t->ipt == 8) &&
+           t->walk_blocks_per_cu >= 1 && t->walk_blocks_per_cu <= 64;
+}
+
+}  // namespace
+
+extern "C" {
+
+int gv_batch_create(gv_ctx_t c, gv_graph_t g, uint32_t slots, uint64_t slot_bytes,
+                    uint64_t row_cap, const gv_tune_t* tune, gv_batch_t* out) {
+    if (!c || !g || !out || slots == 0 || slots > kMaxSlots || slot_bytes < 4096 || row_cap == 0)
+        return GV_ERR_ARG;
+    if (tune && !tune_ok(tune)) return GV_ERR_ARG;
+    *out = nullptr;
+    HIPCHK(hipSetDevice(c->device));
+    gv_batch* b = new gv_batch();
+    b->ctx = c;
+    b->g = g;
+    b->nslots = slots;
+    b->slot_bytes = slot_bytes & ~uint64_t(63);
+    b->row_cap = row_cap;
+    if (tune) b->tune = *tune;
+    b->used.assign(slots, 0);
+    b->gwords = g->ngraphs ? (g->ngraphs + 31) / 32 : 1;
+    const uint64_t ntiles = (row_cap + kScanTile - 1) / kScanTile;
+    bool ok = hipMalloc(&b->d_desc, sizeof(BDesc) * slots) == hipSuccess &&
+              hipMalloc(&b->d_st, sizeof(BState) * slots) == hipSuccess &&
+              hipMalloc(&b->d_half[0], b->slot_bytes * slots) == hipSuccess &&
+              hipMalloc(&b->d_half[1], b->slot_bytes * slots) == hipSuccess &&
+              hipMalloc(&b->d_anc_beg, 4 * row_cap) == hipSuccess &&
+              hipMalloc(&b->d_cnt, 4 * row_cap) == hipSuccess &&
+              hipMalloc(&b->d_anc_i, row_cap) == hipSuccess &&
+              hipMalloc(&b->d_csum, 8 * row_cap) == hipSuccess &&
+              hipMalloc(&b->d_tiles, 8 * (ntiles + 1)) == hipSuccess &&
+              hipMalloc(&b->d_glob, 8 * 4) == hipSuccess &&
+              hipMalloc(&b->d_gbits, 4ull * b->gwords * slots) == hipSuccess &&
+              hipHostMalloc((void**)&b->h_desc, sizeof(BDesc) * slots, hipHostMallocDefault) == hipSuccess &&
+              hipHostMalloc((void**)&b->h_st, sizeof(BState) * slots, hipHostMallocDefault) == hipSuccess;
+    if (!ok) {
+        gv_batch_destroy(b);
+        return GV_ERR_NOMEM;
+    }
+    HIPCHK(hipMemset(b->d_desc, 0, sizeof(BDesc) * slots));
+    HIPCHK(hipMemset(b->d_st, 0, sizeof(BState) * slots));
+    HIPCHK(hipMemset(b->d_glob, 0, 8 * 4));
+    HIPCHK(hipMemset(b->d_gbits, 0, 4ull * b->gwords * slots));
+    int rc = capture(b);
+    if (rc != GV_OK) {
+        gv_batch_destroy(b);
+        return rc;
+    }
+    *out = b;
+    return GV_OK;
+}
+
+int gv_batch_destroy(gv_batch_t b) {
+    if (!b) return GV_ERR_ARG;
+    if (b->ctx) {
+        (void)hipSetDevice(b->ctx->device);
+        (void)hipStreamSynchronize(b->ctx->stream);
+    }
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    void* ptrs[] = {b->d_desc, b->d_st, b->d_half[0], b->d_half[1], b->d_anc_beg, b->d_cnt,
+                    b->d_anc_i, b->d_csum, b->d_tiles, b->d_glob, b->d_gbits};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    if (b->h_desc) (void)hipHostFree(b->h_desc);
+    if (b->h_st) (void)hipHostFree(b->h_st);
+    delete b;
+    return GV_OK;
+}
+
+int gv_batch_set_tune(gv_batch_t b, const gv_tune_t* t) {
+    if (!b || !tune_ok(t)) return GV_ERR_ARG;
+    HIPCHK(hipStreamSynchronize(b->ctx->stream));
+    b->tune = *t;
+    return capture(b);
+}
+
+int gv_batch_admit(gv_batch_t b, const gv_query_t* q, uint64_t tag, uint32_t* slot) {
+    if (!b || !q || !slot || q->k < 2 || q->k > GV_MAXK || q->label0 >= b->g->nlabels)
+        return GV_ERR_ARG;
+    for (uint32_t t = 1; t < q->k; ++t) {
+        const gv_step_t& s = q->step[t];
+        if (s.nback < 1 || s.nback >= GV_MAXK || s.label >= b->g->nlabels || s.ncoll >= GV_MAXK)
+            return GV_ERR_ARG;
+        for (int i = 0; i < s.nback; ++i)
+            if (s.back_col[i] >= t || s.back_part[i] >= b->g->nparts) return GV_ERR_ARG;
+        for (int i = 0; i < s.ncoll; ++i)
+            if (s.coll_col[i] >= t) return GV_ERR_ARG;
+        if (s.restrict_col != GV_NONE && s.restrict_col >= t) return GV_ERR_ARG;
+    }
+    uint32_t s = 0;
+    while (s < b->nslots && b->used[s]) ++s;
+    if (s == b->nslots) return GV_ERR_CAPACITY;
+    // a slot is only re-admitted after release, which synchronises, so its
+    // staging entry is never still in flight here
+    BDesc& d = b->h_desc[s];
+    memset(&d, 0, sizeof(d));
+    d.q = *q;
+    d.start_off = (uint32_t)b->g->lvoff[q->label0];
+    d.start_n = (uint32_t)(b->g->lvoff[q->label0 + 1] - b->g->lvoff[q->label0]);
+    BState& st = b->h_st[s];
+    memset(&st, 0, sizeof(st));
+    st.tag = tag;
+    st.state = GV_SLOT_RUNNING;
+    st.width = 1;
+    st.rows = d.start_n;
+    gv_ctx* c = b->ctx;
+    HIPCHK(hipMemcpyAsync(b->d_desc + s, &d, sizeof(d), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(b->d_st + s, &st, sizeof(st), hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL(k_b_fill, dim3(1), dim3(256), 0, c->stream, b->d_gbits + (uint64_t)s * b->gwords,
+                       (uint64_t)b->gwords);
+    HIPCHK(hipGetLastError());
+    b->used[s] = 1;
+    *slot = s;
+    return GV_OK;
+}
+
+int gv_batch_run(gv_batch_t b, uint32_t iterations) {
+    if (!b) return GV_ERR_ARG;
+    for (uint32_t i = 0; i < iterations; ++i) {
+        ProfScope ps(b->ctx, kKindBatch);
+        HIPCHK(hipGraphLaunch(b->exec, b->ctx->stream));
+    }
+    return GV_OK;
+}
+
+int gv_batch_poll(gv_batch_t b, gv_slot_info_t* info) {
+    if (!b || !info) return GV_ERR_ARG;
+    std::vector<BState> h(b->nslots);
+    HIPCHK(hipMemcpyAsync(h.data(), b->d_st, sizeof(BState) * b->nslots, hipMemcpyDeviceToHost,
+                          b->ctx->stream));
+    HIPCHK(hipStreamSynchronize(b->ctx->stream));
+    for (uint32_t s = 0; s < b->nslots; ++s) {
+        info[s].tag = h[s].tag;
+        info[s].state = b->used[s] ? h[s].state : GV_SLOT_FREE;
+        info[s].width = h[s].width;
+        info[s].rows = h[s].rows;
+        info[s].count = h[s].count;
+        info[s].alg_bytes = h[s].bytes;
+    }
+    return GV_OK;
+}
+
+int gv_batch_release(gv_batch_t b, uint32_t s, uint32_t* bits, uint32_t nwords) {
+    if (!b || s >= b->nslots || !b->used[s]) return GV_ERR_ARG;
+    if (bits && nwords) {
+        if (nwords > b->gwords) return GV_ERR_ARG;
+        HIPCHK(hipMemcpyAsync(bits, b->d_gbits + (uint64_t)s * b->gwords, 4ull * nwords,
+                              hipMemcpyDeviceToHost, b->ctx->stream));
+    }
+    BState& st = b->h_st[s];
+    HIPCHK(hipStreamSynchronize(b->ctx->stream));  // staging entry of s no longer in flight
+    memset(&st, 0, sizeof(st));
+    HIPCHK(hipMemcpyAsync(b->d_st + s, &st, sizeof(st), hipMemcpyHostToDevice, b->ctx->stream));
+    HIPCHK(hipStreamSynchronize(b->ctx->stream));
+    b->used[s] = 0;
+    return GV_OK;
+}
+
+int gv_batch_info(gv_batch_t b, uint32_t* slots, uint64_t* slot_bytes, uint32_t* free_slots) {
+    if (!b) return GV_ERR_ARG;
+    if (slots) *slots = b->nslots;
+    if (slot_bytes) *slot_bytes = b->slot_bytes;
+    if (free_slots) {
+        uint32_t n = 0;
+        for (auto u : b->used) n += u ? 0 : 1;
+        *free_slots = n;
+    }
+    return GV_OK;
+}
+
+}  // extern "C"

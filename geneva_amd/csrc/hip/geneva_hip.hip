@@ -19,88 +19,13 @@
 //   4. k_count  : the same walk for the last step, counting instead of
 //                 writing; optionally fused with a last step that has a
 //                 single back edge, whose count is a binary search per row.
-#include <hip/hip_runtime.h>
+#include "common.h"
 
-#include <algorithm>
-#include <cstdio>
-#include <cstring>
-#include <map>
-#include <mutex>
-#include <string>
-#include <vector>
-
-#include "../../../include/geneva_hip.h"
+namespace gvi {
+thread_local std::string g_last_error;
+}
 
 namespace {
-
-thread_local std::string g_last_error;
-
-#define HIPCHK(call)                                                              \
-    do {                                                                          \
-        hipError_t e_ = (call);                                                   \
-        if (e_ != hipSuccess) {                                                   \
-            g_last_error = std::string(#call) + ": " + hipGetErrorString(e_);     \
-            return GV_ERR_HIP;                                                    \
-        }                                                                         \
-    } while (0)
-
-constexpr int kBlock = 256;             // 4 wave64 per workgroup
-constexpr int kWavesPerBlock = kBlock / 64;
-constexpr int kIPT = 4;                 // candidates per lane per tile
-constexpr uint64_t kTile = 64 * kIPT;   // candidates per wave tile
-constexpr int kScanItems = 16;          // per thread in the scan
-constexpr int kScanTile = kBlock * kScanItems;
-
-struct DevGraph {
-    const uint32_t* loff;   // [npart][nv1][nl1]
-    const uint32_t* nbr;
-    const uint32_t* gof;    // owner graph per vertex (nullable)
-    uint64_t part_stride;   // nv1 * nl1
-    uint32_t nl1;           // dense labels + 1
-};
-
-__device__ __forceinline__ void seg(const DevGraph& g, uint32_t part, uint32_t v, uint32_t l,
-                                    uint32_t& b, uint32_t& e) {
-    const uint64_t base = (uint64_t)part * g.part_stride + (uint64_t)v * g.nl1 + l;
-    b = g.loff[base];
-    e = g.loff[base + 1];
-}
-
-// first index in [b, e) with nbr[i] >= x
-__device__ __forceinline__ uint32_t lower_bound_u32(const uint32_t* a, uint32_t b, uint32_t e,
-                                                    uint32_t x) {
-    uint32_t n = e - b;
-    while (n > 0) {
-        uint32_t h = n >> 1;
-        if (a[b + h] < x) {
-            b += h + 1;
-            n -= h + 1;
-        } else {
-            n = h;
-        }
-    }
-    return b;
-}
-
-__device__ __forceinline__ bool contains_u32(const uint32_t* a, uint32_t b, uint32_t e,
-                                             uint32_t x) {
-    uint32_t p = lower_bound_u32(a, b, e, x);
-    return p < e && a[p] == x;
-}
-
-__device__ __forceinline__ unsigned lane_id() { return __lane_id(); }
-
-__device__ __forceinline__ unsigned lanes_below(uint64_t mask) {
-    return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
-                                     __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
-}
-
-__device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
-}
-
 // ---------------------------------------------------------------------------
 // sizing
 struct SizeArgs {
@@ -112,11 +37,12 @@ struct SizeArgs {
     uint32_t* cnt;
     uint8_t* anc_i;
     unsigned long long* bytes;
+    unsigned long long* req;   // bytes this launch reads and writes, each access once
     gv_step_t st;
 };
 
 __global__ __launch_bounds__(kBlock) void k_size(SizeArgs a) {
-    uint64_t local = 0;
+    uint64_t local = 0, rq = 0;
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r < a.nrows; r += stride) {
         const uint32_t* row = a.rows + r * a.L;
@@ -138,9 +64,14 @@ __global__ __launch_bounds__(kBlock) void k_size(SizeArgs a) {
         a.anc_beg[r] = best_b;
         a.cnt[r] = cut;
         a.anc_i[r] = (uint8_t)best_i;
+        rq += 4ull * a.L + 8ull * a.st.nback + 4ull * probes(best_len) + 9ull;
     }
     local = wave_sum_u64(local);
     if (lane_id() == 0 && local) atomicAdd(a.bytes, (unsigned long long)local);
+    if (a.req) {
+        rq = wave_sum_u64(rq);
+        if (lane_id() == 0 && rq) atomicAdd(a.req, (unsigned long long)rq);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -261,35 +192,22 @@ struct WalkArgs {
     unsigned long long* count;
     uint32_t* gbits;
     unsigned long long* bytes;
+    unsigned long long* req;   // bytes this launch reads and writes, each access once
     gv_step_t st;
     gv_step_t tail;
     int has_tail;
 };
 
-// first r in [lo, hi) with csum[r] > c
-__device__ __forceinline__ uint64_t find_row(const uint64_t* csum, uint64_t lo, uint64_t hi,
-                                             uint64_t c) {
-    uint64_t n = hi - lo;
-    while (n > 0) {
-        uint64_t h = n >> 1;
-        if (csum[lo + h] <= c) {
-            lo += h + 1;
-            n -= h + 1;
-        } else {
-            n = h;
-        }
-    }
-    return lo;
-}
 
 __device__ __forceinline__ bool check_candidate(const WalkArgs& a, const uint32_t* row,
-                                                uint32_t v, uint32_t skip) {
+                                                uint32_t v, uint32_t skip, uint64_t& rq) {
     for (uint32_t j = 0; j < a.st.ncoll; ++j)
         if (row[a.st.coll_col[j]] == v) return false;
     for (uint32_t i = 0; i < a.st.nback; ++i) {
         if (i == skip) continue;
         uint32_t b, e;
         seg(a.g, a.st.back_part[i], row[a.st.back_col[i]], a.st.label, b, e);
+        rq += 8ull + 4ull * probes(e - b);
         if (!contains_u32(a.g.nbr, b, e, v)) return false;
     }
     return true;
@@ -303,7 +221,7 @@ __global__ __launch_bounds__(kBlock) void k_walk(WalkArgs a) {
     const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
     const uint32_t W = a.L + 1;
-    uint64_t my_count = 0, my_bytes = 0;
+    uint64_t my_count = 0, my_bytes = 0, rq = 0;
     uint32_t last_gid = 0xffffffffu;
     for (uint64_t t0 = a.c0 + wave * kTile; t0 < a.c1; t0 += nwaves * kTile) {
         const uint64_t t1 = t0 + kTile < a.c1 ? t0 + kTile : a.c1;
@@ -322,8 +240,12 @@ __global__ __launch_bounds__(kBlock) void k_walk(WalkArgs a) {
                 r = find_row(a.csum, r_lo, r_hi + 1, c);
                 const uint64_t first = a.csum[r] - a.cnt[r];
                 v = a.g.nbr[a.anc_beg[r] + (uint32_t)(c - first)];
-                ok = check_candidate(a, a.rows + r * a.L, v, a.anc_i[r]);
+                rq += 4;
+                ok = check_candidate(a, a.rows + r * a.L, v, a.anc_i[r], rq);
             }
+            // the row and its sizing entries count once per run of lanes sharing it
+            const uint64_t prev_r = __shfl_up(r, 1, 64);
+            if (c < t1 && (lane == 0 || prev_r != r)) rq += 4ull * a.L + 17ull;
             const uint32_t* row = a.rows + r * a.L;
             if (kCount) {
                 uint64_t n = ok ? 1 : 0;
@@ -339,9 +261,13 @@ __global__ __launch_bounds__(kBlock) void k_walk(WalkArgs a) {
                         bound = t.restrict_col == a.L ? v : row[t.restrict_col];
                     const uint32_t cut = lower_bound_u32(a.g.nbr, b, e, bound);
                     n = cut - b;
+                    rq += 8ull + 4ull * probes(e - b);
                     for (uint32_t j = 0; j < t.ncoll; ++j) {
                         const uint32_t x = t.coll_col[j] == a.L ? v : row[t.coll_col[j]];
-                        if (x < bound && contains_u32(a.g.nbr, b, cut, x)) --n;
+                        if (x < bound) {
+                            rq += 4ull * probes(cut - b);
+                            if (contains_u32(a.g.nbr, b, cut, x)) --n;
+                        }
                     }
                     my_bytes += 4ull * W + 4ull * W + 8ull + 4ull * (e - b);
                 }
@@ -361,6 +287,7 @@ __global__ __launch_bounds__(kBlock) void k_walk(WalkArgs a) {
                 if (lane == 0) base = atomicAdd(a.out_n, (unsigned long long)n);
                 base = __shfl(base, 0, 64);
                 my_bytes += ok ? 4ull * W : 0;
+                rq += ok ? 4ull * W : 0;
                 if (base + n > a.out_cap) {
                     if (lane == 0) atomicOr(a.overflow, 1ull);
                     continue;
@@ -386,6 +313,10 @@ __global__ __launch_bounds__(kBlock) void k_walk(WalkArgs a) {
     }
     my_bytes = wave_sum_u64(my_bytes);
     if (lane == 0 && my_bytes) atomicAdd(a.bytes, (unsigned long long)my_bytes);
+    if (a.req) {
+        rq = wave_sum_u64(rq);
+        if (lane == 0 && rq) atomicAdd(a.req, (unsigned long long)rq);
+    }
 }
 
 __global__ void k_copy_back(const uint32_t* src, uint32_t* dst, uint64_t n) {
@@ -402,92 +333,7 @@ __global__ void k_copy_back(const uint32_t* src, uint32_t* dst, uint64_t n) {
 
 }  // namespace
 
-// ---------------------------------------------------------------------------
-// host side
-
-struct gv_ctx {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    hipEvent_t ev[8] = {};
-    // accumulators: [0] count, [1] bytes, [2] rows written, [3] overflow
-    unsigned long long* acc = nullptr;
-    uint32_t* gbits = nullptr;
-    uint32_t gbits_words = 0;
-    // scan scratch
-    uint64_t* tiles = nullptr;
-    uint64_t tiles_cap = 0;
-    int num_cu = 256;
-};
-
-struct gv_graph {
-    gv_ctx* ctx = nullptr;
-    uint64_t nv1 = 0;
-    uint32_t nlabels = 0;
-    uint32_t nparts = 0;
-    uint64_t nnbr = 0;
-    uint64_t dev_bytes = 0;
-    std::vector<uint32_t> label_values;       // dense -> original
-    std::vector<uint32_t> part_edge_label;    // partition -> edge label
-    std::vector<uint64_t> lvoff;              // host copy of start-table offsets
-    uint32_t* d_loff = nullptr;
-    uint32_t* d_nbr = nullptr;
-    uint32_t* d_lverts = nullptr;
-    uint32_t* d_gof = nullptr;
-    uint32_t ngraphs = 0;
-};
-
-struct gv_frontier {
-    gv_ctx* ctx = nullptr;
-    uint64_t cap = 0;
-    uint32_t* anc_beg = nullptr;
-    uint32_t* cnt = nullptr;
-    uint8_t* anc_i = nullptr;
-    uint64_t* csum = nullptr;
-    uint64_t nrows = 0;
-    uint64_t total = 0;   // candidates of the last sizing
-};
-
 namespace {
-
-DevGraph dev_view(const gv_graph* g) {
-    DevGraph d;
-    d.loff = g->d_loff;
-    d.nbr = g->d_nbr;
-    d.gof = g->d_gof;
-    d.nl1 = g->nlabels + 1;
-    d.part_stride = g->nv1 * (uint64_t)(g->nlabels + 1);
-    return d;
-}
-
-int grid_for(uint64_t items, uint64_t per_block, int cap_blocks) {
-    uint64_t b = (items + per_block - 1) / per_block;
-    if (b < 1) b = 1;
-    if (b > (uint64_t)cap_blocks) b = cap_blocks;
-    return (int)b;
-}
-
-bool valid_step(const gv_step_t* s, uint32_t row_len, const gv_graph* g) {
-    if (!s || s->nback < 1 || s->nback >= GV_MAXK || s->ncoll >= GV_MAXK) return false;
-    if (s->label >= g->nlabels) return false;
-    if (row_len + 1 > GV_MAXK) return false;
-    for (int i = 0; i < s->nback; ++i)
-        if (s->back_col[i] >= row_len || s->back_part[i] >= g->nparts) return false;
-    for (int i = 0; i < s->ncoll; ++i)
-        if (s->coll_col[i] >= row_len) return false;
-    if (s->restrict_col != GV_NONE && s->restrict_col >= row_len) return false;
-    return true;
-}
-
-// the tail step reads columns 0..row_len (row_len = the fused survivor)
-bool valid_tail(const gv_step_t* s, uint32_t row_len, const gv_graph* g) {
-    if (!s || s->nback != 1 || s->ncoll >= GV_MAXK) return false;
-    if (s->label >= g->nlabels || row_len + 2 > GV_MAXK) return false;
-    if (s->back_col[0] > row_len || s->back_part[0] >= g->nparts) return false;
-    for (int i = 0; i < s->ncoll; ++i)
-        if (s->coll_col[i] > row_len) return false;
-    if (s->restrict_col != GV_NONE && s->restrict_col > row_len) return false;
-    return true;
-}
 
 int scan_counts(gv_ctx* c, const uint32_t* cnt, uint64_t n, uint64_t* csum) {
     const uint64_t ntiles = (n + kScanTile - 1) / kScanTile;
@@ -501,6 +347,8 @@ int scan_counts(gv_ctx* c, const uint32_t* cnt, uint64_t n, uint64_t* csum) {
         }
         c->tiles_cap = cap;
     }
+    ProfScope ps(c, kKindScan);
+    if (c->prof) c->scan_req += 16ull * n + 24ull * ntiles;
     hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)ntiles), dim3(kBlock), 0, c->stream, cnt, n,
                        c->tiles);
     hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(kBlock), 0, c->stream, c->tiles, ntiles);
@@ -569,6 +417,12 @@ int gv_ctx_destroy(gv_ctx_t c) {
     if (c->acc) hipFree(c->acc);
     if (c->gbits) hipFree(c->gbits);
     if (c->tiles) hipFree(c->tiles);
+    if (c->preq) hipFree(c->preq);
+    for (auto& r : c->recs) {
+        hipEventDestroy(r.a);
+        hipEventDestroy(r.b);
+    }
+    for (auto& e : c->pool) hipEventDestroy(e);
     for (auto& e : c->ev)
         if (e) hipEventDestroy(e);
     hipStreamDestroy(c->stream);
@@ -649,6 +503,7 @@ int gv_graph_create(gv_ctx_t c, const uint32_t* vLabel, uint32_t nblocks,
     const uint64_t nl1 = NL + 1;
     std::vector<uint32_t> loff(nv1 * nl1 * (uint64_t)std::max<uint32_t>(nblocks, 1), 0u);
     std::vector<uint32_t> nbr;
+    std::vector<uint64_t> pairs((uint64_t)std::max<uint32_t>(nblocks, 1) * NL * NL, 0);
     for (uint32_t p = 0; p < nblocks; ++p) {
         const uint32_t* blk = blocks[p];
         const uint32_t tot = blk[0], S = blk[1], n = blk[2], u4 = blk[4];
@@ -681,6 +536,8 @@ int gv_graph_create(gv_ctx_t c, const uint32_t* vLabel, uint32_t nblocks,
                     if (L > 130 && L <= 4128) pre = (L + 128) / 129;
                     else if (L > 4128) pre = 32;
                     for (uint32_t l = next_label; l <= lab; ++l) row[l] = (uint32_t)nbr.size();
+                    if (dl[src] != 0xffffffffu)
+                        pairs[((uint64_t)p * NL + dl[src]) * NL + lab] += r - q - pre;
                     for (uint32_t s = q + pre; s < r; ++s) {
                         if (a3[s] == 0 || a3[s] >= nv1) return GV_ERR_FORMAT;
                         nbr.push_back(a3[s]);
@@ -713,6 +570,7 @@ int gv_graph_create(gv_ctx_t c, const uint32_t* vLabel, uint32_t nblocks,
     g->label_values = vals;
     g->part_edge_label.assign(block_edge_labels, block_edge_labels + nblocks);
     g->lvoff = lvoff;
+    g->pair_edges = pairs;
     if (nbr.empty()) nbr.push_back(0);
     if (lverts.empty()) lverts.push_back(0);
     int rc = GV_OK;
@@ -792,6 +650,16 @@ int gv_graph_info(gv_graph_t g, uint64_t* nv1, uint32_t* nl, uint32_t* np, uint6
     return GV_OK;
 }
 
+int gv_graph_label_stats(gv_graph_t g, uint32_t part, uint64_t* vb, uint64_t* eb) {
+    if (!g || part >= std::max<uint32_t>(g->nparts, 1)) return GV_ERR_ARG;
+    const uint32_t NL = g->nlabels;
+    if (vb)
+        for (uint32_t l = 0; l < NL; ++l) vb[l] = g->lvoff[l + 1] - g->lvoff[l];
+    if (eb)
+        for (uint64_t i = 0; i < (uint64_t)NL * NL; ++i) eb[i] = g->pair_edges[(uint64_t)part * NL * NL + i];
+    return GV_OK;
+}
+
 int gv_graph_start_rows(gv_graph_t g, uint32_t l, const uint32_t** rows, uint64_t* n) {
     if (!g || !rows || !n || l >= g->nlabels) return GV_ERR_ARG;
     *rows = g->d_lverts + g->lvoff[l];
@@ -845,9 +713,13 @@ int gv_size_rows(gv_ctx_t c, gv_graph_t g, const gv_step_t* st, const uint32_t* 
     a.cnt = f->cnt;
     a.anc_i = f->anc_i;
     a.bytes = c->acc + 1;
+    a.req = req_ptr(c, kKindSize);
     a.st = *st;
-    hipLaunchKernelGGL(k_size, dim3(grid_for(nrows, kBlock, c->num_cu * 16)), dim3(kBlock), 0,
-                       c->stream, a);
+    {
+        ProfScope ps(c, kKindSize);
+        hipLaunchKernelGGL(k_size, dim3(grid_for(nrows, kBlock, c->num_cu * 16)), dim3(kBlock), 0,
+                           c->stream, a);
+    }
     HIPCHK(hipGetLastError());
     int rc = scan_counts(c, f->cnt, nrows, f->csum);
     if (rc != GV_OK) return rc;
@@ -862,6 +734,8 @@ static int launch_walk(gv_ctx* c, WalkArgs& a, bool count) {
     const uint64_t span = a.c1 - a.c0;
     const uint64_t tiles = (span + kTile - 1) / kTile;
     const int blocks = grid_for(tiles, kWavesPerBlock, c->num_cu * 32);
+    a.req = req_ptr(c, count ? kKindCount : kKindExt);
+    ProfScope ps(c, count ? kKindCount : kKindExt);
     if (count)
         hipLaunchKernelGGL(k_walk<true>, dim3(blocks), dim3(kBlock), 0, c->stream, a);
     else
@@ -986,6 +860,56 @@ int gv_copy_partial_emb(gv_ctx_t c, const uint32_t* src, uint32_t* dst, uint64_t
         HIPCHK(hipGetLastError());
         end = beg;
     }
+    return GV_OK;
+}
+
+int gv_profile_enable(gv_ctx_t c, int on) {
+    if (!c) return GV_ERR_ARG;
+    if (on && !c->preq) {
+        if (hipMalloc(&c->preq, kKinds * sizeof(unsigned long long)) != hipSuccess) return GV_ERR_NOMEM;
+        HIPCHK(hipMemset(c->preq, 0, kKinds * sizeof(unsigned long long)));
+    }
+    c->prof = on != 0;
+    return GV_OK;
+}
+
+int gv_profile_reset(gv_ctx_t c) {
+    if (!c) return GV_ERR_ARG;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (auto& r : c->recs) {
+        c->pool.push_back(r.a);
+        c->pool.push_back(r.b);
+    }
+    c->recs.clear();
+    c->scan_req = 0;
+    if (c->preq) HIPCHK(hipMemset(c->preq, 0, kKinds * sizeof(unsigned long long)));
+    return GV_OK;
+}
+
+int gv_profile_read(gv_ctx_t c, int kind, uint64_t* launches, double* total_ms,
+                    uint64_t* req_bytes) {
+    if (!c || kind < 0 || kind >= kKinds) return GV_ERR_ARG;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    uint64_t n = 0;
+    double ms = 0;
+    for (auto& r : c->recs) {
+        if (r.kind != kind) continue;
+        float t = 0;
+        HIPCHK(hipEventElapsedTime(&t, r.a, r.b));
+        ms += t;
+        ++n;
+    }
+    uint64_t rq = 0;
+    if (kind == kKindScan) {
+        rq = c->scan_req;
+    } else if (c->preq) {
+        unsigned long long h[kKinds];
+        HIPCHK(hipMemcpy(h, c->preq, sizeof(h), hipMemcpyDeviceToHost));
+        rq = h[kind];
+    }
+    if (launches) *launches = n;
+    if (total_ms) *total_ms = ms;
+    if (req_bytes) *req_bytes = rq;
     return GV_OK;
 }
 

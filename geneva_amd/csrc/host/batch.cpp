@@ -1,0 +1,205 @@
+// Batched mining over gv_batch_*.
+#include "batch.h"
+
+#include <algorithm>
+#include <chrono>
+#include <deque>
+
+namespace geneva {
+
+GraphStats GraphStats::of(gv_graph_t g) {
+    GraphStats s;
+    uint32_t np = 0;
+    gv_graph_info(g, nullptr, &s.nlabels, &np, nullptr, nullptr);
+    s.vertices.assign(s.nlabels, 0);
+    gv_graph_label_stats(g, 0, s.vertices.data(), nullptr);
+    for (uint32_t p = 0; p < std::max<uint32_t>(np, 1); ++p) {
+        std::vector<uint64_t> e((uint64_t)s.nlabels * s.nlabels, 0);
+        gv_graph_label_stats(g, p, nullptr, e.data());
+        s.pairEdges.push_back(e);
+    }
+    return s;
+}
+
+uint64_t estimatePeakBytes(const CompiledQuery& cq, const GraphStats& st) {
+    if (cq.empty) return 0;
+    const gv_query_t& q = cq.q;
+    std::vector<uint32_t> colLabel(q.k);
+    colLabel[0] = q.label0;
+    for (uint32_t t = 1; t < q.k; ++t) colLabel[t] = q.step[t].label;
+    double rows = (double)st.vertices[q.label0];
+    double peak = 0;
+    for (uint32_t t = 1; t < q.k; ++t) {
+        const bool last = t + 1 == q.k;
+        const bool fused = t + 2 == q.k && q.step[t + 1].nback == 1;
+        if (last || fused) break;
+        const gv_step_t& s = q.step[t];
+        double best = 1e300;
+        for (uint32_t i = 0; i < s.nback; ++i) {
+            const uint32_t a = colLabel[s.back_col[i]];
+            const double va = std::max<double>(1.0, (double)st.vertices[a]);
+            const double e = (double)st.pairEdges[s.back_part[i]][(uint64_t)a * st.nlabels + s.label];
+            best = std::min(best, e / va);
+        }
+        rows *= best;
+        peak = std::max(peak, rows * 4.0 * (t + 1));
+    }
+    return peak > 1.8e19 ? UINT64_MAX : (uint64_t)peak;
+}
+
+BatchMiner::BatchMiner(gv_ctx_t ctx, gv_graph_t g, PlainMiner* fallback, const BatchOptions& opt)
+    : ctx_(ctx), g_(g), fallback_(fallback), opt_(opt) {}
+
+BatchMiner::~BatchMiner() {
+    if (b_) gv_batch_destroy(b_);
+}
+
+int BatchMiner::init() {
+    if (b_) return GV_OK;
+    uint32_t ng = 0;
+    gv_graph_owners(g_, &ng);
+    nwords_ = (ng + 31) / 32;
+    return gv_batch_create(ctx_, g_, opt_.slots, opt_.slotBytes, opt_.rowCap, &opt_.tune, &b_);
+}
+
+int BatchMiner::setTune(const gv_tune_t& t) {
+    int rc = gv_batch_set_tune(b_, &t);
+    if (rc == GV_OK) opt_.tune = t;
+    return rc;
+}
+
+int BatchMiner::run(const std::vector<CompiledQuery>& qs, std::vector<MineResult>& out,
+                    BatchStats* st) {
+    int rc = init();
+    if (rc != GV_OK) return rc;
+    auto w0 = std::chrono::steady_clock::now();
+    out.assign(qs.size(), MineResult());
+    std::deque<size_t> pending;
+    for (size_t i = 0; i < qs.size(); ++i) {
+        if (qs[i].empty) {
+            out[i].graphBits.assign(nwords_, 0);
+            continue;
+        }
+        pending.push_back(i);
+    }
+    std::vector<size_t> slotQuery(opt_.slots, SIZE_MAX);
+    std::vector<size_t> overflow;
+    std::vector<gv_slot_info_t> info(opt_.slots);
+    BatchStats local;
+    uint32_t running = 0;
+    gv_mark(ctx_, 2);
+    while (!pending.empty() || running > 0) {
+        uint32_t maxk = 0;
+        while (!pending.empty()) {
+            uint32_t slot = 0;
+            rc = gv_batch_admit(b_, &qs[pending.front()].q, pending.front(), &slot);
+            if (rc == GV_ERR_CAPACITY) break;
+            if (rc != GV_OK) return rc;
+            slotQuery[slot] = pending.front();
+            maxk = std::max(maxk, qs[pending.front()].q.k);
+            pending.pop_front();
+            ++running;
+        }
+        // every admitted query needs at most k-1 iterations; deferred ones loop again
+        const uint32_t iters = std::max<uint32_t>(1, maxk ? maxk - 1 : 1);
+        rc = gv_batch_run(b_, iters);
+        if (rc != GV_OK) return rc;
+        local.graphLaunches += iters;
+        rc = gv_batch_poll(b_, info.data());
+        if (rc != GV_OK) return rc;
+        for (uint32_t s = 0; s < opt_.slots; ++s) {
+            if (slotQuery[s] == SIZE_MAX) continue;
+            const size_t qi = slotQuery[s];
+            if (info[s].state == GV_SLOT_DONE) {
+                MineResult& r = out[qi];
+                r.count = info[s].count;
+                r.algBytes = info[s].alg_bytes;
+                r.graphBits.assign(nwords_, 0);
+                rc = gv_batch_release(b_, s, nwords_ ? r.graphBits.data() : nullptr, nwords_);
+                if (rc != GV_OK) return rc;
+                slotQuery[s] = SIZE_MAX;
+                --running;
+                local.batched++;
+            } else if (info[s].state == GV_SLOT_OVERFLOW) {
+                rc = gv_batch_release(b_, s, nullptr, 0);
+                if (rc != GV_OK) return rc;
+                slotQuery[s] = SIZE_MAX;
+                --running;
+                overflow.push_back(qi);
+            }
+        }
+    }
+    gv_mark(ctx_, 3);
+    float ms = 0;
+    gv_elapsed_ms(ctx_, 2, 3, &ms);
+    local.deviceMs = ms;
+    for (size_t qi : overflow) {
+        local.overflowed++;
+        if (!fallback_) {
+            out[qi].status = GV_ERR_CAPACITY;
+            continue;
+        }
+        rc = fallback_->mine(qs[qi], out[qi]);
+        if (rc != GV_OK) return rc;
+    }
+    local.wallMs = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - w0).count();
+    if (st) *st = local;
+    return GV_OK;
+}
+
+namespace {
+std::map<std::string, gv_tune_t>& tuneCache() {
+    static std::map<std::string, gv_tune_t> m;
+    return m;
+}
+std::string shapeKey(const std::vector<CompiledQuery>& qs) {
+    // histogram of query sizes and of multi-edge steps
+    uint32_t byk[GV_MAXK + 1] = {0};
+    uint32_t cyc = 0;
+    for (auto& q : qs) {
+        byk[q.q.k]++;
+        for (uint32_t t = 1; t < q.q.k; ++t) cyc += q.q.step[t].nback > 1;
+    }
+    std::string k;
+    for (uint32_t i = 0; i <= GV_MAXK; ++i)
+        if (byk[i]) k += std::to_string(i) + ":" + std::to_string(byk[i] * 8 / std::max<size_t>(qs.size(), 1)) + ",";
+    return k + "c" + std::to_string(cyc * 4 / std::max<size_t>(qs.size(), 1));
+}
+}  // namespace
+
+int autotune(BatchMiner& bm, const std::vector<CompiledQuery>& sample, TuneResult& out) {
+    out = TuneResult();
+    const std::string key = shapeKey(sample);
+    auto it = tuneCache().find(key);
+    if (it != tuneCache().end()) {
+        out.tune = it->second;
+        return bm.setTune(out.tune);
+    }
+    const uint32_t ipts[] = {1, 2, 4, 8};
+    const uint32_t bpcs[] = {4, 8, 16, 32};
+    double best = 1e300;
+    std::vector<MineResult> res;
+    for (uint32_t ipt : ipts) {
+        for (uint32_t bpc : bpcs) {
+            gv_tune_t t{ipt, bpc};
+            int rc = bm.setTune(t);
+            if (rc != GV_OK) return rc;
+            BatchStats st;
+            // one untimed pass warms caches and code, the second is measured
+            rc = bm.run(sample, res, nullptr);
+            if (rc != GV_OK) return rc;
+            rc = bm.run(sample, res, &st);
+            if (rc != GV_OK) return rc;
+            out.tried.push_back({t, st.deviceMs});
+            if (st.deviceMs < best) {
+                best = st.deviceMs;
+                out.tune = t;
+            }
+        }
+    }
+    out.bestMs = best;
+    tuneCache()[key] = out.tune;
+    return bm.setTune(out.tune);
+}
+
+}  // namespace geneva

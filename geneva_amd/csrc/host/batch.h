@@ -1,0 +1,99 @@
+// Batched and served mining over the batch engine of libgeneva_hip.
+#pragma once
+#include <cstdint>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../../include/geneva_hip.h"
+#include "miner.h"
+
+namespace geneva {
+
+// Vertex and edge counts by label, per edge-label partition.
+struct GraphStats {
+    uint32_t nlabels = 0;
+    std::vector<uint64_t> vertices;                 // by dense label
+    std::vector<std::vector<uint64_t>> pairEdges;   // [part][a*nl+b]
+    static GraphStats of(gv_graph_t g);
+};
+
+// Estimate of the largest row table the plan materialises, in bytes: the start
+// table is the label's vertex count, each extension multiplies by the average
+// number of neighbours of the new vertex's label seen from its sparsest back
+// neighbour's label.  The count-only last step (and a single-edge last step
+// fused into the one before it) materialises nothing.
+uint64_t estimatePeakBytes(const CompiledQuery& cq, const GraphStats& st);
+
+struct BatchOptions {
+    uint32_t slots = 256;
+    uint64_t slotBytes = 64ull << 20;   // per table, two tables per slot
+    uint64_t rowCap = 1ull << 26;       // rows sized per iteration, whole batch
+    gv_tune_t tune{4, 16};
+};
+
+struct BatchStats {
+    uint64_t batched = 0;       // finished on the batch engine
+    uint64_t overflowed = 0;    // re-run on the plain path
+    uint64_t graphLaunches = 0;
+    double deviceMs = 0;        // batch iterations, device time
+    double wallMs = 0;
+};
+
+class BatchMiner {
+public:
+    BatchMiner(gv_ctx_t ctx, gv_graph_t g, PlainMiner* fallback, const BatchOptions& opt);
+    ~BatchMiner();
+    BatchMiner(const BatchMiner&) = delete;
+    BatchMiner& operator=(const BatchMiner&) = delete;
+
+    int init();
+    int setTune(const gv_tune_t& t);
+    const BatchOptions& options() const { return opt_; }
+    gv_batch_t handle() const { return b_; }
+    PlainMiner* fallback() const { return fallback_; }
+
+    // Mine every query; results in input order.  Queries with a plan error
+    // get status GV_ERR_ARG; overflowing ones are re-run on the plain path.
+    int run(const std::vector<CompiledQuery>& qs, std::vector<MineResult>& out, BatchStats* st);
+
+private:
+    gv_ctx_t ctx_;
+    gv_graph_t g_;
+    PlainMiner* fallback_;
+    BatchOptions opt_;
+    gv_batch_t b_ = nullptr;
+    uint32_t nwords_ = 0;
+};
+
+// Autotuner: times the batch engine under each candidate tuning on a sample of
+// queries and keeps the fastest, remembered per shape key (query sizes of the
+// sample).  Returns the chosen tuning.
+struct TuneResult {
+    gv_tune_t tune{4, 16};
+    double bestMs = 0;
+    std::vector<std::pair<gv_tune_t, double>> tried;
+};
+int autotune(BatchMiner& bm, const std::vector<CompiledQuery>& sample, TuneResult& out);
+
+// Serving: requests with arrival times (seconds from the start) are mined as
+// they arrive.  Batched: every loop turn admits the requests that have
+// arrived, while slots are free and the summed peak-table estimate of the
+// running requests stays under memBudget, then runs one iteration, in which
+// every running request advances by one vertex; requests leave as they
+// finish.  Plain: requests are mined one at a time in arrival order.
+struct ServeResult {
+    std::vector<double> latencyMs;   // arrival to result, per request
+    std::vector<uint64_t> counts;
+    double wallS = 0;                // first arrival to last result
+    uint64_t iterations = 0;
+    uint64_t maxConcurrent = 0;
+    uint64_t overflowed = 0;
+};
+int serveBatched(BatchMiner& bm, const std::vector<CompiledQuery>& reqs,
+                 const std::vector<double>& arrivalS, uint64_t memBudget, const GraphStats& gs,
+                 ServeResult& out);
+int servePlain(PlainMiner& pm, const std::vector<CompiledQuery>& reqs,
+               const std::vector<double>& arrivalS, ServeResult& out);
+
+}  // namespace geneva

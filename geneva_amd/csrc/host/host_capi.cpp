@@ -10,6 +10,7 @@
 #include "gen.h"
 #include "genMatchOrder.h"
 #include "loadDataBase.h"
+#include "batch.h"
 #include "miner.h"
 
 namespace {
@@ -25,8 +26,37 @@ struct gh_engine {
     gv_ctx_t ctx = nullptr;
     gv_graph_t graph = nullptr;
     std::unique_ptr<geneva::PlainMiner> plain;
+    std::unique_ptr<geneva::BatchMiner> batch;
+    geneva::BatchOptions batchOpt;
+    std::unique_ptr<geneva::GraphStats> stats;
     uint32_t ngraphs = 0;
 };
+
+namespace {
+int ensureBatch(gh_engine* e) {
+    if (!e->batch) {
+        e->batch.reset(new geneva::BatchMiner(e->ctx, e->graph, e->plain.get(), e->batchOpt));
+        int rc = e->batch->init();
+        if (rc != GV_OK) {
+            e->batch.reset();
+            return rc;
+        }
+    }
+    return GV_OK;
+}
+int compileAll(gh_engine* e, const char* const* qs, uint32_t n, int single_v,
+               std::vector<geneva::CompiledQuery>& out) {
+    out.assign(n, geneva::CompiledQuery());
+    for (uint32_t i = 0; i < n; ++i) {
+        std::string err;
+        if (!qs[i] || !geneva::planAndCompile(qs[i], single_v != 0, e->graph, out[i], &err)) {
+            setErr("query " + std::to_string(i) + ": " + err);
+            return GV_ERR_ARG;
+        }
+    }
+    return GV_OK;
+}
+}  // namespace
 
 extern "C" {
 
@@ -276,6 +306,7 @@ gh_engine_t gh_engine_create(gh_dataset_t d, int device, uint64_t level_bytes, i
 
 void gh_engine_free(gh_engine_t e) {
     if (!e) return;
+    e->batch.reset();
     e->plain.reset();
     if (e->graph) gv_graph_destroy(e->graph);
     if (e->ctx) gv_ctx_destroy(e->ctx);
@@ -306,6 +337,123 @@ int gh_engine_mine(gh_engine_t e, const char* text, int single_v, uint64_t* coun
         for (uint32_t i = 0; i < nwords; ++i) bits[i] = i < r.graphBits.size() ? r.graphBits[i] : 0;
     }
     return GV_OK;
+}
+
+int gh_engine_batch_config(gh_engine_t e, uint32_t slots, uint64_t slot_bytes, uint64_t row_cap,
+                           uint32_t ipt, uint32_t bpc) {
+    if (!e) return GV_ERR_ARG;
+    e->batch.reset();
+    if (slots) e->batchOpt.slots = slots;
+    if (slot_bytes) e->batchOpt.slotBytes = slot_bytes;
+    if (row_cap) e->batchOpt.rowCap = row_cap;
+    if (ipt) e->batchOpt.tune.ipt = ipt;
+    if (bpc) e->batchOpt.tune.walk_blocks_per_cu = bpc;
+    return GV_OK;
+}
+
+int gh_engine_batch_mine(gh_engine_t e, const char* const* queries, uint32_t n, int single_v,
+                         uint64_t* counts, uint64_t* alg_bytes, uint32_t* bits, uint32_t nwords,
+                         double* stats) {
+    if (!e || (n && !queries)) return GV_ERR_ARG;
+    std::vector<geneva::CompiledQuery> qs;
+    int rc = compileAll(e, queries, n, single_v, qs);
+    if (rc != GV_OK) return rc;
+    rc = ensureBatch(e);
+    if (rc != GV_OK) {
+        setErr(std::string("batch engine: ") + gv_status_string(rc) + " " + gv_last_error());
+        return rc;
+    }
+    std::vector<geneva::MineResult> res;
+    geneva::BatchStats st;
+    rc = e->batch->run(qs, res, &st);
+    if (rc != GV_OK) {
+        setErr(std::string("batch run: ") + gv_status_string(rc) + " " + gv_last_error());
+        return rc;
+    }
+    for (uint32_t i = 0; i < n; ++i) {
+        if (res[i].status != GV_OK) {
+            setErr("query " + std::to_string(i) + " failed");
+            return res[i].status;
+        }
+        if (counts) counts[i] = res[i].count;
+        if (alg_bytes) alg_bytes[i] = res[i].algBytes;
+        if (bits && nwords)
+            for (uint32_t w = 0; w < nwords; ++w)
+                bits[(uint64_t)i * nwords + w] = w < res[i].graphBits.size() ? res[i].graphBits[w] : 0;
+    }
+    if (stats) {
+        stats[0] = (double)st.batched;
+        stats[1] = (double)st.overflowed;
+        stats[2] = (double)st.graphLaunches;
+        stats[3] = st.deviceMs;
+        stats[4] = st.wallMs;
+    }
+    return GV_OK;
+}
+
+int gh_engine_autotune(gh_engine_t e, const char* const* queries, uint32_t n, uint32_t* ipt,
+                       uint32_t* bpc, double* best_ms) {
+    if (!e || (n && !queries)) return GV_ERR_ARG;
+    std::vector<geneva::CompiledQuery> qs;
+    int rc = compileAll(e, queries, n, 0, qs);
+    if (rc != GV_OK) return rc;
+    rc = ensureBatch(e);
+    if (rc != GV_OK) return rc;
+    geneva::TuneResult tr;
+    rc = geneva::autotune(*e->batch, qs, tr);
+    if (rc != GV_OK) return rc;
+    e->batchOpt.tune = tr.tune;
+    if (ipt) *ipt = tr.tune.ipt;
+    if (bpc) *bpc = tr.tune.walk_blocks_per_cu;
+    if (best_ms) *best_ms = tr.bestMs;
+    return GV_OK;
+}
+
+int gh_engine_serve(gh_engine_t e, const char* const* queries, const double* arrival_s, uint32_t n,
+                    int mode, uint64_t mem_budget, double* latency_ms, uint64_t* counts,
+                    double* stats) {
+    if (!e || !arrival_s || !latency_ms || (n && !queries)) return GV_ERR_ARG;
+    for (uint32_t i = 1; i < n; ++i)
+        if (arrival_s[i] < arrival_s[i - 1]) return GV_ERR_ARG;
+    std::vector<geneva::CompiledQuery> qs;
+    int rc = compileAll(e, queries, n, 0, qs);
+    if (rc != GV_OK) return rc;
+    std::vector<double> arr(arrival_s, arrival_s + n);
+    geneva::ServeResult sr;
+    if (mode == 0) {
+        rc = ensureBatch(e);
+        if (rc != GV_OK) return rc;
+        if (!e->stats) e->stats.reset(new geneva::GraphStats(geneva::GraphStats::of(e->graph)));
+        rc = geneva::serveBatched(*e->batch, qs, arr, mem_budget, *e->stats, sr);
+    } else {
+        rc = geneva::servePlain(*e->plain, qs, arr, sr);
+    }
+    if (rc != GV_OK) {
+        setErr(std::string("serve: ") + gv_status_string(rc) + " " + gv_last_error());
+        return rc;
+    }
+    for (uint32_t i = 0; i < n; ++i) {
+        latency_ms[i] = sr.latencyMs[i];
+        if (counts) counts[i] = sr.counts[i];
+    }
+    if (stats) {
+        stats[0] = sr.wallS;
+        stats[1] = (double)sr.iterations;
+        stats[2] = (double)sr.maxConcurrent;
+        stats[3] = (double)sr.overflowed;
+    }
+    return GV_OK;
+}
+
+int gh_engine_profile(gh_engine_t e, int on) {
+    return e ? gv_profile_enable(e->ctx, on) : GV_ERR_ARG;
+}
+
+int gh_engine_profile_reset(gh_engine_t e) { return e ? gv_profile_reset(e->ctx) : GV_ERR_ARG; }
+
+int gh_engine_profile_read(gh_engine_t e, int kind, uint64_t* launches, double* total_ms,
+                           uint64_t* req_bytes) {
+    return e ? gv_profile_read(e->ctx, kind, launches, total_ms, req_bytes) : GV_ERR_ARG;
 }
 
 }  // extern "C"

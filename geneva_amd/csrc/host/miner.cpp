@@ -6,6 +6,10 @@
 
 namespace geneva {
 
+namespace {
+constexpr uint64_t kCountChunk = 1ull << 31;  // candidates per counting launch
+}
+
 bool parseQueryPattern(const std::string& text, QueryPattern& q, std::string* err) {
     std::istringstream in(text);
     std::string line;
@@ -209,13 +213,19 @@ int PlainMiner::run(const gv_query_t& q, uint32_t width, const uint32_t* rows, u
     uint64_t total = 0;
     rc = gv_size_rows(ctx_, g_, &st, rows, width, n, fr, &total);
     if (rc != GV_OK || total == 0) return rc;
-    if (width + 1 == q.k) {
-        out.launches++;
-        return gv_count_emb(ctx_, g_, &st, nullptr, rows, width, n, fr, 0, total);
-    }
-    if (width + 2 == q.k && q.step[width + 1].nback == 1) {
-        out.launches++;
-        return gv_count_emb(ctx_, g_, &st, &q.step[width + 1], rows, width, n, fr, 0, total);
+    const bool last = width + 1 == q.k;
+    const bool fusedTail = width + 2 == q.k && q.step[width + 1].nback == 1;
+    if (last || fusedTail) {
+        // count-only: no output table, so any range works; launches are cut at
+        // kCountChunk candidates to keep each kernel short
+        const gv_step_t* tail = fusedTail ? &q.step[width + 1] : nullptr;
+        for (uint64_t c0 = 0; c0 < total; c0 += kCountChunk) {
+            const uint64_t c1 = std::min(total, c0 + kCountChunk);
+            out.launches++;
+            rc = gv_count_emb(ctx_, g_, &st, tail, rows, width, n, fr, c0, c1);
+            if (rc != GV_OK) return rc;
+        }
+        return GV_OK;
     }
     rc = ensureRows(width + 1);
     if (rc != GV_OK) return rc;

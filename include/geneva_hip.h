@@ -110,6 +110,11 @@ int gv_graph_part_index(gv_graph_t g, uint32_t edge_label, uint32_t* part);
  * device bytes held. */
 int gv_graph_info(gv_graph_t g, uint64_t* nv1, uint32_t* nlabels, uint32_t* nparts,
                   uint64_t* nnbr, uint64_t* dev_bytes);
+/* Label statistics of partition `part`: vertices per dense label
+ * (nlabels entries) and directed edges per (source label, neighbour label)
+ * (nlabels*nlabels entries, row-major).  Either pointer may be null. */
+int gv_graph_label_stats(gv_graph_t g, uint32_t part, uint64_t* vertices_by_label,
+                         uint64_t* edges_by_label_pair);
 /* The one-column start table of a label: every vertex carrying it, ascending. */
 int gv_graph_start_rows(gv_graph_t g, uint32_t dense_label, const uint32_t** rows_dev,
                         uint64_t* nrows);
@@ -160,6 +165,73 @@ int gv_counters_read(gv_ctx_t ctx, uint64_t* count, uint64_t* alg_bytes,
 
 /* Overlap-safe device copy of n uints (src/reduction.cu:280-296). */
 int gv_copy_partial_emb(gv_ctx_t ctx, const uint32_t* src_dev, uint32_t* dst_dev, uint64_t n);
+
+/* ---- batched execution ------------------------------------------------------
+ * Many queries advance together: every iteration moves each running query one
+ * vertex further, each at its own depth, in one fixed sequence of kernels
+ * (prepare, size, scan, extend/count, retire) that reads all sizes from
+ * device memory.  The sequence is captured once in a hipGraph and replayed, so
+ * an iteration costs one graph launch and no host synchronisation.  A query
+ * occupies a slot: two private row tables of `slot_bytes` each (input and
+ * output, swapped every iteration).  A query whose table would exceed its slot
+ * stops with state GV_SLOT_OVERFLOW and its partial counts are discarded; the
+ * caller re-runs it on the chunked plain path.  Up to 1024 slots. */
+typedef struct gv_batch* gv_batch_t;
+
+typedef struct {
+    uint32_t ipt;             /* candidates per lane per extension tile: 1, 2, 4 or 8 */
+    uint32_t walk_blocks_per_cu;  /* persistent grid of the extension kernel */
+} gv_tune_t;
+
+#define GV_SLOT_FREE 0
+#define GV_SLOT_RUNNING 1
+#define GV_SLOT_DONE 2
+#define GV_SLOT_OVERFLOW 3
+
+typedef struct {
+    uint64_t tag;             /* caller's id, given at admission */
+    uint32_t state;           /* GV_SLOT_* */
+    uint32_t width;           /* columns placed */
+    uint64_t rows;            /* rows of the current table */
+    uint64_t count;           /* embeddings counted (valid when DONE) */
+    uint64_t alg_bytes;       /* algorithmic bytes (same figure as the plain path) */
+} gv_slot_info_t;
+
+/* row_cap: rows the whole batch may size in one iteration (queries beyond it
+ * wait for the next iteration). */
+int gv_batch_create(gv_ctx_t ctx, gv_graph_t g, uint32_t slots, uint64_t slot_bytes,
+                    uint64_t row_cap, const gv_tune_t* tune, gv_batch_t* out);
+int gv_batch_destroy(gv_batch_t b);
+/* Re-capture the iteration with other tuning parameters. */
+int gv_batch_set_tune(gv_batch_t b, const gv_tune_t* tune);
+/* Place a compiled query in a free slot (stream-ordered; effective at the next
+ * iteration).  GV_ERR_CAPACITY when no slot is free. */
+int gv_batch_admit(gv_batch_t b, const gv_query_t* q, uint64_t tag, uint32_t* slot);
+/* Launch `iterations` iterations (asynchronous). */
+int gv_batch_run(gv_batch_t b, uint32_t iterations);
+/* Copy every slot's state to info[0..slots) (synchronises). */
+int gv_batch_poll(gv_batch_t b, gv_slot_info_t* info);
+/* Free a finished slot; graph_bits (nullable) receives its matched graph ids. */
+int gv_batch_release(gv_batch_t b, uint32_t slot, uint32_t* graph_bits, uint32_t nwords);
+/* Slots, rows a slot table holds at a given width, free slots. */
+int gv_batch_info(gv_batch_t b, uint32_t* slots, uint64_t* slot_bytes, uint32_t* free_slots);
+
+/* Per-kernel profiling on the context's stream.  While enabled, every launch
+ * is bracketed by HIP events and each kernel kind counts the bytes its
+ * algorithm reads and writes (each access once).  Kinds: 0 sizing (k_size),
+ * 1 scan (k_scan_*), 2 extension (k_walk, writing rows), 3 counting (k_walk,
+ * last step), 4 batch iteration (one graph launch).  read synchronises and
+ * returns the launch count, the summed device milliseconds and the requested
+ * bytes since the last reset. */
+#define GV_KIND_SIZE 0
+#define GV_KIND_SCAN 1
+#define GV_KIND_EXTEND 2
+#define GV_KIND_COUNT 3
+#define GV_KIND_BATCH 4
+int gv_profile_enable(gv_ctx_t ctx, int on);
+int gv_profile_reset(gv_ctx_t ctx);
+int gv_profile_read(gv_ctx_t ctx, int kind, uint64_t* launches, double* total_ms,
+                    uint64_t* req_bytes);
 
 /* Timing on the context's stream (HIP events): record a mark, and the
  * milliseconds between two marks (synchronises on the second). */

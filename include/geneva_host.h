@@ -73,6 +73,39 @@ int gh_engine_mine(gh_engine_t e, const char* query_text, int single_v, uint64_t
                    uint64_t* alg_bytes, double* device_ms, uint32_t* graph_bits,
                    uint32_t nwords, uint64_t* launches);
 
+/* ---- batched path ------------------------------------------------------------
+ * Configure the batch engine (slots, bytes per slot table, rows sized per
+ * iteration, candidates per lane per tile, extension blocks per CU); the
+ * engine is (re)built on the next batched call. */
+int gh_engine_batch_config(gh_engine_t e, uint32_t slots, uint64_t slot_bytes, uint64_t row_cap,
+                           uint32_t ipt, uint32_t blocks_per_cu);
+/* Mine n queries together.  counts/alg_bytes: n entries; graph_bits: n*nwords
+ * (nullable); stats (nullable, 5 doubles): batched, overflowed (re-run on the
+ * plain path), graph launches, device ms of the batch iterations, wall ms. */
+int gh_engine_batch_mine(gh_engine_t e, const char* const* queries, uint32_t n, int single_v,
+                         uint64_t* counts, uint64_t* alg_bytes, uint32_t* graph_bits,
+                         uint32_t nwords, double* stats);
+/* Autotune the batch engine on a sample; reports the chosen candidates per
+ * lane and blocks per CU and its device ms. */
+int gh_engine_autotune(gh_engine_t e, const char* const* queries, uint32_t n, uint32_t* ipt,
+                       uint32_t* blocks_per_cu, double* best_ms);
+/* Serve n requests arriving at arrival_s[i] seconds (non-decreasing).
+ * mode 0: continuous batching with admission under mem_budget bytes of
+ * estimated peak tables; mode 1: plain path, one request at a time.
+ * latency_ms and counts receive n entries; stats (nullable, 4 doubles): wall
+ * seconds, iterations, most requests in flight, requests re-run on the plain
+ * path. */
+int gh_engine_serve(gh_engine_t e, const char* const* queries, const double* arrival_s, uint32_t n,
+                    int mode, uint64_t mem_budget, double* latency_ms, uint64_t* counts,
+                    double* stats);
+
+/* Per-kernel profiling of the engine's context (see gv_profile_* in
+ * geneva_hip.h): enable/disable, reset, and read one kernel kind. */
+int gh_engine_profile(gh_engine_t e, int on);
+int gh_engine_profile_reset(gh_engine_t e);
+int gh_engine_profile_read(gh_engine_t e, int kind, uint64_t* launches, double* total_ms,
+                           uint64_t* req_bytes);
+
 #ifdef __cplusplus
 }
 #endif

@@ -32,7 +32,7 @@ def lib():
         L.oracle_plan.argtypes = [C.c_char_p, C.c_char_p, C.c_uint64]
         L.oracle_mine.restype = C.c_int
         L.oracle_mine.argtypes = [C.c_void_p, C.c_char_p, C.c_int, C.c_int, C.c_uint64, C.c_uint64,
-                                  u32p, C.c_uint32, C.POINTER(C.c_uint8), C.POINTER(C.c_uint64),
+                                  C.c_uint64, u32p, C.c_uint32, C.POINTER(C.c_uint8), C.POINTER(C.c_uint64),
                                   C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
                                   C.POINTER(C.c_double)]
         _lib = L
@@ -75,7 +75,7 @@ class OracleGraph:
         if h and _lib is not None:
             _lib.oracle_graph_destroy(h)
 
-    def mine(self, query_text, mode="count", threads=0, start_begin=0, start_end=0):
+    def mine(self, query_text, mode="count", threads=0, start_begin=0, start_end=0, stride=1):
         cnt, by, ns = C.c_uint64(0), C.c_uint64(0), C.c_uint64(0)
         secs = C.c_double(0)
         bits = None
@@ -84,7 +84,7 @@ class OracleGraph:
             bits = (C.c_uint8 * self.ngraphs)()
             gop = self.graph_of.ctypes.data_as(C.POINTER(C.c_uint32))
         rc = lib().oracle_mine(self._h, query_text.encode(), MODES[mode], threads, start_begin,
-                               start_end, gop, self.ngraphs if bits is not None else 0, bits,
+                               start_end, stride, gop, self.ngraphs if bits is not None else 0, bits,
                                C.byref(cnt), C.byref(by), C.byref(ns), C.byref(secs))
         if rc != 0:
             raise ValueError("oracle: malformed query")

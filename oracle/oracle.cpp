@@ -429,13 +429,15 @@ uint64_t oracle_plan(const char* query, char* out, uint64_t cap) {
 }
 
 // Mine one query.  mode 0 = count (threads OpenMP workers; start vertices
-// restricted to the index range [startBegin, startEnd) of the label-sorted
-// start list, startEnd = 0 meaning all), 1 = faithful, 2 = brute force.
+// restricted to indices startBegin, startBegin+stride, ... below startEnd of
+// the id-sorted start list, startEnd = 0 meaning all), 1 = faithful,
+// 2 = brute force.
 // graphOf (nullable): graph id of every vertex (nv+1 entries); bits receives
 // one byte per graph (1 = matched).  Returns 0 on success, negative on a
 // malformed query.
 int oracle_mine(void* h, const char* query, int mode, int threads, uint64_t startBegin,
-                uint64_t startEnd, const uint32_t* graphOf, uint32_t ngraphs, uint8_t* bits,
+                uint64_t startEnd, uint64_t stride, const uint32_t* graphOf, uint32_t ngraphs,
+                uint8_t* bits,
                 uint64_t* countOut, uint64_t* bytesOut, uint64_t* nstartOut, double* secs) {
     Graph& g = *(Graph*)h;
     OPlan P;
@@ -460,12 +462,14 @@ int oracle_mine(void* h, const char* query, int mode, int threads, uint64_t star
             const std::vector<unsigned>& starts = it->second;
             uint64_t e = startEnd == 0 ? starts.size() : std::min<uint64_t>(startEnd, starts.size());
             uint64_t b = std::min<uint64_t>(startBegin, e);
-            nstart = e - b;
+            const uint64_t st = stride ? stride : 1;
+            nstart = (e - b + st - 1) / st;
             const bool wantBits = !gof.empty();
             int nt = threads > 0 ? threads : omp_get_max_threads();
             std::vector<std::vector<uint8_t>> tb(wantBits ? nt : 0, std::vector<uint8_t>(ngraphs, 0));
 #pragma omp parallel for num_threads(nt) schedule(dynamic, 16) reduction(+ : count, bytes)
-            for (uint64_t i = b; i < e; ++i) {
+            for (uint64_t j = 0; j < nstart; ++j) {
+                const uint64_t i = b + j * st;
                 unsigned row[32];
                 row[0] = starts[i];
                 Acc acc;

@@ -100,7 +100,22 @@ def plan_corpus():
     return qs
 
 
+def full_workload2():
+    ds = configs.workload2()
+    ogw = O.OracleGraph.of(ds)
+    cases = []
+    for qt in ds.queries:
+        r = ogw.mine(qt, threads=os.cpu_count())
+        cases.append({"query": qt, "count": r.count, "alg_bytes": r.alg_bytes,
+                      "oracle_seconds": r.seconds, "threads": os.cpu_count()})
+        print("workload2 full:", r.count, r.seconds, flush=True)
+    json.dump({"workload": 2, "scale": 1.0, "cases": cases},
+              open(os.path.join(HERE, "workload2_full.json"), "w"), indent=1)
+
+
 def main():
+    if "--only-full2" in sys.argv:
+        return full_workload2()
     full2 = "--full2" in sys.argv
     if not os.path.exists(O.REF_CPUMINE):
         sys.exit("oracle/_ref/cpumine missing: make -C oracle ref")
@@ -175,16 +190,7 @@ def main():
     json.dump({"workloads": wl}, open(os.path.join(HERE, "scaled.json"), "w"), indent=1)
 
     if full2:
-        ds = configs.workload2()
-        ogw = O.OracleGraph.of(ds)
-        cases = []
-        for qt in ds.queries:
-            r = ogw.mine(qt, threads=os.cpu_count())
-            cases.append({"query": qt, "count": r.count, "alg_bytes": r.alg_bytes,
-                          "oracle_seconds": r.seconds, "threads": os.cpu_count()})
-            print("workload2 full:", r.count, r.seconds, flush=True)
-        json.dump({"workload": 2, "scale": 1.0, "cases": cases},
-                  open(os.path.join(HERE, "workload2_full.json"), "w"), indent=1)
+        full_workload2()
 
 
 if __name__ == "__main__":

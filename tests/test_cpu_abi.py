@@ -1,0 +1,65 @@
+"""CPU tests of the C-ABI boundary: both libraries load without a GPU and
+export every function their headers declare; argument checks reject bad calls
+without touching a device."""
+import ctypes as C
+import os
+import re
+
+import geneva_amd as G
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared(header):
+    text = open(os.path.join(ROOT, "include", header)).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    names = re.findall(r"^\s*(?:const\s+)?[\w\s\*]+?\b(g[vh]_\w+)\s*\(", text, flags=re.M)
+    return sorted(set(names))
+
+
+def _exports(path):
+    import subprocess
+    out = subprocess.run(["nm", "-D", "--defined-only", path], capture_output=True, text=True).stdout
+    return {l.split()[-1] for l in out.splitlines() if " T " in l}
+
+
+def test_hip_library_exports_every_declared_symbol():
+    names = _declared("geneva_hip.h")
+    assert len(names) >= 25
+    exp = _exports(G.HIP_LIB)
+    missing = [n for n in names if n not in exp]
+    assert not missing, missing
+    lib = C.CDLL(G.HIP_LIB)
+    for n in names:
+        getattr(lib, n)
+
+
+def test_host_library_exports_every_declared_symbol():
+    names = _declared("geneva_host.h")
+    assert len(names) >= 20
+    exp = _exports(G.HOST_LIB)
+    missing = [n for n in names if n not in exp]
+    assert not missing, missing
+
+
+def test_argument_checks_without_a_device():
+    lib = C.CDLL(G.HIP_LIB)
+    lib.gv_status_string.restype = C.c_char_p
+    assert lib.gv_status_string(0) == b"ok"
+    assert lib.gv_ctx_create(0, None) == -1                    # null out pointer
+    assert lib.gv_graph_destroy(None) == -1
+    assert lib.gv_frontier_create(None, 10, None) == -1
+    assert lib.gv_size_rows(None, None, None, None, 1, 1, None, None) == -1
+    n = C.c_int(-1)
+    assert lib.gv_device_count(C.byref(n)) == 0
+    assert n.value >= 0
+
+
+def test_engine_requires_a_gpu_and_never_falls_back():
+    import pytest
+    ds = G.Dataset.database(5, 5, 8, 2.0, 0.0, 2, 0.0, 1)
+    n = C.c_int(0)
+    C.CDLL(G.HIP_LIB).gv_device_count(C.byref(n))
+    if n.value == 0:
+        with pytest.raises(G.GenevaError):
+            G.Engine(ds)

@@ -1,0 +1,105 @@
+"""GPU tests of the batched engine (hipGraph iterations, device work queue over
+slots) and of the serving loop: every result must equal the plain path's and
+the oracle's, alone and in the mix, through repeated graph replays."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import geneva_amd as G
+from geneva_amd import configs
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def w3(require_gpu):
+    gold = json.load(open(os.path.join(GOLDEN, "scaled.json")))
+    w = [x for x in gold["workloads"] if x["workload"] == 3][0]
+    ds = configs.workload3(scale=w["scale"], **w["kw"])
+    eng = G.Engine(ds)
+    eng.batch_config(slots=64, slot_bytes=8 << 20, row_cap=1 << 22)
+    return ds, eng, w["cases"]
+
+
+def test_batch_matches_golden_and_plain(w3):
+    ds, eng, cases = w3
+    qs = [c["query"] for c in cases]
+    print("batch of", len(qs), "queries", flush=True)
+    res, st = eng.mine_batch(qs)
+    print("stats", st, flush=True)
+    for c, r in zip(cases, res):
+        assert r.count == c["count"]
+        assert r.alg_bytes == c["alg_bytes"]
+        assert sorted(r.graph_ids) == c["graph_ids"]
+    # replay the captured iteration again, with eager plain-path work between
+    for rep in range(3):
+        _ = eng.mine(qs[rep])
+        res2, _ = eng.mine_batch(qs)
+        assert [r.count for r in res2] == [r.count for r in res]
+        assert [r.graph_ids for r in res2] == [r.graph_ids for r in res]
+
+
+def test_each_result_same_alone_and_in_the_mix(w3):
+    ds, eng, cases = w3
+    qs = [c["query"] for c in cases][:16]
+    mix, _ = eng.mine_batch(qs)
+    for q, m in zip(qs, mix):
+        alone, _ = eng.mine_batch([q])
+        plain = eng.mine(q)
+        assert alone[0].count == m.count == plain.count
+        assert alone[0].graph_ids == m.graph_ids == plain.graph_ids
+
+
+def test_overflow_falls_back_to_plain_path(require_gpu):
+    import oracle as O
+    ds = configs.workload4(scale=0.005, nqueries=8)
+    og = O.OracleGraph.of(ds)
+    eng = G.Engine(ds)
+    eng.batch_config(slots=8, slot_bytes=1 << 16, row_cap=1 << 20)   # tiny slot tables
+    res, st = eng.mine_batch(ds.queries)
+    print("stats", st, flush=True)
+    assert st["overflowed"] >= 1
+    for q, r in zip(ds.queries, res):
+        assert r.count == og.mine(q, threads=4).count
+
+
+def test_row_cap_defers_slots_without_changing_results(w3):
+    ds, eng, cases = w3
+    qs = [c["query"] for c in cases][:24]
+    want = [c["count"] for c in cases][:24]
+    eng.batch_config(slots=64, slot_bytes=8 << 20, row_cap=1 << 14)
+    res, st = eng.mine_batch(qs)
+    assert [r.count for r in res] == want
+    eng.batch_config(slots=64, slot_bytes=8 << 20, row_cap=1 << 22)
+
+
+def test_tunings_agree(w3):
+    ds, eng, cases = w3
+    qs = [c["query"] for c in cases][:20]
+    want = [c["count"] for c in cases][:20]
+    for ipt in (1, 2, 4, 8):
+        for bpc in (4, 32):
+            eng.batch_config(slots=64, slot_bytes=8 << 20, row_cap=1 << 22, ipt=ipt, blocks_per_cu=bpc)
+            res, _ = eng.mine_batch(qs)
+            assert [r.count for r in res] == want, (ipt, bpc)
+    tuned = eng.autotune(qs)
+    print("autotune", tuned, flush=True)
+    res, _ = eng.mine_batch(qs)
+    assert [r.count for r in res] == want
+
+
+def test_serving_results_match(w3):
+    ds, eng, cases = w3
+    qs = [c["query"] for c in cases]
+    want = [c["count"] for c in cases]
+    rng = np.random.default_rng(5)
+    arr = np.cumsum(rng.exponential(1 / 2000.0, size=len(qs))).tolist()
+    lat, cnt, st = eng.serve(qs, arr, batched=True)
+    print("serve stats", st, "p50", np.percentile(lat, 50), "p99", np.percentile(lat, 99), flush=True)
+    assert cnt == want
+    assert all(l > 0 for l in lat)
+    lat2, cnt2, st2 = eng.serve(qs, arr, batched=False)
+    assert cnt2 == want

@@ -95,7 +95,8 @@ def test_hub_graph_and_chunked_frontiers(require_gpu):
     rng = np.random.default_rng(7)
     n = 30000
     vl = np.concatenate([[0], rng.integers(1, 4, size=n)]).astype(np.uint32)
-    hub = [(1, v, 1) for v in range(2, 20001)]
+    # the hub touches every vertex, so the sources form one id interval
+    hub = [(1, v, 1) for v in range(2, n + 1)]
     rest = rng.integers(2, n + 1, size=(40000, 2))
     rest = [(int(a), int(b), 1) for a, b in rest if a != b]
     ds = G.Dataset.from_arrays(vl, np.array(hub + rest, dtype=np.uint32))
