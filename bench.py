@@ -50,6 +50,101 @@ def cpu_baseline(ds, queries, budget_s, threads):
                       + "; ".join(samples) + f"; {tot_secs:.1f}s of CPU work"}
 
 
+def copy_calibration(mib):
+    """Run the library's overlap-safe copy kernel on a known size: 4n bytes read
+    and 4n bytes written per element count n, in launches of n/2 elements, so
+    PMC readings of FETCH_SIZE / WRITE_SIZE can be scaled to bytes."""
+    import ctypes as C
+    import geneva_amd as G
+    hip = C.CDLL(G.HIP_LIB)
+    ctx = C.c_void_p()
+    assert hip.gv_ctx_create(0, C.byref(ctx)) == 0
+    n = (mib << 20) // 4
+    buf = C.c_void_p()
+    assert hip.gv_malloc(ctx, C.c_uint64(4 * (n + n // 2)), C.byref(buf)) == 0
+    src = buf.value
+    dst = src + 4 * (n // 2)
+    hip.gv_copy_partial_emb.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64]
+    for _ in range(3):
+        assert hip.gv_copy_partial_emb(ctx, src, dst, n) == 0
+    assert hip.gv_ctx_synchronize(ctx) == 0
+    print(json.dumps({"copy_calibration": {"elements": n, "launches_per_copy": 2,
+                                           "bytes_read_per_launch": 4 * (n // 2),
+                                           "bytes_written_per_launch": 4 * (n // 2), "copies": 3}}))
+    hip.gv_free(ctx, buf)
+    hip.gv_ctx_destroy(ctx)
+
+
+def pct(xs, q):
+    import numpy as np
+    return float(np.percentile(np.asarray(xs, dtype=float), q)) if len(xs) else 0.0
+
+
+def batched_and_served(args):
+    """Workload 3 (database of 100,000 graphs, 256 queries of 4-8 vertices):
+    plain path vs batched engine (autotuned, captured iteration graph); and
+    workload 5 (the same database, Poisson arrivals, queries of 3-10 vertices):
+    continuous batching vs the plain one-at-a-time server."""
+    import numpy as np
+    import geneva_amd as G
+    from geneva_amd import configs
+    t0 = time.perf_counter()
+    ds = configs.workload3(scale=args.db_scale)
+    eng = G.Engine(ds, device=0, level_bytes=1 << 30)
+    eng.batch_config(slots=256, slot_bytes=64 << 20, row_cap=1 << 28)
+    qs = ds.queries
+    log(f"workload 3 scale {args.db_scale}: nv={ds.nv} graphs={ds.ngraphs} queries={len(qs)} "
+        f"ready in {time.perf_counter() - t0:.1f}s")
+    tune = eng.autotune(qs[:96])
+    log(f"autotune: {tune}")
+    # plain path: one query at a time
+    for q in qs[:8]:
+        eng.mine(q)
+    tp = time.perf_counter()
+    plain = [eng.mine(q) for q in qs]
+    tp = time.perf_counter() - tp
+    # batched: all queries in one engine run (warm once)
+    eng.mine_batch(qs)
+    tb = time.perf_counter()
+    reps = max(1, args.batch_reps)
+    for _ in range(reps):
+        batched, st = eng.mine_batch(qs)
+    tb = (time.perf_counter() - tb) / reps
+    same = all(a.count == b.count and a.graph_ids == b.graph_ids for a, b in zip(plain, batched))
+    emb = sum(r.count for r in batched)
+    out = {"workload": "workload3: 100,000 graphs n~U[20,200] ER/BA(m=2), 32 labels Zipf(1.1), "
+                       "256 walk queries of 4-8 vertices" if args.db_scale == 1.0 else
+                       f"workload3 scale {args.db_scale}",
+           "queries": len(qs), "embeddings": emb,
+           "plain_queries_per_s": round(len(qs) / tp, 2), "plain_embeddings_per_s": round(emb / tp, 1),
+           "batched_queries_per_s": round(len(qs) / tb, 2), "batched_embeddings_per_s": round(emb / tb, 1),
+           "speedup_vs_plain": round(tp / tb, 2), "counts_and_graph_ids_equal_plain": same,
+           "overflowed_to_plain": int(st["overflowed"]), "graph_launches": int(st["graph_launches"]),
+           "tune": tune}
+    # serving sweep over workload 3's database
+    serv = []
+    rng = np.random.default_rng(configs.seed(5))
+    for i, qps in enumerate(args.qps):
+        n = int(min(args.serve_requests, max(200, qps * 1.0)))
+        reqs = configs.workload5_queries(ds, n, seed_offset=i)
+        arr = np.cumsum(rng.exponential(1.0 / qps, size=n))
+        arr = (arr - arr[0]).tolist()
+        lb, cb, sb = eng.serve(reqs, arr, batched=True, mem_budget=args.mem_budget << 30)
+        lp, cp, sp = eng.serve(reqs, arr, batched=False)
+        serv.append({"offered_qps": qps, "requests": n,
+                     "batched": {"queries_per_s": round(n / sb["wall_s"], 1), "p50_ms": round(pct(lb, 50), 3),
+                                 "p99_ms": round(pct(lb, 99), 3), "max_in_flight": int(sb["max_concurrent"]),
+                                 "iterations": int(sb["iterations"]), "to_plain": int(sb["overflowed"])},
+                     "plain": {"queries_per_s": round(n / sp["wall_s"], 1), "p50_ms": round(pct(lp, 50), 3),
+                               "p99_ms": round(pct(lp, 99), 3)},
+                     "counts_equal": cb == cp})
+        log(f"serve qps={qps}: {serv[-1]}")
+    out_s = {"workload": "workload5: workload3 database resident, Poisson arrivals, walk queries of 3-10 vertices",
+             "points": serv}
+    eng.close()
+    return out, out_s
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -60,7 +155,17 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--level-bytes", type=int, default=1 << 31)
+    ap.add_argument("--no-batch", action="store_true", help="skip the workload 3/5 sections")
+    ap.add_argument("--db-scale", type=float, default=1.0)
+    ap.add_argument("--batch-reps", type=int, default=2)
+    ap.add_argument("--qps", type=float, nargs="*", default=[200, 1000, 5000])
+    ap.add_argument("--serve-requests", type=int, default=1500)
+    ap.add_argument("--mem-budget", type=int, default=32, help="GiB of estimated tables in flight")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run the copy kernel on MIB mebibytes (PMC scaling)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        return copy_calibration(args.copy_calibration)
     if args.gpus != 1:
         log("bench.py measures one GPU: the embedding-extension path here is built and measured "
             "on a single MI355X (multi-GPU sharding is a design note, not part of this build).")
@@ -131,6 +236,11 @@ def main():
                 "alg_bytes_per_launch": req // max(launches, 1)}
     kernels = {k: {"launches": v[0], "ms": round(v[1], 3), "req_bytes": v[2]} for k, v in prof.items()}
 
+    eng.close()
+    batched = served = None
+    if not args.no_batch:
+        batched, served = batched_and_served(args)
+
     cpu = None
     if not args.no_cpu:
         threads = min(os.cpu_count() or 1, 16)
@@ -162,6 +272,8 @@ def main():
         "roofline": roofline,
         "kernels": kernels,
         "cpu_baseline": cpu,
+        "batched": batched,
+        "served": served,
     }
     print(json.dumps(line), flush=True)
 

@@ -47,7 +47,7 @@ def lib():
         "gh_ds_load": (vp, [cs, cs]),
         "gh_ds_from_arrays": (vp, [u32, _u32p(), u64, _u32p(), _u32p(), u32]),
         "gh_ds_free": (None, [vp]),
-        "gh_ds_add_walk_queries": (i32, [vp, u32, u32, u32, i32, u32, u64]),
+        "gh_ds_add_walk_queries": (i32, [vp, u32, u32, u32, i32, u32, u64, u32]),
         "gh_ds_add_shape_queries": (i32, [vp, u64]),
         "gh_ds_add_query": (i32, [vp, cs]),
         "gh_ds_nv": (u32, [vp]),
@@ -66,6 +66,9 @@ def lib():
         "gh_engine_graph_bytes": (u64, [vp]),
         "gh_engine_mine": (i32, [vp, cs, i32, C.POINTER(u64), C.POINTER(u64), C.POINTER(dbl),
                                  _u32p(), u32, C.POINTER(u64)]),
+        "gh_engine_mine_shard": (i32, [vp, cs, i32, u64, u64, u64, C.POINTER(u64), C.POINTER(u64),
+                                       C.POINTER(dbl), _u32p(), u32, C.POINTER(u64)]),
+        "gh_engine_count_by_rows": (i32, [vp, i32]),
         "gh_engine_batch_config": (i32, [vp, u32, u64, u64, u32, u32]),
         "gh_engine_batch_mine": (i32, [vp, C.POINTER(cs), u32, i32, C.POINTER(u64), C.POINTER(u64),
                                        _u32p(), u32, C.POINTER(dbl)]),
@@ -127,8 +130,9 @@ class Dataset:
         return Dataset(lib().gh_ds_from_arrays(len(vlabel) - 1, p(vlabel), len(edges), p(edges),
                                                p(go), ngraphs if go is not None else 0))
 
-    def add_walk_queries(self, nq, kmin, kmax, induced=False, min_extra=0, seed=0):
-        rc = lib().gh_ds_add_walk_queries(self._h, nq, kmin, kmax, int(induced), min_extra, seed)
+    def add_walk_queries(self, nq, kmin, kmax, induced=False, min_extra=0, seed=0, min_deg=0):
+        rc = lib().gh_ds_add_walk_queries(self._h, nq, kmin, kmax, int(induced), min_extra, seed,
+                                          min_deg)
         if rc:
             raise _err("add_walk_queries", rc)
         return self
@@ -236,7 +240,7 @@ class Engine:
         return lib().gh_engine_graph_bytes(self._h)
 
     KERNELS = {0: "k_size", 1: "k_scan", 2: "k_walk<extend>", 3: "k_walk<count>",
-               4: "batch_iteration"}
+               4: "batch_iteration", 5: "k_count_rows"}
 
     def batch_config(self, slots=0, slot_bytes=0, row_cap=0, ipt=0, blocks_per_cu=0):
         """Configure the batch engine (0 keeps a setting); rebuilt on next use."""
@@ -261,9 +265,10 @@ class Engine:
                                         bits, nwords, st)
         if rc:
             raise _err("mine_batch", rc)
+        words = np.ctypeslib.as_array(bits)
         res = []
         for i in range(n):
-            w = list(bits)[i * nwords:(i + 1) * nwords]
+            w = words[i * nwords:(i + 1) * nwords]
             res.append(MineResult(counts[i], ab[i], 0.0, _bits_to_ids(w, self.ngraphs), 0))
         stats = dict(zip(("batched", "overflowed", "graph_launches", "device_ms", "wall_ms"), list(st)))
         return res, stats
@@ -290,6 +295,12 @@ class Engine:
         stats = dict(zip(("wall_s", "iterations", "max_concurrent", "overflowed"), list(st)))
         return list(lat)[:n], list(cnt)[:n], stats
 
+    def count_by_rows(self, on=True):
+        """Count single-back-edge last steps per row (default) or by a candidate walk."""
+        rc = lib().gh_engine_count_by_rows(self._h, int(on))
+        if rc:
+            raise _err("count_by_rows", rc)
+
     def profile(self, on=True):
         rc = lib().gh_engine_profile(self._h, int(on))
         if rc:
@@ -307,13 +318,18 @@ class Engine:
             out[name] = (n.value, ms.value, rq.value)
         return out
 
-    def mine(self, query_text, single_v=False):
+    def mine(self, query_text, single_v=False, start_begin=0, start_end=0, stride=1):
+        """Mine one query on the plain path.  start_begin/start_end/stride pick a
+        shard of the first vertex's candidates (indices into the id-sorted list
+        of vertices with its label; start_end 0 = all); shard counts of a
+        partition add up to the whole count."""
         count, ab, launches = C.c_uint64(0), C.c_uint64(0), C.c_uint64(0)
         ms = C.c_double(0)
         nwords = (self.ngraphs + 31) // 32 if self.ngraphs > 1 else 0
         bits = (C.c_uint32 * max(nwords, 1))()
-        rc = lib().gh_engine_mine(self._h, query_text.encode(), int(single_v), C.byref(count),
-                                  C.byref(ab), C.byref(ms), bits, nwords, C.byref(launches))
+        rc = lib().gh_engine_mine_shard(self._h, query_text.encode(), int(single_v), start_begin,
+                                        start_end, stride, C.byref(count), C.byref(ab),
+                                        C.byref(ms), bits, nwords, C.byref(launches))
         if rc:
             raise _err("mine", rc)
         return MineResult(count.value, ab.value, ms.value,

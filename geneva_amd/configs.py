@@ -14,7 +14,9 @@ deterministic one, so a (workload, scale) pair fixes the bytes.
      of 4-8 vertices.
   4  Chung-Lu graph, exponent 1.9, 4M vertices, mean degree 30, largest
      weight 1e5; 64 labels Zipf(1.0); 64 induced random-walk queries of 6-8
-     vertices with at least two independent cycles.
+     vertices with at least two independent cycles and every query vertex on a
+     cycle (a pendant vertex hung on a hub of degree ~5e4 gives a single query
+     ~1e15 embeddings, beyond any enumerator).
   5  workload 3's database with a Poisson request stream of random-walk
      queries of 3-10 vertices.
 """
@@ -47,7 +49,8 @@ def workload4(scale=1.0, nqueries=64):
     nv = max(2000, int(4_000_000 * scale))
     wmax = max(50.0, 1e5 * scale)
     ds = Dataset.chung_lu(nv, 1.9, 30.0, wmax, 64, 1.0, seed(4))
-    return ds.add_walk_queries(nqueries, 6, 8, induced=True, min_extra=2, seed=seed(4) + 100)
+    return ds.add_walk_queries(nqueries, 6, 8, induced=True, min_extra=2, seed=seed(4) + 100,
+                               min_deg=2)
 
 
 def workload5_queries(ds, n, seed_offset=0):

@@ -40,6 +40,10 @@ struct BState {              // dynamic, polled by the host (mirrors gv_slot_inf
     unsigned long long out_rows;
     unsigned long long overflow;
     uint64_t row_end;        // end of this slot's rows in the iteration row space
+    uint64_t cand_base;      // first candidate of this slot in the iteration's space
+    uint64_t cand_end;
+    uint64_t tile_end;       // end of this slot's tiles (tiles never straddle slots)
+    uint64_t rtile_end;      // end of this slot's sizing tiles (kBlock rows each)
 };
 
 struct BArgs {
@@ -57,7 +61,8 @@ struct BArgs {
     uint64_t* csum;
     uint64_t row_cap;
     uint64_t* tiles;
-    unsigned long long* glob;   // [0] rows, [1] candidates
+    unsigned long long* glob;   // [0] rows, [1] candidates, [2] tiles, [3] sizing tiles
+    uint64_t tile;              // candidates per walk tile (64 * IPT)
     uint32_t* gbits;            // nslots * gwords
     uint32_t gwords;
 };
@@ -142,31 +147,49 @@ __global__ __launch_bounds__(1024) void k_b_prep(BArgs a) {
         mode = 0;
     }
     ex = block_scan_1024(rows, wsum, tot);
+    const uint64_t total_rows = tot;
+    // sizing tiles: kBlock rows of one slot each
+    const uint64_t nrt = (rows + kBlock - 1) / kBlock;
+    uint64_t rtot;
+    const uint64_t rex = block_scan_1024(nrt, wsum, rtot);
     if (s < a.nslots) {
         a.st[s].mode = mode;
         a.st[s].row_end = ex + rows;
+        a.st[s].rtile_end = rex + nrt;
     }
     if (s == 0) {
-        a.glob[0] = tot;
+        a.glob[0] = total_rows;
         a.glob[1] = 0;
+        a.glob[3] = rtot;
     }
 }
 
 __global__ __launch_bounds__(kBlock) void k_b_size(BArgs a) {
-    __shared__ uint64_t row_end[kMaxSlots];
-    for (uint32_t i = threadIdx.x; i < a.nslots; i += blockDim.x) row_end[i] = a.st[i].row_end;
+    __shared__ uint64_t rtile_end[kMaxSlots];
+    for (uint32_t i = threadIdx.x; i < a.nslots; i += blockDim.x) rtile_end[i] = a.st[i].rtile_end;
     __syncthreads();
-    const uint64_t nrows = a.glob[0];
-    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    const uint64_t ntiles = a.glob[3];
+    const unsigned lane = __lane_id();
     uint32_t cur = 0xffffffffu;
     uint64_t acc = 0;
-    for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r < nrows; r += stride) {
-        const uint32_t s = slot_of(row_end, a.nslots, r);
+    for (uint64_t RT = blockIdx.x; RT < ntiles; RT += gridDim.x) {
+        // every row of this tile belongs to one slot: its step is read once
+        const uint32_t s = __builtin_amdgcn_readfirstlane(slot_of(rtile_end, a.nslots, RT));
+        if (s != cur) {
+            const uint64_t w = wave_sum_u64(acc);
+            if (lane == 0 && w) atomicAdd(&a.st[cur].bytes, (unsigned long long)w);
+            cur = s;
+            acc = 0;
+        }
         const BState& S = a.st[s];
         const BDesc& D = a.desc[s];
         const uint32_t L = S.width;
+        const uint64_t rows = L == 1 ? D.start_n : S.rows;
+        const uint64_t nrt = (rows + kBlock - 1) / kBlock;
+        const uint64_t local = (RT - (rtile_end[s] - nrt)) * kBlock + threadIdx.x;
+        if (local >= rows) continue;
+        const uint64_t r = S.row_end - rows + local;
         const gv_step_t& st = D.q.step[L];
-        const uint64_t local = r - (row_end[s] - (S.width == 1 ? D.start_n : S.rows));
         const uint32_t* row = slot_rows(a, s, S, D) + local * L;
         uint32_t best_b = 0, best_len = 0xffffffffu, best_i = 0;
         uint64_t by = 4ull * L;
@@ -183,16 +206,14 @@ __global__ __launch_bounds__(kBlock) void k_b_size(BArgs a) {
         }
         const uint32_t bound = st.restrict_col != GV_NONE ? row[st.restrict_col] : 0xffffffffu;
         a.anc_beg[r] = best_b;
-        a.cnt[r] = lower_bound_u32(a.g.nbr, best_b, best_b + best_len, bound) - best_b;
+        a.cnt[r] = bound == 0xffffffffu
+                       ? best_len
+                       : lower_bound_u32(a.g.nbr, best_b, best_b + best_len, bound) - best_b;
         a.anc_i[r] = (uint8_t)best_i;
-        if (s != cur) {
-            if (acc) atomicAdd(&a.st[cur].bytes, (unsigned long long)acc);
-            cur = s;
-            acc = 0;
-        }
         acc += by;
     }
-    if (acc) atomicAdd(&a.st[cur].bytes, (unsigned long long)acc);
+    const uint64_t w = wave_sum_u64(acc);
+    if (lane == 0 && w && cur != 0xffffffffu) atomicAdd(&a.st[cur].bytes, (unsigned long long)w);
 }
 
 // ---- scan with the extent in device memory --------------------------------
@@ -279,138 +300,90 @@ __global__ __launch_bounds__(kBlock) void k_b_scan_final(BArgs a) {
 
 // ---- walk -------------------------------------------------------------------
 
-__device__ __forceinline__ bool b_check(const DevGraph& g, const gv_step_t& st, const uint32_t* row,
-                                        uint32_t v, uint32_t skip) {
-    for (uint32_t j = 0; j < st.ncoll; ++j)
-        if (row[st.coll_col[j]] == v) return false;
-    for (uint32_t i = 0; i < st.nback; ++i) {
-        if (i == skip) continue;
-        uint32_t b, e;
-        seg(g, st.back_part[i], row[st.back_col[i]], st.label, b, e);
-        if (!contains_u32(g.nbr, b, e, v)) return false;
+// Lay the walk out in tiles that never straddle two slots: per slot, its
+// candidate range from the scan and ceil(range / tile) tiles.
+__global__ __launch_bounds__(1024) void k_b_tiles(BArgs a) {
+    __shared__ uint64_t wsum[16];
+    const uint32_t s = threadIdx.x;
+    uint64_t nt = 0, cb = 0, ce = 0;
+    if (s < a.nslots) {
+        const BState& S = a.st[s];
+        if (S.mode != 0) {
+            const uint64_t rows = S.width == 1 ? a.desc[s].start_n : S.rows;
+            const uint64_t re = S.row_end, rb = re - rows;
+            cb = rb ? a.csum[rb - 1] : 0;
+            ce = a.csum[re - 1];
+            nt = (ce - cb + a.tile - 1) / a.tile;
+        }
     }
-    return true;
+    uint64_t tot;
+    const uint64_t ex = block_scan_1024(nt, wsum, tot);
+    if (s < a.nslots) {
+        a.st[s].cand_base = cb;
+        a.st[s].cand_end = ce;
+        a.st[s].tile_end = ex + nt;
+    }
+    if (s == 0) a.glob[2] = tot;
 }
 
 template <int IPT>
 __global__ __launch_bounds__(kBlock) void k_b_walk(BArgs a) {
-    __shared__ uint64_t row_end[kMaxSlots];
+    __shared__ uint64_t tile_end[kMaxSlots];
     __shared__ uint32_t stage[kWavesPerBlock][64 * GV_MAXK];
-    for (uint32_t i = threadIdx.x; i < a.nslots; i += blockDim.x) row_end[i] = a.st[i].row_end;
+    for (uint32_t i = threadIdx.x; i < a.nslots; i += blockDim.x) tile_end[i] = a.st[i].tile_end;
     __syncthreads();
-    const uint64_t nrows = a.glob[0];
-    const uint64_t total = a.glob[1];
-    const uint64_t tile = 64ull * IPT;
+    const uint64_t ntiles = a.glob[2];
     const unsigned lane = __lane_id(), wib = threadIdx.x >> 6;
     const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
-    // per-lane accumulators of the slot the lane last worked for, flushed when
-    // the lane moves to another slot and at the end
-    uint32_t cur = 0xffffffffu, last_gid = 0xffffffffu;
-    uint64_t acc_count = 0, acc_bytes = 0;
-    auto switch_to = [&](uint32_t s) {
-        if (s == cur) return;
-        if (cur != 0xffffffffu) {
-            if (acc_count) atomicAdd(&a.st[cur].count, (unsigned long long)acc_count);
-            if (acc_bytes) atomicAdd(&a.st[cur].bytes, (unsigned long long)acc_bytes);
+    uint32_t cur = 0xffffffffu;
+    TileAcc acc;
+    auto flush = [&]() {
+        if (cur == 0xffffffffu) return;
+        const uint64_t c = wave_sum_u64(acc.count), b = wave_sum_u64(acc.bytes);
+        if (lane == 0) {
+            if (c) atomicAdd(&a.st[cur].count, (unsigned long long)c);
+            if (b) atomicAdd(&a.st[cur].bytes, (unsigned long long)b);
         }
-        cur = s;
-        acc_count = acc_bytes = 0;
-        last_gid = 0xffffffffu;
+        acc = TileAcc();
     };
-    for (uint64_t t0 = wave * tile; t0 < total; t0 += nwaves * tile) {
-        const uint64_t t1 = t0 + tile < total ? t0 + tile : total;
-        uint64_t r_lo = 0, r_hi = 0;
-        if (lane == 0) r_lo = find_row(a.csum, 0, nrows, t0);
-        if (lane == 1) r_hi = find_row(a.csum, 0, nrows, t1 - 1);
-        r_lo = __shfl(r_lo, 0, 64);
-        r_hi = __shfl(r_hi, 1, 64);
-        for (int it = 0; it < IPT; ++it) {
-            const uint64_t c = t0 + (uint64_t)it * 64 + lane;
-            const bool valid = c < t1;
-            bool ok = false;
-            uint32_t v = 0, s = 0, mode = 0, W = 0, L = 0;
-            const uint32_t* row = nullptr;
-            const BDesc* D = nullptr;
-            if (valid) {
-                const uint64_t r = find_row(a.csum, r_lo, r_hi + 1, c);
-                s = slot_of(row_end, a.nslots, r);
-                const BState& S = a.st[s];
-                D = &a.desc[s];
-                L = S.width;
-                W = L + 1;
-                mode = S.mode;
-                const uint64_t local = r - (row_end[s] - (L == 1 ? D->start_n : S.rows));
-                row = slot_rows(a, s, S, *D) + local * L;
-                v = a.g.nbr[a.anc_beg[r] + (uint32_t)(c - (a.csum[r] - a.cnt[r]))];
-                ok = b_check(a.g, D->q.step[L], row, v, a.anc_i[r]);
-            }
-            if (valid && mode >= 2) {
-                switch_to(s);
-                uint64_t n = ok ? 1 : 0;
-                if (ok && mode == 3) {
-                    const gv_step_t& tl = D->q.step[L + 1];
-                    const uint32_t bc = tl.back_col[0];
-                    const uint32_t w = bc == L ? v : row[bc];
-                    uint32_t b, e;
-                    seg(a.g, tl.back_part[0], w, tl.label, b, e);
-                    uint32_t bound = 0xffffffffu;
-                    if (tl.restrict_col != GV_NONE) bound = tl.restrict_col == L ? v : row[tl.restrict_col];
-                    const uint32_t cut = lower_bound_u32(a.g.nbr, b, e, bound);
-                    n = cut - b;
-                    for (uint32_t j = 0; j < tl.ncoll; ++j) {
-                        const uint32_t x = tl.coll_col[j] == L ? v : row[tl.coll_col[j]];
-                        if (x < bound && contains_u32(a.g.nbr, b, cut, x)) --n;
-                    }
-                    acc_bytes += 8ull * W + 8ull + 4ull * (e - b);
-                }
-                acc_count += n;
-                if (n && a.g.gof) {
-                    const uint32_t gid = a.g.gof[row[0]];
-                    if (gid != last_gid) {
-                        atomicOr(&a.gbits[(uint64_t)s * a.gwords + (gid >> 5)], 1u << (gid & 31));
-                        last_gid = gid;
-                    }
-                }
-            }
-            // extension: survivors of each slot present in the wave, one group at a time
-            const bool surv = valid && mode == 1 && ok;
-            if (surv) {
-                switch_to(s);
-                acc_bytes += 4ull * W;
-            }
-            uint64_t pend = __ballot(surv);
-            while (pend) {
-                const int leader = __ffsll((long long)pend) - 1;
-                const uint32_t ls = __shfl(s, leader, 64);
-                const uint64_t mask = __ballot(surv && s == ls);
-                pend &= ~mask;
-                const uint32_t n = (uint32_t)__popcll(mask);
-                const uint32_t Wg = __shfl(W, leader, 64);
-                uint64_t base = 0;
-                if (lane == (unsigned)leader) base = atomicAdd(&a.st[ls].out_rows, (unsigned long long)n);
-                base = __shfl(base, leader, 64);
-                const uint64_t cap_rows = a.slot_uints / Wg;
-                if (base + n > cap_rows) {
-                    if (lane == (unsigned)leader) atomicOr(&a.st[ls].overflow, 1ull);
-                    continue;
-                }
-                uint32_t* stg = stage[wib];
-                if ((mask >> lane) & 1ull) {
-                    const uint32_t p = lanes_below(mask) * Wg;
-                    for (uint32_t j = 0; j + 1 < Wg; ++j) stg[p + j] = row[j];
-                    stg[p + Wg - 1] = v;
-                }
-                __builtin_amdgcn_wave_barrier();
-                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-                uint32_t* dst = slot_out(a, ls, a.st[ls]) + base * Wg;
-                for (uint32_t j = lane; j < n * Wg; j += 64) dst[j] = stg[j];
-                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-                __builtin_amdgcn_wave_barrier();
-            }
+    for (uint64_t T = wave; T < ntiles; T += nwaves) {
+        // the tile's slot, made wave-uniform so its descriptor is read once
+        const uint32_t s = __builtin_amdgcn_readfirstlane(slot_of(tile_end, a.nslots, T));
+        if (s != cur) {
+            flush();
+            cur = s;
+        }
+        const BState& S = a.st[s];
+        const BDesc& D = a.desc[s];
+        const uint32_t L = S.width;
+        const uint64_t rows = L == 1 ? D.start_n : S.rows;
+        const uint64_t ntile_s = (S.cand_end - S.cand_base + a.tile - 1) / a.tile;
+        const uint64_t k = T - (tile_end[s] - ntile_s);
+        const uint64_t t0 = S.cand_base + k * a.tile;
+        const uint64_t t1 = t0 + a.tile < S.cand_end ? t0 + a.tile : S.cand_end;
+        TileIn in;
+        in.rows = slot_rows(a, s, S, D);
+        in.row_base = S.row_end - rows;
+        in.L = L;
+        in.csum = a.csum;
+        in.cnt = a.cnt;
+        in.anc_beg = a.anc_beg;
+        in.anc_i = a.anc_i;
+        in.row_lo = S.row_end - rows;
+        in.row_hi = S.row_end;
+        uint32_t* gb = a.g.gof ? a.gbits + (uint64_t)s * a.gwords : nullptr;
+        const gv_step_t& st = D.q.step[L];
+        if (S.mode == 1) {
+            TileOut o{slot_out(a, s, S), a.slot_uints / (L + 1), &a.st[s].out_rows, &a.st[s].overflow};
+            walk_tile<false, IPT>(a.g, st, st, false, in, t0, t1, o, stage[wib], gb, acc);
+        } else {
+            TileOut o{nullptr, 0, nullptr, nullptr};
+            const gv_step_t& tl = D.q.step[L + 1 < GV_MAXK ? L + 1 : GV_MAXK - 1];
+            walk_tile<true, IPT>(a.g, st, tl, S.mode == 3, in, t0, t1, o, nullptr, gb, acc);
         }
     }
-    switch_to(0xffffffffu);
+    flush();
 }
 
 __global__ __launch_bounds__(1024) void k_b_post(BArgs a) {
@@ -485,6 +458,7 @@ BArgs bargs(gv_batch* b) {
     a.glob = b->d_glob;
     a.gbits = b->d_gbits;
     a.gwords = b->gwords;
+    a.tile = 64ull * b->tune.ipt;
     return a;
 }
 
@@ -508,6 +482,7 @@ int capture(gv_batch* b) {
     hipLaunchKernelGGL(k_b_scan_reduce, dim3(scan_blocks), dim3(kBlock), 0, c->stream, a);
     hipLaunchKernelGGL(k_b_scan_tiles, dim3(1), dim3(kBlock), 0, c->stream, a);
     hipLaunchKernelGGL(k_b_scan_final, dim3(scan_blocks), dim3(kBlock), 0, c->stream, a);
+    hipLaunchKernelGGL(k_b_tiles, dim3(1), dim3(1024), 0, c->stream, a);
     switch (b->tune.ipt) {
         case 1: hipLaunchKernelGGL(k_b_walk<1>, dim3(walk_blocks), dim3(kBlock), 0, c->stream, a); break;
         case 2: hipLaunchKernelGGL(k_b_walk<2>, dim3(walk_blocks), dim3(kBlock), 0, c->stream, a); break;

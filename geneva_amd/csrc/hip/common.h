@@ -89,10 +89,122 @@ __device__ __forceinline__ uint64_t find_row(const uint64_t* csum, uint64_t lo, 
     return lo;
 }
 
+// Count of the last vertex when it has a single back edge (step `t`, rows of
+// width L extended by candidate v in column L): the label segment of its
+// neighbour, cut below the restriction bound, minus mapped vertices inside.
+// The part that does not involve v is cached per row: for most last steps
+// (the neighbour and the bound are earlier columns) every candidate of a row
+// shares it, and only v's own collision is checked per candidate.
+// Per-lane cache of the fused last step's row-dependent part.  The last query
+// vertex u has one back edge, to column back_col[0] of the row (or to the
+// candidate v of the step being walked, column L), an optional restriction
+// f(u) < f(row[restrict_col]) and same-label columns it must differ from.
+struct TailCache {
+    uint64_t r = ~0ull;
+    uint32_t b = 0, e = 0, cut = 0, bound = 0;
+    uint64_t cnt = 0;
+    // bound = v: same-label row values that lie in the segment (else ~0u),
+    // so that the count below v is the rank of v minus those below it
+    uint32_t x0 = ~0u, x1 = ~0u, x2 = ~0u;
+};
+
 // entries a binary search over n sorted entries reads: floor(log2 n) + 1
 __device__ __forceinline__ uint32_t probes(uint32_t n) { return n ? 32u - __clz(n) : 0u; }
 
+// Count of the last vertex's candidates for one survivor v (at row r) of the
+// second-to-last step.  same_seg: the last vertex's segment is the one v was
+// drawn from (same anchor column, label and partition), and v sits at index
+// idx of it, so the rank of v needs no search.
+__device__ __forceinline__ uint64_t tail_count(const DevGraph& g, const gv_step_t& t, uint32_t L,
+                                               const uint32_t* row, uint32_t v, uint64_t r,
+                                               bool same_seg, uint32_t idx, TailCache& tc,
+                                               uint64_t& rq) {
+    const bool segdep = t.back_col[0] == L;
+    const bool bnddep = t.restrict_col == L;
+    uint32_t nx = 0;   // same-label columns other than L
+    for (uint32_t j = 0; j < t.ncoll; ++j) nx += t.coll_col[j] != L;
+    if (segdep || (bnddep && nx > 3)) {
+        // everything depends on v: segment of v, cut, collisions, per candidate
+        const uint32_t w = segdep ? v : row[t.back_col[0]];
+        uint32_t b, e;
+        seg(g, t.back_part[0], w, t.label, b, e);
+        const uint32_t bound =
+            t.restrict_col == GV_NONE ? 0xffffffffu : (bnddep ? v : row[t.restrict_col]);
+        const uint32_t cut = bound == 0xffffffffu ? e : lower_bound_u32(g.nbr, b, e, bound);
+        rq += 8ull + (bound != 0xffffffffu ? 4ull * probes(e - b) : 0ull);
+        uint64_t n = cut - b;
+        for (uint32_t j = 0; j < t.ncoll; ++j) {
+            const uint32_t x = t.coll_col[j] == L ? v : row[t.coll_col[j]];
+            if (x < bound) {
+                rq += 4ull * probes(cut - b);
+                if (contains_u32(g.nbr, b, cut, x)) --n;
+            }
+        }
+        tc.b = b;
+        tc.e = e;
+        tc.r = ~0ull;
+        return n;
+    }
+    if (r != tc.r) {
+        seg(g, t.back_part[0], row[t.back_col[0]], t.label, tc.b, tc.e);
+        rq += 8ull;
+        if (!bnddep) {
+            tc.bound = t.restrict_col == GV_NONE ? 0xffffffffu : row[t.restrict_col];
+            // ids are below 0xffffffff, so an absent bound cuts nothing
+            tc.cut = tc.bound == 0xffffffffu ? tc.e : lower_bound_u32(g.nbr, tc.b, tc.e, tc.bound);
+            rq += tc.bound != 0xffffffffu ? 4ull * probes(tc.e - tc.b) : 0ull;
+            tc.cnt = tc.cut - tc.b;
+            for (uint32_t j = 0; j < t.ncoll; ++j) {
+                if (t.coll_col[j] == L) continue;
+                const uint32_t x = row[t.coll_col[j]];
+                if (x < tc.bound) {
+                    rq += 4ull * probes(tc.cut - tc.b);
+                    if (contains_u32(g.nbr, tc.b, tc.cut, x)) --tc.cnt;
+                }
+            }
+        } else {
+            uint32_t k = 0;
+            tc.x0 = tc.x1 = tc.x2 = ~0u;
+            for (uint32_t j = 0; j < t.ncoll; ++j) {
+                if (t.coll_col[j] == L) continue;
+                const uint32_t x = row[t.coll_col[j]];
+                rq += 4ull * probes(tc.e - tc.b);
+                const uint32_t y = contains_u32(g.nbr, tc.b, tc.e, x) ? x : ~0u;
+                if (k == 0) tc.x0 = y;
+                else if (k == 1) tc.x1 = y;
+                else tc.x2 = y;
+                ++k;
+            }
+        }
+        tc.r = r;
+    }
+    if (!bnddep) {
+        uint64_t n = tc.cnt;
+        // v itself is excluded where u shares its label (v is in the segment
+        // when it was drawn from it)
+        for (uint32_t j = 0; j < t.ncoll; ++j) {
+            if (t.coll_col[j] != L || v >= tc.bound) continue;
+            if (!same_seg) rq += 4ull * probes(tc.cut - tc.b);
+            if (same_seg || contains_u32(g.nbr, tc.b, tc.cut, v)) --n;
+        }
+        return n;
+    }
+    // bound = v: candidates of u are the segment entries below v (v itself is
+    // never below v), minus the same-label row values among them
+    uint32_t cut;
+    if (same_seg) {
+        cut = tc.b + idx;
+    } else {
+        cut = lower_bound_u32(g.nbr, tc.b, tc.e, v);
+        rq += 4ull * probes(tc.e - tc.b);
+    }
+    return (uint64_t)(cut - tc.b) - (tc.x0 < v) - (tc.x1 < v) - (tc.x2 < v);
+}
+
+
 __device__ __forceinline__ unsigned lane_id() { return __lane_id(); }
+
+
 
 __device__ __forceinline__ unsigned lanes_below(uint64_t mask) {
     return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
@@ -103,6 +215,210 @@ __device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
     return v;
+}
+
+
+// ---- one tile of the candidate walk, shared by the plain and batch kernels --
+//
+// A tile is up to 64*IPT consecutive candidate indices of one extension step.
+// When the tile lies in a single row (hub rows: almost always), the row's
+// data is loaded once and held in scalars (up to 3 same-label values and 2
+// non-anchor neighbour segments) and the lanes stream consecutive anchor
+// entries; otherwise each lane finds its row by binary search in the scan.
+
+struct TileIn {
+    const uint32_t* rows;    // table of the step's input rows (row r at rows + (r - row_base) * L)
+    uint64_t row_base;
+    uint32_t L;
+    const uint64_t* csum;    // inclusive candidate prefix per row
+    const uint32_t* cnt;
+    const uint32_t* anc_beg;
+    const uint8_t* anc_i;
+    uint64_t row_lo, row_hi; // rows searched for the tile's candidates
+};
+
+struct TileOut {             // extension output
+    uint32_t* out;
+    uint64_t cap;            // rows
+    unsigned long long* out_n;
+    unsigned long long* overflow;
+};
+
+struct TileAcc {
+    uint64_t count = 0, bytes = 0, rq = 0;
+    uint32_t last_gid = 0xffffffffu;
+    TailCache tc;
+};
+
+template <bool kCount, int IPT>
+__device__ __forceinline__ void walk_tile(const DevGraph& g, const gv_step_t& st,
+                                          const gv_step_t& tail, bool has_tail, const TileIn& in,
+                                          uint64_t t0, uint64_t t1, const TileOut& o,
+                                          uint32_t* stage, uint32_t* gbits, TileAcc& acc) {
+    const unsigned lane = __lane_id();
+    const uint32_t L = in.L, W = L + 1;
+    uint64_t r_lo = 0, r_hi = 0;
+    if (lane == 0) r_lo = find_row(in.csum, in.row_lo, in.row_hi, t0);
+    if (lane == 1) r_hi = find_row(in.csum, in.row_lo, in.row_hi, t1 - 1);
+    r_lo = __shfl(r_lo, 0, 64);
+    r_hi = __shfl(r_hi, 1, 64);
+    auto rowp = [&](uint64_t r) { return in.rows + (r - in.row_base) * L; };
+
+    // anchors (back-edge indices) whose segment is also the fused last
+    // vertex's segment
+    uint32_t smask = 0;
+    if (kCount && has_tail && tail.label == st.label && tail.back_col[0] != L) {
+        for (uint32_t i = 0; i < st.nback; ++i)
+            if (st.back_col[i] == tail.back_col[0] && st.back_part[i] == tail.back_part[0])
+                smask |= 1u << i;
+    }
+
+    // ---- checked candidates of the tile: ok[it], v[it], r[it], and for the
+    // fused last step the index of v in the segment when it is shared (else ~0u)
+    bool okk[IPT];
+    uint32_t vv[IPT];
+    uint64_t rr[IPT];
+    uint32_t six[IPT];
+    if (r_lo == r_hi && st.ncoll <= 3 && st.nback <= 3) {
+        // one row: its data is loaded once and kept in scalars; ids are below
+        // 0xffffffff, so an unused same-label slot never matches
+        const uint64_t r = r_lo;
+        const uint64_t first = in.csum[r] - in.cnt[r];
+        const uint32_t abeg = in.anc_beg[r];
+        const uint32_t skip = in.anc_i[r];
+        const uint32_t* row = rowp(r);
+        const uint32_t cv0 = st.ncoll > 0 ? row[st.coll_col[0]] : 0xffffffffu;
+        const uint32_t cv1 = st.ncoll > 1 ? row[st.coll_col[1]] : 0xffffffffu;
+        const uint32_t cv2 = st.ncoll > 2 ? row[st.coll_col[2]] : 0xffffffffu;
+        const bool shared = (smask >> skip) & 1u;
+        uint32_t ob0 = 0, oe0 = 0, ob1 = 0, oe1 = 0, nob = 0;
+        for (uint32_t i = 0; i < st.nback; ++i) {
+            if (i == skip) continue;
+            uint32_t b, e;
+            seg(g, st.back_part[i], row[st.back_col[i]], st.label, b, e);
+            if (nob == 0) {
+                ob0 = b;
+                oe0 = e;
+            } else {
+                ob1 = b;
+                oe1 = e;
+            }
+            ++nob;
+        }
+        if (lane == 0) acc.rq += 4ull * L + 17ull + 8ull * nob;
+#pragma unroll
+        for (int it = 0; it < IPT; ++it) {
+            const uint64_t c = t0 + (uint64_t)it * 64 + lane;
+            uint32_t v = 0;
+            bool ok = false;
+            if (c < t1) {
+                v = g.nbr[abeg + (uint32_t)(c - first)];
+                acc.rq += 4;
+                ok = v != cv0 && v != cv1 && v != cv2;
+                if (ok && nob > 0) {
+                    acc.rq += 4ull * probes(oe0 - ob0);
+                    ok = contains_u32(g.nbr, ob0, oe0, v);
+                }
+                if (ok && nob > 1) {
+                    acc.rq += 4ull * probes(oe1 - ob1);
+                    ok = contains_u32(g.nbr, ob1, oe1, v);
+                }
+            }
+            okk[it] = ok;
+            vv[it] = v;
+            rr[it] = r;
+            six[it] = shared ? (uint32_t)(c - first) : ~0u;
+        }
+    } else {
+#pragma unroll
+        for (int it = 0; it < IPT; ++it) {
+            const uint64_t c = t0 + (uint64_t)it * 64 + lane;
+            bool ok = false;
+            uint32_t v = 0, sx = ~0u;
+            uint64_t r = r_lo;
+            if (c < t1) {
+                r = find_row(in.csum, r_lo, r_hi + 1, c);
+                const uint64_t first = in.csum[r] - in.cnt[r];
+                v = g.nbr[in.anc_beg[r] + (uint32_t)(c - first)];
+                acc.rq += 4;
+                const uint32_t* row = rowp(r);
+                ok = true;
+                for (uint32_t j = 0; j < st.ncoll && ok; ++j) ok = row[st.coll_col[j]] != v;
+                const uint32_t skip = in.anc_i[r];
+                if ((smask >> skip) & 1u) sx = (uint32_t)(c - first);
+                for (uint32_t i = 0; i < st.nback && ok; ++i) {
+                    if (i == skip) continue;
+                    uint32_t b, e;
+                    seg(g, st.back_part[i], row[st.back_col[i]], st.label, b, e);
+                    acc.rq += 8ull + 4ull * probes(e - b);
+                    ok = contains_u32(g.nbr, b, e, v);
+                }
+            }
+            // the row and its sizing entries count once per run of lanes sharing it
+            const uint64_t prev_r = __shfl_up(r, 1, 64);
+            if (c < t1 && (lane == 0 || prev_r != r)) acc.rq += 4ull * L + 17ull;
+            okk[it] = ok;
+            vv[it] = v;
+            rr[it] = r;
+            six[it] = sx;
+        }
+    }
+
+    if (kCount) {
+#pragma unroll
+        for (int it = 0; it < IPT; ++it) {
+            uint64_t n = okk[it] ? 1 : 0;
+            const uint32_t* row = rowp(rr[it]);
+            if (okk[it] && has_tail) {
+                n = tail_count(g, tail, L, row, vv[it], rr[it], six[it] != ~0u, six[it], acc.tc,
+                               acc.rq);
+                acc.bytes += 8ull * W + 8ull + 4ull * (acc.tc.e - acc.tc.b);
+            }
+            acc.count += n;
+            if (n && gbits) {
+                const uint32_t gid = g.gof[row[0]];
+                if (gid != acc.last_gid) {
+                    atomicOr(&gbits[gid >> 5], 1u << (gid & 31));
+                    acc.last_gid = gid;
+                }
+            }
+        }
+        return;
+    }
+    // extension: one reservation for the whole tile, then each group of 64
+    // candidates stages its survivors in LDS and writes them contiguously
+    uint32_t tot = 0;
+#pragma unroll
+    for (int it = 0; it < IPT; ++it) tot += (uint32_t)__popcll(__ballot(okk[it]));
+    if (tot == 0) return;
+    uint64_t base = 0;
+    if (lane == 0) base = atomicAdd(o.out_n, (unsigned long long)tot);
+    base = __shfl(base, 0, 64);
+    if (base + tot > o.cap) {
+        if (lane == 0) atomicOr(o.overflow, 1ull);
+        return;
+    }
+#pragma unroll
+    for (int it = 0; it < IPT; ++it) {
+        const uint64_t mask = __ballot(okk[it]);
+        const uint32_t n = (uint32_t)__popcll(mask);
+        if (n == 0) continue;
+        if (okk[it]) {
+            acc.bytes += 4ull * W;
+            acc.rq += 4ull * W;
+            const uint32_t* row = rowp(rr[it]);
+            const uint32_t p = lanes_below(mask) * W;
+            for (uint32_t j = 0; j < L; ++j) stage[p + j] = row[j];
+            stage[p + L] = vv[it];
+        }
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        uint32_t* dst = o.out + base * W;
+        for (uint32_t j = lane; j < n * W; j += 64) dst[j] = stage[j];
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        base += n;
+    }
 }
 
 }  // namespace
@@ -176,7 +492,8 @@ DevGraph dev_view(const gv_graph* g) {
     return d;
 }
 
-enum { kKindSize = 0, kKindScan = 1, kKindExt = 2, kKindCount = 3, kKindBatch = 4, kKinds = 5 };
+enum { kKindSize = 0, kKindScan = 1, kKindExt = 2, kKindCount = 3, kKindBatch = 4, kKindRows = 5,
+       kKinds = 6 };
 
 hipEvent_t pool_get(gv_ctx* c) {
     if (!c->pool.empty()) {

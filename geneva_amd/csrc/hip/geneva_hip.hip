@@ -60,14 +60,103 @@ __global__ __launch_bounds__(kBlock) void k_size(SizeArgs a) {
             }
         }
         uint32_t bound = a.st.restrict_col != GV_NONE ? row[a.st.restrict_col] : 0xffffffffu;
-        uint32_t cut = lower_bound_u32(a.g.nbr, best_b, best_b + best_len, bound) - best_b;
+        uint32_t cut = bound == 0xffffffffu
+                           ? best_len
+                           : lower_bound_u32(a.g.nbr, best_b, best_b + best_len, bound) - best_b;
         a.anc_beg[r] = best_b;
         a.cnt[r] = cut;
         a.anc_i[r] = (uint8_t)best_i;
-        rq += 4ull * a.L + 8ull * a.st.nback + 4ull * probes(best_len) + 9ull;
+        rq += 4ull * a.L + 8ull * a.st.nback + (bound != 0xffffffffu ? 4ull * probes(best_len) : 0ull) + 9ull;
     }
     local = wave_sum_u64(local);
     if (lane_id() == 0 && local) atomicAdd(a.bytes, (unsigned long long)local);
+    if (a.req) {
+        rq = wave_sum_u64(rq);
+        if (lane_id() == 0 && rq) atomicAdd(a.req, (unsigned long long)rq);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// count by rows: the last materialised step has a single back edge, so a row's
+// candidates are one label segment cut at the restriction, and its survivors
+// are that range minus the same-label values of the row found in it.  The
+// count needs no walk over candidates:
+//   mode 0 (last step):            s
+//   mode 1 (tail independent of v): s * T, T = the last vertex's count for the row
+//   mode 2 (tail = a second vertex below v in the same segment, same label):
+//                                   s (s - 1) / 2, the ordered pairs of survivors
+// (s = survivors of the row).
+struct RowsArgs {
+    DevGraph g;
+    const uint32_t* rows;
+    uint64_t nrows;
+    uint32_t L;
+    int mode;
+    gv_step_t st;
+    gv_step_t tail;
+    unsigned long long* count;
+    unsigned long long* bytes;
+    unsigned long long* req;
+    uint32_t* gbits;
+};
+
+__global__ __launch_bounds__(kBlock) void k_count_rows(RowsArgs a) {
+    uint64_t cnt = 0, by = 0, rq = 0;
+    uint32_t last_gid = 0xffffffffu;
+    const uint32_t W = a.L + 1;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r < a.nrows; r += stride) {
+        const uint32_t* row = a.rows + r * a.L;
+        uint32_t b, e;
+        seg(a.g, a.st.back_part[0], row[a.st.back_col[0]], a.st.label, b, e);
+        by += 4ull * a.L + 8ull + 4ull * (e - b);
+        const uint32_t bound = a.st.restrict_col != GV_NONE ? row[a.st.restrict_col] : 0xffffffffu;
+        // ids are below 0xffffffff, so an absent bound cuts nothing
+        const uint32_t cut = bound == 0xffffffffu ? e : lower_bound_u32(a.g.nbr, b, e, bound);
+        rq += 4ull * a.L + 8ull + (bound != 0xffffffffu ? 4ull * probes(e - b) : 0ull);
+        uint64_t s = cut - b;
+        for (uint32_t j = 0; j < a.st.ncoll; ++j) {
+            const uint32_t x = row[a.st.coll_col[j]];
+            if (x >= bound) continue;
+            rq += 4ull * probes(cut - b);
+            if (contains_u32(a.g.nbr, b, cut, x)) --s;
+        }
+        uint64_t n = s;
+        if (a.mode == 1 && s) {
+            uint32_t tb, te;
+            seg(a.g, a.tail.back_part[0], row[a.tail.back_col[0]], a.tail.label, tb, te);
+            const uint32_t tbound =
+                a.tail.restrict_col != GV_NONE ? row[a.tail.restrict_col] : 0xffffffffu;
+            const uint32_t tcut = tbound == 0xffffffffu ? te : lower_bound_u32(a.g.nbr, tb, te, tbound);
+            rq += 8ull + (tbound != 0xffffffffu ? 4ull * probes(te - tb) : 0ull);
+            uint64_t t = tcut - tb;
+            for (uint32_t j = 0; j < a.tail.ncoll; ++j) {
+                const uint32_t x = row[a.tail.coll_col[j]];
+                if (x >= tbound) continue;
+                rq += 4ull * probes(tcut - tb);
+                if (contains_u32(a.g.nbr, tb, tcut, x)) --t;
+            }
+            n = s * t;
+            by += s * (8ull * W + 8ull + 4ull * (te - tb));
+        } else if (a.mode == 2) {
+            n = s ? s * (s - 1) / 2 : 0;
+            by += s * (8ull * W + 8ull + 4ull * (e - b));
+        }
+        cnt += n;
+        if (n && a.gbits) {
+            const uint32_t gid = a.g.gof[row[0]];
+            if (gid != last_gid) {
+                atomicOr(&a.gbits[gid >> 5], 1u << (gid & 31));
+                last_gid = gid;
+            }
+        }
+    }
+    cnt = wave_sum_u64(cnt);
+    by = wave_sum_u64(by);
+    if (lane_id() == 0) {
+        if (cnt) atomicAdd(a.count, (unsigned long long)cnt);
+        if (by) atomicAdd(a.bytes, (unsigned long long)by);
+    }
     if (a.req) {
         rq = wave_sum_u64(rq);
         if (lane_id() == 0 && rq) atomicAdd(a.req, (unsigned long long)rq);
@@ -199,19 +288,6 @@ struct WalkArgs {
 };
 
 
-__device__ __forceinline__ bool check_candidate(const WalkArgs& a, const uint32_t* row,
-                                                uint32_t v, uint32_t skip, uint64_t& rq) {
-    for (uint32_t j = 0; j < a.st.ncoll; ++j)
-        if (row[a.st.coll_col[j]] == v) return false;
-    for (uint32_t i = 0; i < a.st.nback; ++i) {
-        if (i == skip) continue;
-        uint32_t b, e;
-        seg(a.g, a.st.back_part[i], row[a.st.back_col[i]], a.st.label, b, e);
-        rq += 8ull + 4ull * probes(e - b);
-        if (!contains_u32(a.g.nbr, b, e, v)) return false;
-    }
-    return true;
-}
 
 template <bool kCount>
 __global__ __launch_bounds__(kBlock) void k_walk(WalkArgs a) {
@@ -220,102 +296,32 @@ __global__ __launch_bounds__(kBlock) void k_walk(WalkArgs a) {
     const unsigned wib = threadIdx.x >> 6;
     const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
-    const uint32_t W = a.L + 1;
-    uint64_t my_count = 0, my_bytes = 0, rq = 0;
-    uint32_t last_gid = 0xffffffffu;
+    TileIn in;
+    in.rows = a.rows;
+    in.row_base = 0;
+    in.L = a.L;
+    in.csum = a.csum;
+    in.cnt = a.cnt;
+    in.anc_beg = a.anc_beg;
+    in.anc_i = a.anc_i;
+    in.row_lo = 0;
+    in.row_hi = a.nrows;
+    TileOut o{a.out, a.out_cap, a.out_n, a.overflow};
+    TileAcc acc;
     for (uint64_t t0 = a.c0 + wave * kTile; t0 < a.c1; t0 += nwaves * kTile) {
         const uint64_t t1 = t0 + kTile < a.c1 ? t0 + kTile : a.c1;
-        // rows holding the first and last candidate of the tile
-        uint64_t r_lo = 0, r_hi = 0;
-        if (lane == 0) r_lo = find_row(a.csum, 0, a.nrows, t0);
-        if (lane == 1) r_hi = find_row(a.csum, 0, a.nrows, t1 - 1);
-        r_lo = __shfl(r_lo, 0, 64);
-        r_hi = __shfl(r_hi, 1, 64);
-        for (int it = 0; it < kIPT; ++it) {
-            const uint64_t c = t0 + (uint64_t)it * 64 + lane;
-            bool ok = false;
-            uint32_t v = 0;
-            uint64_t r = 0;
-            if (c < t1) {
-                r = find_row(a.csum, r_lo, r_hi + 1, c);
-                const uint64_t first = a.csum[r] - a.cnt[r];
-                v = a.g.nbr[a.anc_beg[r] + (uint32_t)(c - first)];
-                rq += 4;
-                ok = check_candidate(a, a.rows + r * a.L, v, a.anc_i[r], rq);
-            }
-            // the row and its sizing entries count once per run of lanes sharing it
-            const uint64_t prev_r = __shfl_up(r, 1, 64);
-            if (c < t1 && (lane == 0 || prev_r != r)) rq += 4ull * a.L + 17ull;
-            const uint32_t* row = a.rows + r * a.L;
-            if (kCount) {
-                uint64_t n = ok ? 1 : 0;
-                if (ok && a.has_tail) {
-                    // last vertex with a single back edge: count it directly
-                    const gv_step_t& t = a.tail;
-                    const uint32_t bc = t.back_col[0];
-                    const uint32_t w = bc == a.L ? v : row[bc];
-                    uint32_t b, e;
-                    seg(a.g, t.back_part[0], w, t.label, b, e);
-                    uint32_t bound = 0xffffffffu;
-                    if (t.restrict_col != GV_NONE)
-                        bound = t.restrict_col == a.L ? v : row[t.restrict_col];
-                    const uint32_t cut = lower_bound_u32(a.g.nbr, b, e, bound);
-                    n = cut - b;
-                    rq += 8ull + 4ull * probes(e - b);
-                    for (uint32_t j = 0; j < t.ncoll; ++j) {
-                        const uint32_t x = t.coll_col[j] == a.L ? v : row[t.coll_col[j]];
-                        if (x < bound) {
-                            rq += 4ull * probes(cut - b);
-                            if (contains_u32(a.g.nbr, b, cut, x)) --n;
-                        }
-                    }
-                    my_bytes += 4ull * W + 4ull * W + 8ull + 4ull * (e - b);
-                }
-                my_count += n;
-                if (n && a.gbits) {
-                    const uint32_t gid = a.g.gof[row[0]];
-                    if (gid != last_gid) {
-                        atomicOr(&a.gbits[gid >> 5], 1u << (gid & 31));
-                        last_gid = gid;
-                    }
-                }
-            } else {
-                const uint64_t mask = __ballot(ok);
-                const uint32_t n = (uint32_t)__popcll(mask);
-                if (n == 0) continue;
-                uint64_t base = 0;
-                if (lane == 0) base = atomicAdd(a.out_n, (unsigned long long)n);
-                base = __shfl(base, 0, 64);
-                my_bytes += ok ? 4ull * W : 0;
-                rq += ok ? 4ull * W : 0;
-                if (base + n > a.out_cap) {
-                    if (lane == 0) atomicOr(a.overflow, 1ull);
-                    continue;
-                }
-                uint32_t* st = stage[wib];
-                if (ok) {
-                    const uint32_t p = lanes_below(mask) * W;
-                    for (uint32_t j = 0; j < a.L; ++j) st[p + j] = row[j];
-                    st[p + a.L] = v;
-                }
-                __builtin_amdgcn_wave_barrier();
-                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-                uint32_t* dst = a.out + base * W;
-                for (uint32_t j = lane; j < n * W; j += 64) dst[j] = st[j];
-                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-                __builtin_amdgcn_wave_barrier();
-            }
-        }
+        walk_tile<kCount, kIPT>(a.g, a.st, a.tail, a.has_tail != 0, in, t0, t1, o,
+                                kCount ? nullptr : stage[wib], a.gbits, acc);
     }
     if (kCount) {
-        my_count = wave_sum_u64(my_count);
-        if (lane == 0 && my_count) atomicAdd(a.count, (unsigned long long)my_count);
+        acc.count = wave_sum_u64(acc.count);
+        if (lane == 0 && acc.count) atomicAdd(a.count, (unsigned long long)acc.count);
     }
-    my_bytes = wave_sum_u64(my_bytes);
-    if (lane == 0 && my_bytes) atomicAdd(a.bytes, (unsigned long long)my_bytes);
+    acc.bytes = wave_sum_u64(acc.bytes);
+    if (lane == 0 && acc.bytes) atomicAdd(a.bytes, (unsigned long long)acc.bytes);
     if (a.req) {
-        rq = wave_sum_u64(rq);
-        if (lane == 0 && rq) atomicAdd(a.req, (unsigned long long)rq);
+        acc.rq = wave_sum_u64(acc.rq);
+        if (lane == 0 && acc.rq) atomicAdd(a.req, (unsigned long long)acc.rq);
     }
 }
 
@@ -804,6 +810,57 @@ int gv_count_emb(gv_ctx_t c, gv_graph_t g, const gv_step_t* st, const gv_step_t*
     }
     a.gbits = (g->d_gof && c->gbits && c->gbits_words * 32ull >= g->ngraphs) ? c->gbits : nullptr;
     return launch_walk(c, a, true);
+}
+
+int gv_count_rows_mode(const gv_step_t* st, const gv_step_t* tail, uint32_t L) {
+    if (!st || st->nback != 1) return -1;
+    if (!tail) return 0;
+    if (tail->nback != 1) return -1;
+    bool collL = false;
+    for (uint32_t j = 0; j < tail->ncoll; ++j) collL |= tail->coll_col[j] == L;
+    if (tail->back_col[0] != L && tail->restrict_col != L && !collL) return 1;
+    // the last vertex draws from v's own segment and must lie below v: its
+    // same-label columns are then v and exactly those of v's step
+    if (tail->back_col[0] == st->back_col[0] && tail->back_part[0] == st->back_part[0] &&
+        tail->label == st->label && tail->restrict_col == L && collL &&
+        tail->ncoll == st->ncoll + 1) {
+        for (uint32_t j = 0; j < st->ncoll; ++j)
+            if (tail->coll_col[j] != st->coll_col[j]) return -1;
+        if (tail->coll_col[st->ncoll] != L) return -1;
+        return 2;
+    }
+    return -1;
+}
+
+int gv_count_rows(gv_ctx_t c, gv_graph_t g, const gv_step_t* st, const gv_step_t* tail,
+                  const uint32_t* rows, uint32_t L, uint64_t nrows) {
+    if (!c || !g || !valid_step(st, L, g)) return GV_ERR_ARG;
+    if (tail && !valid_tail(tail, L, g)) return GV_ERR_ARG;
+    const int mode = gv_count_rows_mode(st, tail, L);
+    if (mode < 0) return GV_ERR_ARG;
+    if (nrows == 0) return GV_OK;
+    if (!rows) return GV_ERR_ARG;
+    HIPCHK(hipSetDevice(c->device));
+    RowsArgs a;
+    memset(&a, 0, sizeof(a));
+    a.g = dev_view(g);
+    a.rows = rows;
+    a.nrows = nrows;
+    a.L = L;
+    a.mode = mode;
+    a.st = *st;
+    if (tail) a.tail = *tail;
+    a.count = c->acc + 0;
+    a.bytes = c->acc + 1;
+    a.req = req_ptr(c, kKindRows);
+    a.gbits = (g->d_gof && c->gbits && c->gbits_words * 32ull >= g->ngraphs) ? c->gbits : nullptr;
+    {
+        ProfScope ps(c, kKindRows);
+        hipLaunchKernelGGL(k_count_rows, dim3(grid_for(nrows, kBlock, c->num_cu * 16)), dim3(kBlock),
+                           0, c->stream, a);
+    }
+    HIPCHK(hipGetLastError());
+    return GV_OK;
 }
 
 int gv_counters_reset(gv_ctx_t c, gv_graph_t g) {

@@ -28,7 +28,7 @@ uint64_t estimatePeakBytes(const CompiledQuery& cq, const GraphStats& st);
 struct BatchOptions {
     uint32_t slots = 256;
     uint64_t slotBytes = 64ull << 20;   // per table, two tables per slot
-    uint64_t rowCap = 1ull << 26;       // rows sized per iteration, whole batch
+    uint64_t rowCap = 1ull << 28;       // rows sized per iteration, whole batch
     gv_tune_t tune{4, 16};
 };
 

@@ -268,7 +268,7 @@ uint32_t edgeLabelOf(const Csr& c, uint32_t a, uint32_t b) {
 
 std::vector<std::string> genWalkQueries(const Dataset& ds, uint32_t nq, uint32_t kmin,
                                         uint32_t kmax, bool induced, uint32_t minExtra,
-                                        uint64_t seed) {
+                                        uint64_t seed, uint32_t minDeg) {
     Rng r(seed);
     Csr c = csrOf(ds.g);
     // start candidates: vertices with an edge, grouped by owner graph
@@ -316,6 +316,14 @@ std::vector<std::string> genWalkQueries(const Dataset& ds, uint32_t nq, uint32_t
                 }
         }
         if (qe.size() < (size_t)k - 1 + minExtra) continue;
+        if (minDeg) {
+            std::vector<uint32_t> qd(k, 0);
+            for (auto& kv : qe) {
+                qd[kv.first.first]++;
+                qd[kv.first.second]++;
+            }
+            if (*std::min_element(qd.begin(), qd.end()) < minDeg) continue;
+        }
         std::vector<uint32_t> labels(k);
         for (uint32_t i = 0; i < k; ++i) labels[i] = ds.g.vlabel[chosen[i]];
         std::vector<std::pair<uint32_t, uint32_t>> es;

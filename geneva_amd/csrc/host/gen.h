@@ -65,10 +65,12 @@ Dataset genChungLu(uint32_t nv, double gamma, double meanDeg, double wmax, uint3
 // random chosen vertex and a random neighbour, adding the vertex and the walk
 // edge, until k distinct vertices.  induced: add every data edge among the
 // chosen vertices.  minExtra: resample until |E| >= k - 1 + minExtra.
+// minDeg: resample until every query vertex has at least minDeg query
+// neighbours (2: every vertex lies on a cycle).
 // k ~ U[kmin, kmax].  Query vertex 0 is the walk start.
 std::vector<std::string> genWalkQueries(const Dataset& ds, uint32_t nq, uint32_t kmin,
                                         uint32_t kmax, bool induced, uint32_t minExtra,
-                                        uint64_t seed);
+                                        uint64_t seed, uint32_t minDeg = 0);
 
 // Fixed shapes on 5 vertices: path, star, cycle, tailed triangle (triangle
 // plus a 2-edge tail) and a 4-clique with a pendant vertex.  Vertex labels are

@@ -153,9 +153,9 @@ gh_dataset_t gh_ds_from_arrays(uint32_t nv, const uint32_t* vlabel, uint64_t m,
 void gh_ds_free(gh_dataset_t ds) { delete ds; }
 
 int gh_ds_add_walk_queries(gh_dataset_t d, uint32_t nq, uint32_t kmin, uint32_t kmax, int induced,
-                           uint32_t min_extra, uint64_t seed) {
+                           uint32_t min_extra, uint64_t seed, uint32_t min_deg) {
     if (!d || kmin < 2 || kmax < kmin || kmax > GV_MAXK) return GV_ERR_ARG;
-    auto q = geneva::genWalkQueries(d->ds, nq, kmin, kmax, induced != 0, min_extra, seed);
+    auto q = geneva::genWalkQueries(d->ds, nq, kmin, kmax, induced != 0, min_extra, seed, min_deg);
     d->ds.queries.insert(d->ds.queries.end(), q.begin(), q.end());
     return GV_OK;
 }
@@ -322,9 +322,24 @@ uint64_t gh_engine_graph_bytes(gh_engine_t e) {
 int gh_engine_mine(gh_engine_t e, const char* text, int single_v, uint64_t* count,
                    uint64_t* alg_bytes, double* ms, uint32_t* bits, uint32_t nwords,
                    uint64_t* launches) {
-    if (!e || !text) return GV_ERR_ARG;
+    return gh_engine_mine_shard(e, text, single_v, 0, 0, 1, count, alg_bytes, ms, bits, nwords,
+                                launches);
+}
+
+int gh_engine_mine_shard(gh_engine_t e, const char* text, int single_v, uint64_t start_begin,
+                         uint64_t start_end, uint64_t stride, uint64_t* count, uint64_t* alg_bytes,
+                         double* ms, uint32_t* bits, uint32_t nwords, uint64_t* launches) {
+    if (!e || !text || stride == 0) return GV_ERR_ARG;
     geneva::MineResult r;
-    int rc = e->plain->mineText(text, single_v != 0, r);
+    geneva::CompiledQuery cq;
+    std::string err;
+    int rc = GV_OK;
+    if (!geneva::planAndCompile(text, single_v != 0, e->graph, cq, &err)) {
+        r.status = rc = GV_ERR_ARG;
+        r.error = err;
+    } else {
+        rc = e->plain->mine(cq, geneva::PlainMiner::Shard{start_begin, start_end, stride}, r);
+    }
     if (rc != GV_OK) {
         setErr(r.error.empty() ? std::string(gv_status_string(rc)) : r.error);
         return rc;
@@ -336,6 +351,12 @@ int gh_engine_mine(gh_engine_t e, const char* text, int single_v, uint64_t* coun
     if (bits && nwords) {
         for (uint32_t i = 0; i < nwords; ++i) bits[i] = i < r.graphBits.size() ? r.graphBits[i] : 0;
     }
+    return GV_OK;
+}
+
+int gh_engine_count_by_rows(gh_engine_t e, int on) {
+    if (!e) return GV_ERR_ARG;
+    e->plain->setCountByRows(on != 0);
     return GV_OK;
 }
 

@@ -178,6 +178,7 @@ PlainMiner::~PlainMiner() {
         if (l.rows) gv_free(ctx_, l.rows);
         if (l.fr) gv_frontier_destroy(l.fr);
     }
+    if (shardRows_) gv_free(ctx_, shardRows_);
 }
 
 int PlainMiner::ensureFrontier(uint32_t width, uint64_t n) {
@@ -207,14 +208,22 @@ int PlainMiner::run(const gv_query_t& q, uint32_t width, const uint32_t* rows, u
                     MineResult& out) {
     if (n == 0) return GV_OK;
     const gv_step_t& st = q.step[width];
+    const bool last = width + 1 == q.k;
+    const bool fusedTail = width + 2 == q.k && q.step[width + 1].nback == 1;
+    if (last || fusedTail) {
+        // a single back edge: counted per row, no candidate walk
+        const gv_step_t* tail = fusedTail ? &q.step[width + 1] : nullptr;
+        if (countByRows_ && gv_count_rows_mode(&st, tail, width) >= 0) {
+            out.launches++;
+            return gv_count_rows(ctx_, g_, &st, tail, rows, width, n);
+        }
+    }
     int rc = ensureFrontier(width, n);
     if (rc != GV_OK) return rc;
     gv_frontier_t fr = lv_[width].fr;
     uint64_t total = 0;
     rc = gv_size_rows(ctx_, g_, &st, rows, width, n, fr, &total);
     if (rc != GV_OK || total == 0) return rc;
-    const bool last = width + 1 == q.k;
-    const bool fusedTail = width + 2 == q.k && q.step[width + 1].nback == 1;
     if (last || fusedTail) {
         // count-only: no output table, so any range works; launches are cut at
         // kCountChunk candidates to keep each kernel short
@@ -245,7 +254,12 @@ int PlainMiner::run(const gv_query_t& q, uint32_t width, const uint32_t* rows, u
 }
 
 int PlainMiner::mine(const CompiledQuery& cq, MineResult& out) {
+    return mine(cq, Shard(), out);
+}
+
+int PlainMiner::mine(const CompiledQuery& cq, const Shard& sh, MineResult& out) {
     out = MineResult();
+    if (sh.stride == 0) return out.status = GV_ERR_ARG;
     uint64_t nv1 = 0;
     gv_graph_info(g_, &nv1, nullptr, nullptr, nullptr, nullptr);
     int rc = gv_counters_reset(ctx_, g_);
@@ -255,6 +269,30 @@ int PlainMiner::mine(const CompiledQuery& cq, MineResult& out) {
         const uint32_t* rows = nullptr;
         uint64_t n = 0;
         rc = gv_graph_start_rows(g_, cq.q.label0, &rows, &n);
+        if (rc == GV_OK) {
+            const uint64_t e = sh.end == 0 ? n : std::min(sh.end, n);
+            const uint64_t b = std::min(sh.begin, e);
+            if (sh.stride == 1) {
+                rows += b;
+                n = e - b;
+            } else {
+                // gather the shard's vertices into a table of its own
+                std::vector<uint32_t> all(e), pick;
+                if (e) rc = gv_memcpy_d2h(ctx_, all.data(), rows, e * 4ull);
+                for (uint64_t i = b; rc == GV_OK && i < e; i += sh.stride) pick.push_back(all[i]);
+                if (rc == GV_OK && pick.size() > shardCap_) {
+                    if (shardRows_) gv_free(ctx_, shardRows_);
+                    shardRows_ = nullptr;
+                    shardCap_ = 0;
+                    rc = gv_malloc(ctx_, pick.size() * 4ull, (void**)&shardRows_);
+                    if (rc == GV_OK) shardCap_ = pick.size();
+                }
+                if (rc == GV_OK && !pick.empty())
+                    rc = gv_memcpy_h2d(ctx_, shardRows_, pick.data(), pick.size() * 4ull);
+                rows = shardRows_;
+                n = pick.size();
+            }
+        }
         if (rc == GV_OK) rc = run(cq.q, 1, rows, n, out);
         if (rc != GV_OK) {
             out.status = rc;

@@ -64,6 +64,19 @@ public:
     int mine(const CompiledQuery& cq, MineResult& out);
     int mineText(const std::string& text, bool singleV, MineResult& out);
 
+    // One shard of the start vertices: indices begin, begin+stride, ... below
+    // end (0 = all) of the id-sorted list of vertices carrying the first
+    // query vertex's label.  Counts over the shards of a partition add up to
+    // the whole count; (0, 0, 1) is the whole query.
+    struct Shard {
+        uint64_t begin = 0, end = 0, stride = 1;
+    };
+    int mine(const CompiledQuery& cq, const Shard& sh, MineResult& out);
+
+    // Count a last step with a single back edge per row (gv_count_rows)
+    // instead of walking its candidates (on by default).
+    void setCountByRows(bool on) { countByRows_ = on; }
+
 private:
     struct Level {
         uint32_t* rows = nullptr;   // table of width `col` (rows produced by step col-1)
@@ -80,6 +93,9 @@ private:
     gv_graph_t g_;
     uint64_t levelBytes_;
     Level lv_[GV_MAXK + 1];
+    bool countByRows_ = true;
+    uint32_t* shardRows_ = nullptr;   // start table of a strided shard
+    uint64_t shardCap_ = 0;
 };
 
 }  // namespace geneva

@@ -157,6 +157,22 @@ int gv_count_emb(gv_ctx_t ctx, gv_graph_t g, const gv_step_t* step, const gv_ste
                  const uint32_t* rows_dev, uint32_t row_len, uint64_t nrows, gv_frontier_t f,
                  uint64_t cand_begin, uint64_t cand_end);
 
+/* Count-only final phase without a candidate walk, for a last materialised
+ * step with a single back edge (its survivors per row are one label segment
+ * cut at the restriction, minus the row's same-label values in it).  Row r's
+ * count is, by gv_count_rows_mode(step, tail, row_len):
+ *   0  (tail == NULL): the survivors s;
+ *   1  tail independent of the survivor (its back edge, restriction and
+ *      same-label columns all in the row): s * T(row);
+ *   2  tail draws from the survivor's own segment and lies below it: s(s-1)/2.
+ * -1 = not applicable (use gv_size_rows + gv_count_emb; gv_count_rows then
+ * returns GV_ERR_ARG).  Counts, algorithmic bytes and graph bits accumulate
+ * exactly as gv_count_emb's do.  Replaces, like gv_count_emb, the isLastPhase
+ * branch of extEmb_1V_NoHash (include/miningPattern.h:29-31).  Asynchronous. */
+int gv_count_rows_mode(const gv_step_t* step, const gv_step_t* tail, uint32_t row_len);
+int gv_count_rows(gv_ctx_t ctx, gv_graph_t g, const gv_step_t* step, const gv_step_t* tail,
+                  const uint32_t* rows_dev, uint32_t row_len, uint64_t nrows);
+
 /* Accumulators of a context: embedding count, algorithmic bytes and matched
  * graph bitset (ngraphs bits).  reset zeroes them; read synchronises. */
 int gv_counters_reset(gv_ctx_t ctx, gv_graph_t g);
@@ -228,6 +244,7 @@ int gv_batch_info(gv_batch_t b, uint32_t* slots, uint64_t* slot_bytes, uint32_t*
 #define GV_KIND_EXTEND 2
 #define GV_KIND_COUNT 3
 #define GV_KIND_BATCH 4
+#define GV_KIND_ROWS 5
 int gv_profile_enable(gv_ctx_t ctx, int on);
 int gv_profile_reset(gv_ctx_t ctx);
 int gv_profile_read(gv_ctx_t ctx, int kind, uint64_t* launches, double* total_ms,

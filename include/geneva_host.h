@@ -37,8 +37,10 @@ gh_dataset_t gh_ds_from_arrays(uint32_t nv, const uint32_t* vlabel, uint64_t m,
                                const uint32_t* edges, const uint32_t* graph_of,
                                uint32_t ngraphs);
 void gh_ds_free(gh_dataset_t ds);
+/* min_deg: resample until every query vertex has at least min_deg query
+ * neighbours (0 = no condition). */
 int gh_ds_add_walk_queries(gh_dataset_t ds, uint32_t nq, uint32_t kmin, uint32_t kmax,
-                           int induced, uint32_t min_extra, uint64_t seed);
+                           int induced, uint32_t min_extra, uint64_t seed, uint32_t min_deg);
 int gh_ds_add_shape_queries(gh_dataset_t ds, uint64_t seed);
 int gh_ds_add_query(gh_dataset_t ds, const char* text);
 uint32_t gh_ds_nv(gh_dataset_t ds);
@@ -72,6 +74,18 @@ uint64_t gh_engine_graph_bytes(gh_engine_t e);
 int gh_engine_mine(gh_engine_t e, const char* query_text, int single_v, uint64_t* count,
                    uint64_t* alg_bytes, double* device_ms, uint32_t* graph_bits,
                    uint32_t nwords, uint64_t* launches);
+/* One shard of the start vertices: indices start_begin, start_begin+stride, ...
+ * below start_end (0 = all) of the id-sorted list of vertices carrying the
+ * first query vertex's label.  Counts of the shards of a partition add up to
+ * the whole count (the unit a multi-GPU run would hand each device). */
+int gh_engine_mine_shard(gh_engine_t e, const char* query_text, int single_v,
+                         uint64_t start_begin, uint64_t start_end, uint64_t stride,
+                         uint64_t* count, uint64_t* alg_bytes, double* device_ms,
+                         uint32_t* graph_bits, uint32_t nwords, uint64_t* launches);
+
+/* Plain path: count a last step with a single back edge per row
+ * (gv_count_rows, default on) or by walking its candidates (off). */
+int gh_engine_count_by_rows(gh_engine_t e, int on);
 
 /* ---- batched path ------------------------------------------------------------
  * Configure the batch engine (slots, bytes per slot table, rows sized per

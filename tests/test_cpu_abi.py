@@ -63,3 +63,38 @@ def test_engine_requires_a_gpu_and_never_falls_back():
     if n.value == 0:
         with pytest.raises(G.GenevaError):
             G.Engine(ds)
+
+
+def test_count_rows_mode_selection():
+    """gv_count_rows_mode: which last steps are counted per row."""
+    import ctypes as C
+    hip = C.CDLL(G.HIP_LIB)
+
+    class Step(C.Structure):
+        _fields_ = [("label", C.c_uint32), ("col", C.c_uint32), ("nback", C.c_uint8),
+                    ("back_col", C.c_uint8 * 16), ("back_part", C.c_uint8 * 16),
+                    ("restrict_col", C.c_uint8), ("ncoll", C.c_uint8), ("coll_col", C.c_uint8 * 16)]
+
+    def step(label, back, restrict=0xff, coll=()):
+        s = Step()
+        s.label, s.nback, s.restrict_col, s.ncoll = label, len(back), restrict, len(coll)
+        for i, b in enumerate(back):
+            s.back_col[i] = b
+        for i, c in enumerate(coll):
+            s.coll_col[i] = c
+        return s
+
+    f = hip.gv_count_rows_mode
+    f.argtypes = [C.POINTER(Step), C.POINTER(Step), C.c_uint32]
+    f.restype = C.c_int
+    L = 3
+    st = step(2, [0], coll=(1,))
+    assert f(C.byref(st), None, L) == 0
+    assert f(C.byref(step(2, [0, 1])), None, L) == -1                          # two back edges
+    assert f(C.byref(st), C.byref(step(5, [1])), L) == 1                       # tail off column 1
+    assert f(C.byref(st), C.byref(step(5, [0], restrict=2)), L) == 1
+    assert f(C.byref(st), C.byref(step(5, [L])), L) == -1                      # tail off v
+    assert f(C.byref(st), C.byref(step(2, [0], restrict=L, coll=(1, L))), L) == 2
+    assert f(C.byref(st), C.byref(step(2, [0], coll=(1, L))), L) == -1         # no restriction to v
+    assert f(C.byref(st), C.byref(step(2, [1], restrict=L, coll=(1, L))), L) == -1  # other segment
+    assert f(C.byref(st), C.byref(step(2, [0], restrict=L, coll=(L,))), L) == -1    # columns differ

@@ -128,3 +128,38 @@ def test_bad_query_raises(w1):
         eng.mine("v 0 1\nv 1 2\n")          # no edges
     with pytest.raises(G.GenevaError):
         eng.mine("v 0 1\nv 1 2\nv 2 3\ne 0 1 1\n")  # disconnected
+
+
+ROW_COUNT_QUERIES = [
+    # (query, mode of the counted step): one edge (last step), a path whose
+    # last vertex hangs off an earlier column, and stars whose last two leaves
+    # share label and centre (pairs below v in one segment)
+    ("v 0 1\nv 1 2\ne 0 1 1\n", 0),
+    ("v 0 1\nv 1 1\ne 0 1 1\n", 0),
+    ("v 0 1\nv 1 2\nv 2 3\nv 3 1\ne 0 1 1\ne 1 2 1\ne 2 3 1\n", 1),
+    ("v 0 1\nv 1 2\nv 2 2\nv 3 2\ne 0 1 1\ne 0 2 1\ne 0 3 1\n", 2),
+    ("v 0 1\nv 1 1\nv 2 1\nv 3 1\nv 4 1\ne 0 1 1\ne 0 2 1\ne 0 3 1\ne 0 4 1\n", 2),
+    ("v 0 1\nv 1 2\nv 2 1\nv 3 1\ne 0 1 1\ne 1 2 1\ne 1 3 1\n", 2),
+]
+
+
+def test_count_by_rows_equals_candidate_walk_and_oracle(require_gpu):
+    """Counting the last step per row (segment arithmetic) gives the counts,
+    graph ids and algorithmic bytes of the candidate walk and of the oracle."""
+    import oracle as O
+    for seed in range(3):
+        ds = G.Dataset.database(200, 5, 60, 4.0 + 2 * seed, 0.5, 3 + seed, 0.0, 2000 + seed)
+        og = O.OracleGraph.of(ds)
+        rows, walk = G.Engine(ds), G.Engine(ds)
+        walk.count_by_rows(False)
+        rows.profile(True)
+        for q, mode in ROW_COUNT_QUERIES:
+            print("row count seed", seed, "mode", mode, "query", q.replace("\n", " "), flush=True)
+            want = og.mine(q)
+            a, b = rows.mine(q), walk.mine(q)
+            assert a.count == b.count == want.count
+            assert a.graph_ids == b.graph_ids == want.graph_ids
+            assert a.alg_bytes == b.alg_bytes == want.alg_bytes
+        assert rows.profile_read()["k_count_rows"][0] > 0
+        rows.close()
+        walk.close()
