@@ -222,7 +222,7 @@ def main():
 
     # roofline of the dominant kernel over the timed region
     dom = max(prof.items(), key=lambda kv: kv[1][1])
-    name, (launches, total_ms, req) = dom
+    name, (launches, total_ms, req, units) = dom
     avg_ms = total_ms / max(launches, 1)
     achieved = req / (total_ms / 1e3) / 1e9 if total_ms > 0 else 0.0
     traffic = None
@@ -233,8 +233,10 @@ def main():
     roofline = {"bound": "hbm", "kernel": name, "achieved": round(achieved, 2),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                 "traffic": traffic, "launches": launches, "avg_launch_ms": round(avg_ms, 4),
-                "alg_bytes_per_launch": req // max(launches, 1)}
-    kernels = {k: {"launches": v[0], "ms": round(v[1], 3), "req_bytes": v[2]} for k, v in prof.items()}
+                "alg_bytes_per_launch": req // max(launches, 1),
+                "units_per_launch": units // max(launches, 1)}
+    kernels = {k: {"launches": v[0], "ms": round(v[1], 3), "req_bytes": v[2], "units": v[3]}
+               for k, v in prof.items()}
 
     eng.close()
     batched = served = None

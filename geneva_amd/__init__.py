@@ -79,6 +79,7 @@ def lib():
         "gh_engine_profile": (i32, [vp, i32]),
         "gh_engine_profile_reset": (i32, [vp]),
         "gh_engine_profile_read": (i32, [vp, i32, C.POINTER(u64), C.POINTER(dbl), C.POINTER(u64)]),
+        "gh_engine_profile_units": (i32, [vp, i32, C.POINTER(u64)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -198,11 +199,20 @@ def dv_phase_count(query_text):
 
 
 class MineResult:
-    __slots__ = ("count", "alg_bytes", "ms", "graph_ids", "launches")
+    """count (uint64), alg_bytes, device ms, launches, and the matched graph ids
+    (decoded from the bitset on first access)."""
+    __slots__ = ("count", "alg_bytes", "ms", "launches", "_words", "_ngraphs", "_ids")
 
-    def __init__(self, count, alg_bytes, ms, graph_ids, launches):
-        self.count, self.alg_bytes, self.ms = count, alg_bytes, ms
-        self.graph_ids, self.launches = graph_ids, launches
+    def __init__(self, count, alg_bytes, ms, words, launches, ngraphs=0):
+        self.count, self.alg_bytes, self.ms, self.launches = count, alg_bytes, ms, launches
+        self._words, self._ngraphs, self._ids = words, ngraphs, None
+
+    @property
+    def graph_ids(self):
+        if self._ids is None:
+            self._ids = _bits_to_ids(self._words, self._ngraphs) if self._words is not None \
+                else frozenset()
+        return self._ids
 
     def __repr__(self):
         return (f"MineResult(count={self.count}, alg_bytes={self.alg_bytes}, ms={self.ms:.3f}, "
@@ -269,7 +279,7 @@ class Engine:
         res = []
         for i in range(n):
             w = words[i * nwords:(i + 1) * nwords]
-            res.append(MineResult(counts[i], ab[i], 0.0, _bits_to_ids(w, self.ngraphs), 0))
+            res.append(MineResult(counts[i], ab[i], 0.0, w, 0, self.ngraphs))
         stats = dict(zip(("batched", "overflowed", "graph_launches", "device_ms", "wall_ms"), list(st)))
         return res, stats
 
@@ -308,14 +318,17 @@ class Engine:
         lib().gh_engine_profile_reset(self._h)
 
     def profile_read(self):
-        """{kernel: (launches, total device ms, requested bytes)} since the last reset."""
+        """{kernel: (launches, total device ms, requested bytes, units)} since the last
+        reset; units are rows (k_size, k_scan, k_count_rows) or candidates (k_walk)."""
         out = {}
         for k, name in self.KERNELS.items():
             n, ms, rq = C.c_uint64(0), C.c_double(0), C.c_uint64(0)
             rc = lib().gh_engine_profile_read(self._h, k, C.byref(n), C.byref(ms), C.byref(rq))
             if rc:
                 raise _err("profile_read", rc)
-            out[name] = (n.value, ms.value, rq.value)
+            u = C.c_uint64(0)
+            lib().gh_engine_profile_units(self._h, k, C.byref(u))
+            out[name] = (n.value, ms.value, rq.value, u.value)
         return out
 
     def mine(self, query_text, single_v=False, start_begin=0, start_end=0, stride=1):
@@ -333,4 +346,5 @@ class Engine:
         if rc:
             raise _err("mine", rc)
         return MineResult(count.value, ab.value, ms.value,
-                          _bits_to_ids(list(bits)[:nwords], self.ngraphs), launches.value)
+                          np.ctypeslib.as_array(bits)[:nwords] if nwords else None,
+                          launches.value, self.ngraphs)

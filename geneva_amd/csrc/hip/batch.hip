@@ -37,6 +37,7 @@ struct BState {              // dynamic, polled by the host (mirrors gv_slot_inf
     // internal
     uint32_t sel;            // half holding the input table
     uint32_t mode;           // this iteration: 0 idle, 1 extend, 2 count, 3 count + tail
+    int32_t rmode;           // count modes: count_rows_mode of the step, or -1 (walk)
     unsigned long long out_rows;
     unsigned long long overflow;
     uint64_t row_end;        // end of this slot's rows in the iteration row space
@@ -130,6 +131,10 @@ __global__ __launch_bounds__(1024) void k_b_prep(BArgs a) {
                 const bool last = S.width + 1 == D.q.k;
                 const bool tail = S.width + 2 == D.q.k && D.q.step[S.width + 1].nback == 1;
                 mode = last ? 2 : (tail ? 3 : 1);
+                S.rmode = mode == 1 ? -1
+                                    : count_rows_mode(&D.q.step[S.width],
+                                                      mode == 3 ? &D.q.step[S.width + 1] : nullptr,
+                                                      S.width);
                 rows = n;
                 if (n > a.row_cap) {  // can never fit one iteration
                     S.state = GV_SLOT_OVERFLOW;
@@ -170,16 +175,22 @@ __global__ __launch_bounds__(kBlock) void k_b_size(BArgs a) {
     __syncthreads();
     const uint64_t ntiles = a.glob[3];
     const unsigned lane = __lane_id();
-    uint32_t cur = 0xffffffffu;
-    uint64_t acc = 0;
+    uint32_t cur = 0xffffffffu, last_gid = 0xffffffffu;
+    uint64_t acc = 0, cacc = 0, rq = 0;
+    auto flush = [&]() {
+        if (cur == 0xffffffffu) return;
+        const uint64_t w = wave_sum_u64(acc), c = wave_sum_u64(cacc);
+        if (lane == 0 && w) atomicAdd(&a.st[cur].bytes, (unsigned long long)w);
+        if (lane == 0 && c) atomicAdd(&a.st[cur].count, (unsigned long long)c);
+        acc = cacc = 0;
+    };
     for (uint64_t RT = blockIdx.x; RT < ntiles; RT += gridDim.x) {
         // every row of this tile belongs to one slot: its step is read once
         const uint32_t s = __builtin_amdgcn_readfirstlane(slot_of(rtile_end, a.nslots, RT));
         if (s != cur) {
-            const uint64_t w = wave_sum_u64(acc);
-            if (lane == 0 && w) atomicAdd(&a.st[cur].bytes, (unsigned long long)w);
+            flush();
             cur = s;
-            acc = 0;
+            last_gid = 0xffffffffu;
         }
         const BState& S = a.st[s];
         const BDesc& D = a.desc[s];
@@ -204,16 +215,30 @@ __global__ __launch_bounds__(kBlock) void k_b_size(BArgs a) {
                 best_i = i;
             }
         }
-        const uint32_t bound = st.restrict_col != GV_NONE ? row[st.restrict_col] : 0xffffffffu;
         a.anc_beg[r] = best_b;
-        a.cnt[r] = bound == 0xffffffffu
-                       ? best_len
-                       : lower_bound_u32(a.g.nbr, best_b, best_b + best_len, bound) - best_b;
         a.anc_i[r] = (uint8_t)best_i;
+        if (S.rmode >= 0) {
+            // counted here, per row: no candidates are left for the walk
+            const gv_step_t& tl = D.q.step[L + 1 < GV_MAXK ? L + 1 : GV_MAXK - 1];
+            const uint64_t n = row_count(a.g, st, tl, S.rmode, L, row, best_b, best_b + best_len, by, rq);
+            a.cnt[r] = 0;
+            cacc += n;
+            if (n && a.g.gof) {
+                const uint32_t gid = a.g.gof[row[0]];
+                if (gid != last_gid) {
+                    atomicOr(&a.gbits[(uint64_t)s * a.gwords + (gid >> 5)], 1u << (gid & 31));
+                    last_gid = gid;
+                }
+            }
+        } else {
+            const uint32_t bound = st.restrict_col != GV_NONE ? row[st.restrict_col] : 0xffffffffu;
+            a.cnt[r] = bound == 0xffffffffu
+                           ? best_len
+                           : lower_bound_u32(a.g.nbr, best_b, best_b + best_len, bound) - best_b;
+        }
         acc += by;
     }
-    const uint64_t w = wave_sum_u64(acc);
-    if (lane == 0 && w && cur != 0xffffffffu) atomicAdd(&a.st[cur].bytes, (unsigned long long)w);
+    flush();
 }
 
 // ---- scan with the extent in device memory --------------------------------
@@ -330,6 +355,7 @@ template <int IPT>
 __global__ __launch_bounds__(kBlock) void k_b_walk(BArgs a) {
     __shared__ uint64_t tile_end[kMaxSlots];
     __shared__ uint32_t stage[kWavesPerBlock][64 * GV_MAXK];
+    __shared__ uint64_t lx[kWavesPerBlock][64 * IPT + 1];
     for (uint32_t i = threadIdx.x; i < a.nslots; i += blockDim.x) tile_end[i] = a.st[i].tile_end;
     __syncthreads();
     const uint64_t ntiles = a.glob[2];
@@ -376,11 +402,11 @@ __global__ __launch_bounds__(kBlock) void k_b_walk(BArgs a) {
         const gv_step_t& st = D.q.step[L];
         if (S.mode == 1) {
             TileOut o{slot_out(a, s, S), a.slot_uints / (L + 1), &a.st[s].out_rows, &a.st[s].overflow};
-            walk_tile<false, IPT>(a.g, st, st, false, in, t0, t1, o, stage[wib], gb, acc);
+            walk_tile<false, IPT>(a.g, st, st, false, in, t0, t1, o, stage[wib], lx[wib], gb, acc);
         } else {
             TileOut o{nullptr, 0, nullptr, nullptr};
             const gv_step_t& tl = D.q.step[L + 1 < GV_MAXK ? L + 1 : GV_MAXK - 1];
-            walk_tile<true, IPT>(a.g, st, tl, S.mode == 3, in, t0, t1, o, nullptr, gb, acc);
+            walk_tile<true, IPT>(a.g, st, tl, S.mode == 3, in, t0, t1, o, nullptr, lx[wib], gb, acc);
         }
     }
     flush();
@@ -430,8 +456,16 @@ struct gv_batch {
     uint32_t* d_gbits = nullptr;
     uint32_t gwords = 0;
     std::vector<uint8_t> used;          // host view: slot occupied
+    // Admission stages a slot's descriptor and state in pinned memory and
+    // marks it dirty; the next run or poll copies each run of consecutive
+    // dirty slots with one copy per array.  A staging entry is rewritten only
+    // after a synchronisation that followed its copy (epochs below).
+    std::vector<uint8_t> dirty;
+    std::vector<uint64_t> flush_epoch;  // epoch of the slot's last staging copy
+    uint64_t epoch = 1;                 // advanced by every synchronisation here
     BDesc* h_desc = nullptr;            // pinned staging, one entry per slot
     BState* h_st = nullptr;
+    BState* h_zero = nullptr;           // pinned, all zero: the state of a released slot
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
 };
@@ -499,6 +533,42 @@ int capture(gv_batch* b) {
     return GV_OK;
 }
 
+int sync(gv_batch* b) {
+    HIPCHK(hipStreamSynchronize(b->ctx->stream));
+    ++b->epoch;
+    return GV_OK;
+}
+
+// Copy the staged descriptors and states of dirty slots, one copy per run of
+// consecutive dirty slots, and clear their graph bitsets.
+int flush(gv_batch* b) {
+    gv_ctx* c = b->ctx;
+    uint32_t s = 0;
+    while (s < b->nslots) {
+        if (!b->dirty[s]) {
+            ++s;
+            continue;
+        }
+        uint32_t e = s;
+        while (e < b->nslots && b->dirty[e]) ++e;
+        const uint32_t n = e - s;
+        HIPCHK(hipMemcpyAsync(b->d_desc + s, b->h_desc + s, sizeof(BDesc) * n, hipMemcpyHostToDevice,
+                              c->stream));
+        HIPCHK(hipMemcpyAsync(b->d_st + s, b->h_st + s, sizeof(BState) * n, hipMemcpyHostToDevice,
+                              c->stream));
+        const uint64_t words = (uint64_t)b->gwords * n;
+        hipLaunchKernelGGL(k_b_fill, dim3((unsigned)std::min<uint64_t>((words + 255) / 256, 1024)),
+                           dim3(256), 0, c->stream, b->d_gbits + (uint64_t)s * b->gwords, words);
+        HIPCHK(hipGetLastError());
+        for (uint32_t i = s; i < e; ++i) {
+            b->dirty[i] = 0;
+            b->flush_epoch[i] = b->epoch;
+        }
+        s = e;
+    }
+    return GV_OK;
+}
+
 bool tune_ok(const gv_tune_t* t) {
     return t && (t->ipt == 1 || t->ipt == 2 || t->ipt == 4 || t->ipt == 8) &&
            t->walk_blocks_per_cu >= 1 && t->walk_blocks_per_cu <= 64;
@@ -523,6 +593,8 @@ int gv_batch_create(gv_ctx_t c, gv_graph_t g, uint32_t slots, uint64_t slot_byte
     b->row_cap = row_cap;
     if (tune) b->tune = *tune;
     b->used.assign(slots, 0);
+    b->dirty.assign(slots, 0);
+    b->flush_epoch.assign(slots, 0);
     b->gwords = g->ngraphs ? (g->ngraphs + 31) / 32 : 1;
     const uint64_t ntiles = (row_cap + kScanTile - 1) / kScanTile;
     bool ok = hipMalloc(&b->d_desc, sizeof(BDesc) * slots) == hipSuccess &&
@@ -537,11 +609,13 @@ int gv_batch_create(gv_ctx_t c, gv_graph_t g, uint32_t slots, uint64_t slot_byte
               hipMalloc(&b->d_glob, 8 * 4) == hipSuccess &&
               hipMalloc(&b->d_gbits, 4ull * b->gwords * slots) == hipSuccess &&
               hipHostMalloc((void**)&b->h_desc, sizeof(BDesc) * slots, hipHostMallocDefault) == hipSuccess &&
-              hipHostMalloc((void**)&b->h_st, sizeof(BState) * slots, hipHostMallocDefault) == hipSuccess;
+              hipHostMalloc((void**)&b->h_st, sizeof(BState) * slots, hipHostMallocDefault) == hipSuccess &&
+              hipHostMalloc((void**)&b->h_zero, sizeof(BState), hipHostMallocDefault) == hipSuccess;
     if (!ok) {
         gv_batch_destroy(b);
         return GV_ERR_NOMEM;
     }
+    memset(b->h_zero, 0, sizeof(BState));
     HIPCHK(hipMemset(b->d_desc, 0, sizeof(BDesc) * slots));
     HIPCHK(hipMemset(b->d_st, 0, sizeof(BState) * slots));
     HIPCHK(hipMemset(b->d_glob, 0, 8 * 4));
@@ -569,13 +643,15 @@ int gv_batch_destroy(gv_batch_t b) {
         if (p) (void)hipFree(p);
     if (b->h_desc) (void)hipHostFree(b->h_desc);
     if (b->h_st) (void)hipHostFree(b->h_st);
+    if (b->h_zero) (void)hipHostFree(b->h_zero);
     delete b;
     return GV_OK;
 }
 
 int gv_batch_set_tune(gv_batch_t b, const gv_tune_t* t) {
     if (!b || !tune_ok(t)) return GV_ERR_ARG;
-    HIPCHK(hipStreamSynchronize(b->ctx->stream));
+    int rc = sync(b);
+    if (rc != GV_OK) return rc;
     b->tune = *t;
     return capture(b);
 }
@@ -596,8 +672,12 @@ int gv_batch_admit(gv_batch_t b, const gv_query_t* q, uint64_t tag, uint32_t* sl
     uint32_t s = 0;
     while (s < b->nslots && b->used[s]) ++s;
     if (s == b->nslots) return GV_ERR_CAPACITY;
-    // a slot is only re-admitted after release, which synchronises, so its
-    // staging entry is never still in flight here
+    // the staging entry may still be read by a copy issued since the last
+    // synchronisation (only if the slot was released without a poll between)
+    if (b->flush_epoch[s] == b->epoch) {
+        int rc = sync(b);
+        if (rc != GV_OK) return rc;
+    }
     BDesc& d = b->h_desc[s];
     memset(&d, 0, sizeof(d));
     d.q = *q;
@@ -609,12 +689,7 @@ int gv_batch_admit(gv_batch_t b, const gv_query_t* q, uint64_t tag, uint32_t* sl
     st.state = GV_SLOT_RUNNING;
     st.width = 1;
     st.rows = d.start_n;
-    gv_ctx* c = b->ctx;
-    HIPCHK(hipMemcpyAsync(b->d_desc + s, &d, sizeof(d), hipMemcpyHostToDevice, c->stream));
-    HIPCHK(hipMemcpyAsync(b->d_st + s, &st, sizeof(st), hipMemcpyHostToDevice, c->stream));
-    hipLaunchKernelGGL(k_b_fill, dim3(1), dim3(256), 0, c->stream, b->d_gbits + (uint64_t)s * b->gwords,
-                       (uint64_t)b->gwords);
-    HIPCHK(hipGetLastError());
+    b->dirty[s] = 1;
     b->used[s] = 1;
     *slot = s;
     return GV_OK;
@@ -622,6 +697,8 @@ int gv_batch_admit(gv_batch_t b, const gv_query_t* q, uint64_t tag, uint32_t* sl
 
 int gv_batch_run(gv_batch_t b, uint32_t iterations) {
     if (!b) return GV_ERR_ARG;
+    int rc = flush(b);
+    if (rc != GV_OK) return rc;
     for (uint32_t i = 0; i < iterations; ++i) {
         ProfScope ps(b->ctx, kKindBatch);
         HIPCHK(hipGraphLaunch(b->exec, b->ctx->stream));
@@ -629,12 +706,18 @@ int gv_batch_run(gv_batch_t b, uint32_t iterations) {
     return GV_OK;
 }
 
-int gv_batch_poll(gv_batch_t b, gv_slot_info_t* info) {
-    if (!b || !info) return GV_ERR_ARG;
+int gv_batch_poll_bits(gv_batch_t b, gv_slot_info_t* info, uint32_t* bits, uint32_t nwords) {
+    if (!b || !info || (bits && nwords != b->gwords)) return GV_ERR_ARG;
+    int rc = flush(b);
+    if (rc != GV_OK) return rc;
     std::vector<BState> h(b->nslots);
     HIPCHK(hipMemcpyAsync(h.data(), b->d_st, sizeof(BState) * b->nslots, hipMemcpyDeviceToHost,
                           b->ctx->stream));
-    HIPCHK(hipStreamSynchronize(b->ctx->stream));
+    if (bits)
+        HIPCHK(hipMemcpyAsync(bits, b->d_gbits, 4ull * b->gwords * b->nslots, hipMemcpyDeviceToHost,
+                              b->ctx->stream));
+    rc = sync(b);
+    if (rc != GV_OK) return rc;
     for (uint32_t s = 0; s < b->nslots; ++s) {
         info[s].tag = h[s].tag;
         info[s].state = b->used[s] ? h[s].state : GV_SLOT_FREE;
@@ -646,18 +729,29 @@ int gv_batch_poll(gv_batch_t b, gv_slot_info_t* info) {
     return GV_OK;
 }
 
+int gv_batch_poll(gv_batch_t b, gv_slot_info_t* info) { return gv_batch_poll_bits(b, info, nullptr, 0); }
+
 int gv_batch_release(gv_batch_t b, uint32_t s, uint32_t* bits, uint32_t nwords) {
     if (!b || s >= b->nslots || !b->used[s]) return GV_ERR_ARG;
+    if (bits && nwords && nwords > b->gwords) return GV_ERR_ARG;
+    if (b->dirty[s]) {
+        // never reached the device: nothing to undo there
+        b->dirty[s] = 0;
+        b->used[s] = 0;
+        return GV_OK;
+    }
     if (bits && nwords) {
-        if (nwords > b->gwords) return GV_ERR_ARG;
         HIPCHK(hipMemcpyAsync(bits, b->d_gbits + (uint64_t)s * b->gwords, 4ull * nwords,
                               hipMemcpyDeviceToHost, b->ctx->stream));
     }
-    BState& st = b->h_st[s];
-    HIPCHK(hipStreamSynchronize(b->ctx->stream));  // staging entry of s no longer in flight
-    memset(&st, 0, sizeof(st));
-    HIPCHK(hipMemcpyAsync(b->d_st + s, &st, sizeof(st), hipMemcpyHostToDevice, b->ctx->stream));
-    HIPCHK(hipStreamSynchronize(b->ctx->stream));
+    // the slot stops here whatever its state: its device state becomes free
+    // (from the constant zero entry, so no staging entry is involved)
+    HIPCHK(hipMemcpyAsync(b->d_st + s, b->h_zero, sizeof(BState), hipMemcpyHostToDevice,
+                          b->ctx->stream));
+    if (bits && nwords) {
+        int rc = sync(b);
+        if (rc != GV_OK) return rc;
+    }
     b->used[s] = 0;
     return GV_OK;
 }

@@ -89,6 +89,34 @@ __device__ __forceinline__ uint64_t find_row(const uint64_t* csum, uint64_t lo, 
     return lo;
 }
 
+// The same search done by a whole wave (all lanes call it with the same
+// arguments; the result is uniform): each round probes 64 evenly spaced rows
+// with one load per lane and keeps the interval between the last false and
+// the first true probe, so n rows take ceil(log64 n) dependent loads instead
+// of log2 n.  Requires csum[hi - 1] > c.
+__device__ __forceinline__ uint64_t wave_find_row(const uint64_t* csum, uint64_t lo, uint64_t hi,
+                                                  uint64_t c) {
+    const unsigned lane = __lane_id();
+    while (hi - lo > 1) {
+        const uint64_t n = hi - lo;
+        const uint64_t step = (n + 63) >> 6;
+        const uint64_t x = lo + (uint64_t)lane * step;
+        const bool p = x < hi && csum[x] > c;
+        const uint64_t m = __ballot(p);
+        if (m == 0) {
+            // every probe below hi is false: the answer lies past the last one
+            const uint64_t last = (n - 1) / step < 63 ? (n - 1) / step : 63;
+            lo = lo + last * step + 1;
+            continue;
+        }
+        const unsigned j = __builtin_ctzll(m);
+        if (j == 0) return lo;
+        hi = lo + (uint64_t)j * step + 1;   // answer <= probe j
+        lo = lo + (uint64_t)(j - 1) * step + 1;   // answer > probe j - 1
+    }
+    return lo;
+}
+
 // Count of the last vertex when it has a single back edge (step `t`, rows of
 // width L extended by candidate v in column L): the label segment of its
 // neighbour, cut below the restriction bound, minus mapped vertices inside.
@@ -202,6 +230,79 @@ __device__ __forceinline__ uint64_t tail_count(const DevGraph& g, const gv_step_
 }
 
 
+// ---- counting a last step per row ----------------------------------------
+//
+// When the last materialised step has a single back edge, a row's candidates
+// are one label segment cut at the restriction, and its survivors are that
+// range minus the row's same-label values found in it (s of them).  The row's
+// count is then, by mode:
+//   0  the step is the last:                                    s
+//   1  the last vertex depends on the row only (its back edge,
+//      restriction and same-label columns all lie in the row):  s * T(row)
+//   2  the last vertex is drawn from the survivor's own segment,
+//      has the same label and must lie below the survivor:      s (s - 1) / 2
+// -1: not applicable (the candidates are walked).
+__host__ __device__ inline int count_rows_mode(const gv_step_t* st, const gv_step_t* tail,
+                                               uint32_t L) {
+    if (!st || st->nback != 1) return -1;
+    if (!tail) return 0;
+    if (tail->nback != 1) return -1;
+    bool collL = false;
+    for (uint32_t j = 0; j < tail->ncoll; ++j) collL |= tail->coll_col[j] == L;
+    if (tail->back_col[0] != L && tail->restrict_col != L && !collL) return 1;
+    // same segment and label: the last vertex's same-label columns are then
+    // exactly the survivor's own and the survivor
+    if (tail->back_col[0] == st->back_col[0] && tail->back_part[0] == st->back_part[0] &&
+        tail->label == st->label && tail->restrict_col == L && tail->ncoll == st->ncoll + 1) {
+        for (uint32_t j = 0; j < st->ncoll; ++j)
+            if (tail->coll_col[j] != st->coll_col[j]) return -1;
+        return tail->coll_col[st->ncoll] == L ? 2 : -1;
+    }
+    return -1;
+}
+
+// Count of one row whose step segment is nbr[b, e) (the single back edge's
+// label segment).  Adds the fused last vertex's algorithmic bytes to `by`
+// (the caller adds the row's own) and the requested bytes to `rq`.
+__device__ __forceinline__ uint64_t row_count(const DevGraph& g, const gv_step_t& st,
+                                              const gv_step_t& tail, int mode, uint32_t L,
+                                              const uint32_t* row, uint32_t b, uint32_t e,
+                                              uint64_t& by, uint64_t& rq) {
+    const uint32_t W = L + 1;
+    const uint32_t bound = st.restrict_col != GV_NONE ? row[st.restrict_col] : 0xffffffffu;
+    // ids are below 0xffffffff, so an absent bound cuts nothing
+    const uint32_t cut = bound == 0xffffffffu ? e : lower_bound_u32(g.nbr, b, e, bound);
+    rq += bound != 0xffffffffu ? 4ull * probes(e - b) : 0ull;
+    uint64_t s = cut - b;
+    for (uint32_t j = 0; j < st.ncoll; ++j) {
+        const uint32_t x = row[st.coll_col[j]];
+        if (x >= bound) continue;
+        rq += 4ull * probes(cut - b);
+        if (contains_u32(g.nbr, b, cut, x)) --s;
+    }
+    if (mode == 1 && s) {
+        uint32_t tb, te;
+        seg(g, tail.back_part[0], row[tail.back_col[0]], tail.label, tb, te);
+        const uint32_t tbound = tail.restrict_col != GV_NONE ? row[tail.restrict_col] : 0xffffffffu;
+        const uint32_t tcut = tbound == 0xffffffffu ? te : lower_bound_u32(g.nbr, tb, te, tbound);
+        rq += 8ull + (tbound != 0xffffffffu ? 4ull * probes(te - tb) : 0ull);
+        uint64_t t = tcut - tb;
+        for (uint32_t j = 0; j < tail.ncoll; ++j) {
+            const uint32_t x = row[tail.coll_col[j]];
+            if (x >= tbound) continue;
+            rq += 4ull * probes(tcut - tb);
+            if (contains_u32(g.nbr, tb, tcut, x)) --t;
+        }
+        by += s * (8ull * W + 8ull + 4ull * (te - tb));
+        return s * t;
+    }
+    if (mode == 2) {
+        by += s * (8ull * W + 8ull + 4ull * (e - b));
+        return s ? s * (s - 1) / 2 : 0;
+    }
+    return s;
+}
+
 __device__ __forceinline__ unsigned lane_id() { return __lane_id(); }
 
 
@@ -250,18 +351,18 @@ struct TileAcc {
     TailCache tc;
 };
 
+// lx: LDS of the wave, 64 * IPT + 1 entries: the candidate starts of the
+// tile's rows, so that a lane finds its row without global reads.
 template <bool kCount, int IPT>
 __device__ __forceinline__ void walk_tile(const DevGraph& g, const gv_step_t& st,
                                           const gv_step_t& tail, bool has_tail, const TileIn& in,
                                           uint64_t t0, uint64_t t1, const TileOut& o,
-                                          uint32_t* stage, uint32_t* gbits, TileAcc& acc) {
+                                          uint32_t* stage, uint64_t* lx, uint32_t* gbits,
+                                          TileAcc& acc) {
     const unsigned lane = __lane_id();
     const uint32_t L = in.L, W = L + 1;
-    uint64_t r_lo = 0, r_hi = 0;
-    if (lane == 0) r_lo = find_row(in.csum, in.row_lo, in.row_hi, t0);
-    if (lane == 1) r_hi = find_row(in.csum, in.row_lo, in.row_hi, t1 - 1);
-    r_lo = __shfl(r_lo, 0, 64);
-    r_hi = __shfl(r_hi, 1, 64);
+    const uint64_t r_lo = wave_find_row(in.csum, in.row_lo, in.row_hi, t0);
+    const uint64_t r_hi = wave_find_row(in.csum, r_lo, in.row_hi, t1 - 1);
     auto rowp = [&](uint64_t r) { return in.rows + (r - in.row_base) * L; };
 
     // anchors (back-edge indices) whose segment is also the fused last
@@ -330,6 +431,17 @@ __device__ __forceinline__ void walk_tile(const DevGraph& g, const gv_step_t& st
             six[it] = shared ? (uint32_t)(c - first) : ~0u;
         }
     } else {
+        // rows r_lo..r_hi hold the tile: when they fit, stage their candidate
+        // starts in LDS (lx[i] = first candidate of row r_lo + i, lx[nr] = end)
+        const uint64_t nr = r_hi - r_lo + 1;
+        const bool staged = nr <= 64ull * IPT;
+        if (staged) {
+            for (uint32_t i = lane; i <= nr; i += 64)
+                lx[i] = i < nr ? in.csum[r_lo + i] - in.cnt[r_lo + i] : in.csum[r_hi];
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        }
 #pragma unroll
         for (int it = 0; it < IPT; ++it) {
             const uint64_t c = t0 + (uint64_t)it * 64 + lane;
@@ -337,8 +449,25 @@ __device__ __forceinline__ void walk_tile(const DevGraph& g, const gv_step_t& st
             uint32_t v = 0, sx = ~0u;
             uint64_t r = r_lo;
             if (c < t1) {
-                r = find_row(in.csum, r_lo, r_hi + 1, c);
-                const uint64_t first = in.csum[r] - in.cnt[r];
+                uint64_t first;
+                if (staged) {
+                    // first i with lx[i + 1] > c
+                    uint32_t lo = 0, n = (uint32_t)nr;
+                    while (n > 0) {
+                        const uint32_t h = n >> 1;
+                        if (lx[lo + h + 1] <= c) {
+                            lo += h + 1;
+                            n -= h + 1;
+                        } else {
+                            n = h;
+                        }
+                    }
+                    r = r_lo + lo;
+                    first = lx[lo];
+                } else {
+                    r = find_row(in.csum, r_lo, r_hi + 1, c);
+                    first = in.csum[r] - in.cnt[r];
+                }
                 v = g.nbr[in.anc_beg[r] + (uint32_t)(c - first)];
                 acc.rq += 4;
                 const uint32_t* row = rowp(r);
@@ -362,6 +491,7 @@ __device__ __forceinline__ void walk_tile(const DevGraph& g, const gv_step_t& st
             rr[it] = r;
             six[it] = sx;
         }
+        if (staged) __builtin_amdgcn_wave_barrier();
     }
 
     if (kCount) {
@@ -447,6 +577,7 @@ struct gv_ctx {
     std::vector<Rec> recs;
     std::vector<hipEvent_t> pool;
     unsigned long long* preq = nullptr;   // requested bytes per kind (device)
+    uint64_t units[8] = {};               // rows or candidates per kind (host)
     uint64_t scan_req = 0;                // scan bytes (computed on the host)
 };
 
@@ -511,8 +642,9 @@ struct ProfScope {
     gv_ctx* c;
     int kind;
     hipEvent_t a = nullptr;
-    ProfScope(gv_ctx* c_, int k) : c(c_), kind(k) {
+    ProfScope(gv_ctx* c_, int k, uint64_t units = 0) : c(c_), kind(k) {
         if (!c->prof) return;
+        c->units[k] += units;
         a = pool_get(c);
         if (a) (void)hipEventRecord(a, c->stream);
     }

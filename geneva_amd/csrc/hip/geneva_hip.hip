@@ -77,15 +77,7 @@ __global__ __launch_bounds__(kBlock) void k_size(SizeArgs a) {
 }
 
 // ---------------------------------------------------------------------------
-// count by rows: the last materialised step has a single back edge, so a row's
-// candidates are one label segment cut at the restriction, and its survivors
-// are that range minus the same-label values of the row found in it.  The
-// count needs no walk over candidates:
-//   mode 0 (last step):            s
-//   mode 1 (tail independent of v): s * T, T = the last vertex's count for the row
-//   mode 2 (tail = a second vertex below v in the same segment, same label):
-//                                   s (s - 1) / 2, the ordered pairs of survivors
-// (s = survivors of the row).
+// count by rows (row_count in common.h): one thread per row, no candidate walk
 struct RowsArgs {
     DevGraph g;
     const uint32_t* rows;
@@ -103,45 +95,14 @@ struct RowsArgs {
 __global__ __launch_bounds__(kBlock) void k_count_rows(RowsArgs a) {
     uint64_t cnt = 0, by = 0, rq = 0;
     uint32_t last_gid = 0xffffffffu;
-    const uint32_t W = a.L + 1;
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r < a.nrows; r += stride) {
         const uint32_t* row = a.rows + r * a.L;
         uint32_t b, e;
         seg(a.g, a.st.back_part[0], row[a.st.back_col[0]], a.st.label, b, e);
         by += 4ull * a.L + 8ull + 4ull * (e - b);
-        const uint32_t bound = a.st.restrict_col != GV_NONE ? row[a.st.restrict_col] : 0xffffffffu;
-        // ids are below 0xffffffff, so an absent bound cuts nothing
-        const uint32_t cut = bound == 0xffffffffu ? e : lower_bound_u32(a.g.nbr, b, e, bound);
-        rq += 4ull * a.L + 8ull + (bound != 0xffffffffu ? 4ull * probes(e - b) : 0ull);
-        uint64_t s = cut - b;
-        for (uint32_t j = 0; j < a.st.ncoll; ++j) {
-            const uint32_t x = row[a.st.coll_col[j]];
-            if (x >= bound) continue;
-            rq += 4ull * probes(cut - b);
-            if (contains_u32(a.g.nbr, b, cut, x)) --s;
-        }
-        uint64_t n = s;
-        if (a.mode == 1 && s) {
-            uint32_t tb, te;
-            seg(a.g, a.tail.back_part[0], row[a.tail.back_col[0]], a.tail.label, tb, te);
-            const uint32_t tbound =
-                a.tail.restrict_col != GV_NONE ? row[a.tail.restrict_col] : 0xffffffffu;
-            const uint32_t tcut = tbound == 0xffffffffu ? te : lower_bound_u32(a.g.nbr, tb, te, tbound);
-            rq += 8ull + (tbound != 0xffffffffu ? 4ull * probes(te - tb) : 0ull);
-            uint64_t t = tcut - tb;
-            for (uint32_t j = 0; j < a.tail.ncoll; ++j) {
-                const uint32_t x = row[a.tail.coll_col[j]];
-                if (x >= tbound) continue;
-                rq += 4ull * probes(tcut - tb);
-                if (contains_u32(a.g.nbr, tb, tcut, x)) --t;
-            }
-            n = s * t;
-            by += s * (8ull * W + 8ull + 4ull * (te - tb));
-        } else if (a.mode == 2) {
-            n = s ? s * (s - 1) / 2 : 0;
-            by += s * (8ull * W + 8ull + 4ull * (e - b));
-        }
+        rq += 4ull * a.L + 8ull;
+        const uint64_t n = row_count(a.g, a.st, a.tail, a.mode, a.L, row, b, e, by, rq);
         cnt += n;
         if (n && a.gbits) {
             const uint32_t gid = a.g.gof[row[0]];
@@ -292,6 +253,7 @@ struct WalkArgs {
 template <bool kCount>
 __global__ __launch_bounds__(kBlock) void k_walk(WalkArgs a) {
     __shared__ uint32_t stage[kWavesPerBlock][64 * GV_MAXK];
+    __shared__ uint64_t lx[kWavesPerBlock][kTile + 1];
     const unsigned lane = lane_id();
     const unsigned wib = threadIdx.x >> 6;
     const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
@@ -311,7 +273,7 @@ __global__ __launch_bounds__(kBlock) void k_walk(WalkArgs a) {
     for (uint64_t t0 = a.c0 + wave * kTile; t0 < a.c1; t0 += nwaves * kTile) {
         const uint64_t t1 = t0 + kTile < a.c1 ? t0 + kTile : a.c1;
         walk_tile<kCount, kIPT>(a.g, a.st, a.tail, a.has_tail != 0, in, t0, t1, o,
-                                kCount ? nullptr : stage[wib], a.gbits, acc);
+                                kCount ? nullptr : stage[wib], lx[wib], a.gbits, acc);
     }
     if (kCount) {
         acc.count = wave_sum_u64(acc.count);
@@ -353,7 +315,7 @@ int scan_counts(gv_ctx* c, const uint32_t* cnt, uint64_t n, uint64_t* csum) {
         }
         c->tiles_cap = cap;
     }
-    ProfScope ps(c, kKindScan);
+    ProfScope ps(c, kKindScan, n);
     if (c->prof) c->scan_req += 16ull * n + 24ull * ntiles;
     hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)ntiles), dim3(kBlock), 0, c->stream, cnt, n,
                        c->tiles);
@@ -722,7 +684,7 @@ int gv_size_rows(gv_ctx_t c, gv_graph_t g, const gv_step_t* st, const uint32_t* 
     a.req = req_ptr(c, kKindSize);
     a.st = *st;
     {
-        ProfScope ps(c, kKindSize);
+        ProfScope ps(c, kKindSize, nrows);
         hipLaunchKernelGGL(k_size, dim3(grid_for(nrows, kBlock, c->num_cu * 16)), dim3(kBlock), 0,
                            c->stream, a);
     }
@@ -741,7 +703,7 @@ static int launch_walk(gv_ctx* c, WalkArgs& a, bool count) {
     const uint64_t tiles = (span + kTile - 1) / kTile;
     const int blocks = grid_for(tiles, kWavesPerBlock, c->num_cu * 32);
     a.req = req_ptr(c, count ? kKindCount : kKindExt);
-    ProfScope ps(c, count ? kKindCount : kKindExt);
+    ProfScope ps(c, count ? kKindCount : kKindExt, span);
     if (count)
         hipLaunchKernelGGL(k_walk<true>, dim3(blocks), dim3(kBlock), 0, c->stream, a);
     else
@@ -813,30 +775,14 @@ int gv_count_emb(gv_ctx_t c, gv_graph_t g, const gv_step_t* st, const gv_step_t*
 }
 
 int gv_count_rows_mode(const gv_step_t* st, const gv_step_t* tail, uint32_t L) {
-    if (!st || st->nback != 1) return -1;
-    if (!tail) return 0;
-    if (tail->nback != 1) return -1;
-    bool collL = false;
-    for (uint32_t j = 0; j < tail->ncoll; ++j) collL |= tail->coll_col[j] == L;
-    if (tail->back_col[0] != L && tail->restrict_col != L && !collL) return 1;
-    // the last vertex draws from v's own segment and must lie below v: its
-    // same-label columns are then v and exactly those of v's step
-    if (tail->back_col[0] == st->back_col[0] && tail->back_part[0] == st->back_part[0] &&
-        tail->label == st->label && tail->restrict_col == L && collL &&
-        tail->ncoll == st->ncoll + 1) {
-        for (uint32_t j = 0; j < st->ncoll; ++j)
-            if (tail->coll_col[j] != st->coll_col[j]) return -1;
-        if (tail->coll_col[st->ncoll] != L) return -1;
-        return 2;
-    }
-    return -1;
+    return count_rows_mode(st, tail, L);
 }
 
 int gv_count_rows(gv_ctx_t c, gv_graph_t g, const gv_step_t* st, const gv_step_t* tail,
                   const uint32_t* rows, uint32_t L, uint64_t nrows) {
     if (!c || !g || !valid_step(st, L, g)) return GV_ERR_ARG;
     if (tail && !valid_tail(tail, L, g)) return GV_ERR_ARG;
-    const int mode = gv_count_rows_mode(st, tail, L);
+    const int mode = count_rows_mode(st, tail, L);
     if (mode < 0) return GV_ERR_ARG;
     if (nrows == 0) return GV_OK;
     if (!rows) return GV_ERR_ARG;
@@ -855,7 +801,7 @@ int gv_count_rows(gv_ctx_t c, gv_graph_t g, const gv_step_t* st, const gv_step_t
     a.req = req_ptr(c, kKindRows);
     a.gbits = (g->d_gof && c->gbits && c->gbits_words * 32ull >= g->ngraphs) ? c->gbits : nullptr;
     {
-        ProfScope ps(c, kKindRows);
+        ProfScope ps(c, kKindRows, nrows);
         hipLaunchKernelGGL(k_count_rows, dim3(grid_for(nrows, kBlock, c->num_cu * 16)), dim3(kBlock),
                            0, c->stream, a);
     }
@@ -939,7 +885,14 @@ int gv_profile_reset(gv_ctx_t c) {
     }
     c->recs.clear();
     c->scan_req = 0;
+    for (auto& u : c->units) u = 0;
     if (c->preq) HIPCHK(hipMemset(c->preq, 0, kKinds * sizeof(unsigned long long)));
+    return GV_OK;
+}
+
+int gv_profile_units(gv_ctx_t c, int kind, uint64_t* units) {
+    if (!c || !units || kind < 0 || kind >= kKinds) return GV_ERR_ARG;
+    *units = c->units[kind];
     return GV_OK;
 }
 

@@ -85,6 +85,7 @@ int BatchMiner::run(const std::vector<CompiledQuery>& qs, std::vector<MineResult
     std::vector<size_t> slotQuery(opt_.slots, SIZE_MAX);
     std::vector<size_t> overflow;
     std::vector<gv_slot_info_t> info(opt_.slots);
+    std::vector<uint32_t> bits((size_t)opt_.slots * nwords_);
     BatchStats local;
     uint32_t running = 0;
     gv_mark(ctx_, 2);
@@ -105,7 +106,9 @@ int BatchMiner::run(const std::vector<CompiledQuery>& qs, std::vector<MineResult
         rc = gv_batch_run(b_, iters);
         if (rc != GV_OK) return rc;
         local.graphLaunches += iters;
-        rc = gv_batch_poll(b_, info.data());
+        // states and every slot's graph bits in one synchronised copy
+        rc = nwords_ ? gv_batch_poll_bits(b_, info.data(), bits.data(), nwords_)
+                     : gv_batch_poll(b_, info.data());
         if (rc != GV_OK) return rc;
         for (uint32_t s = 0; s < opt_.slots; ++s) {
             if (slotQuery[s] == SIZE_MAX) continue;
@@ -114,8 +117,8 @@ int BatchMiner::run(const std::vector<CompiledQuery>& qs, std::vector<MineResult
                 MineResult& r = out[qi];
                 r.count = info[s].count;
                 r.algBytes = info[s].alg_bytes;
-                r.graphBits.assign(nwords_, 0);
-                rc = gv_batch_release(b_, s, nwords_ ? r.graphBits.data() : nullptr, nwords_);
+                r.graphBits.assign(bits.begin() + (size_t)s * nwords_, bits.begin() + (size_t)(s + 1) * nwords_);
+                rc = gv_batch_release(b_, s, nullptr, 0);
                 if (rc != GV_OK) return rc;
                 slotQuery[s] = SIZE_MAX;
                 --running;

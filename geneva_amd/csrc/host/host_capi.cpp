@@ -5,6 +5,7 @@
 #include <memory>
 #include <sstream>
 #include <string>
+#include <thread>
 
 #include "../../../include/geneva_hip.h"
 #include "gen.h"
@@ -44,16 +45,32 @@ int ensureBatch(gh_engine* e) {
     }
     return GV_OK;
 }
+// Plans and compiles queries on up to 16 host threads (each plan is
+// independent; the graph is only read).
 int compileAll(gh_engine* e, const char* const* qs, uint32_t n, int single_v,
                std::vector<geneva::CompiledQuery>& out) {
     out.assign(n, geneva::CompiledQuery());
-    for (uint32_t i = 0; i < n; ++i) {
-        std::string err;
-        if (!qs[i] || !geneva::planAndCompile(qs[i], single_v != 0, e->graph, out[i], &err)) {
-            setErr("query " + std::to_string(i) + ": " + err);
+    std::vector<std::string> err(n);
+    std::vector<char> ok(n, 0);
+    auto work = [&](uint32_t lo, uint32_t hi) {
+        for (uint32_t i = lo; i < hi; ++i)
+            ok[i] = qs[i] && geneva::planAndCompile(qs[i], single_v != 0, e->graph, out[i], &err[i]);
+    };
+    const uint32_t nt = std::min<uint32_t>(std::max(1u, std::thread::hardware_concurrency()),
+                                           std::min<uint32_t>(16, (n + 7) / 8));
+    if (nt <= 1) {
+        work(0, n);
+    } else {
+        std::vector<std::thread> th;
+        const uint32_t per = (n + nt - 1) / nt;
+        for (uint32_t t = 0; t < nt; ++t) th.emplace_back(work, std::min(n, t * per), std::min(n, (t + 1) * per));
+        for (auto& t : th) t.join();
+    }
+    for (uint32_t i = 0; i < n; ++i)
+        if (!ok[i]) {
+            setErr("query " + std::to_string(i) + ": " + err[i]);
             return GV_ERR_ARG;
         }
-    }
     return GV_OK;
 }
 }  // namespace
@@ -475,6 +492,10 @@ int gh_engine_profile_reset(gh_engine_t e) { return e ? gv_profile_reset(e->ctx)
 int gh_engine_profile_read(gh_engine_t e, int kind, uint64_t* launches, double* total_ms,
                            uint64_t* req_bytes) {
     return e ? gv_profile_read(e->ctx, kind, launches, total_ms, req_bytes) : GV_ERR_ARG;
+}
+
+int gh_engine_profile_units(gh_engine_t e, int kind, uint64_t* units) {
+    return e ? gv_profile_units(e->ctx, kind, units) : GV_ERR_ARG;
 }
 
 }  // extern "C"

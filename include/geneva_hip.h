@@ -227,7 +227,12 @@ int gv_batch_admit(gv_batch_t b, const gv_query_t* q, uint64_t tag, uint32_t* sl
 int gv_batch_run(gv_batch_t b, uint32_t iterations);
 /* Copy every slot's state to info[0..slots) (synchronises). */
 int gv_batch_poll(gv_batch_t b, gv_slot_info_t* info);
-/* Free a finished slot; graph_bits (nullable) receives its matched graph ids. */
+/* As gv_batch_poll, and every slot's matched-graph bitset into
+ * bits[slot * nwords ..] (nwords = ceil(graphs / 32), all slots, one copy). */
+int gv_batch_poll_bits(gv_batch_t b, gv_slot_info_t* info, uint32_t* bits, uint32_t nwords);
+/* Free a slot (finished or not: a running slot stops).  graph_bits (nullable)
+ * receives its matched graph ids, which synchronises; without it the call
+ * only enqueues the reset. */
 int gv_batch_release(gv_batch_t b, uint32_t slot, uint32_t* graph_bits, uint32_t nwords);
 /* Slots, rows a slot table holds at a given width, free slots. */
 int gv_batch_info(gv_batch_t b, uint32_t* slots, uint64_t* slot_bytes, uint32_t* free_slots);
@@ -249,6 +254,9 @@ int gv_profile_enable(gv_ctx_t ctx, int on);
 int gv_profile_reset(gv_ctx_t ctx);
 int gv_profile_read(gv_ctx_t ctx, int kind, uint64_t* launches, double* total_ms,
                     uint64_t* req_bytes);
+/* Work units of a kind since the last reset: rows (size, scan, rows kinds) or
+ * candidates (extend, count); 0 for batch iterations (extents on the device). */
+int gv_profile_units(gv_ctx_t ctx, int kind, uint64_t* units);
 
 /* Timing on the context's stream (HIP events): record a mark, and the
  * milliseconds between two marks (synchronises on the second). */
