@@ -577,8 +577,10 @@ struct gv_ctx {
     std::vector<Rec> recs;
     std::vector<hipEvent_t> pool;
     unsigned long long* preq = nullptr;   // requested bytes per kind (device)
-    uint64_t units[8] = {};               // rows or candidates per kind (host)
-    uint64_t scan_req = 0;                // scan bytes (computed on the host)
+    unsigned long long* punits = nullptr; // rows or candidates per kind (device)
+    // rows of each level's table when a whole query runs with extents in
+    // device memory (gv_mine_levels)
+    unsigned long long* lvl = nullptr;
 };
 
 struct gv_graph {
@@ -642,9 +644,8 @@ struct ProfScope {
     gv_ctx* c;
     int kind;
     hipEvent_t a = nullptr;
-    ProfScope(gv_ctx* c_, int k, uint64_t units = 0) : c(c_), kind(k) {
+    ProfScope(gv_ctx* c_, int k) : c(c_), kind(k) {
         if (!c->prof) return;
-        c->units[k] += units;
         a = pool_get(c);
         if (a) (void)hipEventRecord(a, c->stream);
     }
@@ -659,6 +660,10 @@ struct ProfScope {
 
 unsigned long long* req_ptr(gv_ctx* c, int kind) {
     return (c->prof && c->preq) ? c->preq + kind : nullptr;
+}
+
+unsigned long long* units_ptr(gv_ctx* c, int kind) {
+    return (c->prof && c->punits) ? c->punits + kind : nullptr;
 }
 
 int grid_for(uint64_t items, uint64_t per_block, int cap_blocks) {

@@ -38,13 +38,29 @@ struct SizeArgs {
     uint8_t* anc_i;
     unsigned long long* bytes;
     unsigned long long* req;   // bytes this launch reads and writes, each access once
+    unsigned long long* units; // rows sized (profiling)
+    // extent in device memory (nullable): rows = min(*nrows_dev, nrows), none
+    // when *stop is set
+    const unsigned long long* nrows_dev;
+    const unsigned long long* stop;
     gv_step_t st;
 };
 
+// rows of a launch whose extent may live in device memory
+__device__ __forceinline__ uint64_t dev_rows(uint64_t n, const unsigned long long* n_dev,
+                                             const unsigned long long* stop) {
+    if (stop && *stop) return 0;
+    if (!n_dev) return n;
+    const uint64_t d = *n_dev;
+    return d < n ? d : n;
+}
+
 __global__ __launch_bounds__(kBlock) void k_size(SizeArgs a) {
     uint64_t local = 0, rq = 0;
+    const uint64_t nrows = dev_rows(a.nrows, a.nrows_dev, a.stop);
+    if (a.units && blockIdx.x == 0 && threadIdx.x == 0 && nrows) atomicAdd(a.units, (unsigned long long)nrows);
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r < a.nrows; r += stride) {
+    for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r < nrows; r += stride) {
         const uint32_t* row = a.rows + r * a.L;
         uint32_t best_b = 0, best_len = 0xffffffffu, best_i = 0;
         local += 4ull * a.L;
@@ -89,14 +105,19 @@ struct RowsArgs {
     unsigned long long* count;
     unsigned long long* bytes;
     unsigned long long* req;
+    unsigned long long* units;
+    const unsigned long long* nrows_dev;   // as in SizeArgs
+    const unsigned long long* stop;
     uint32_t* gbits;
 };
 
 __global__ __launch_bounds__(kBlock) void k_count_rows(RowsArgs a) {
     uint64_t cnt = 0, by = 0, rq = 0;
     uint32_t last_gid = 0xffffffffu;
+    const uint64_t nrows = dev_rows(a.nrows, a.nrows_dev, a.stop);
+    if (a.units && blockIdx.x == 0 && threadIdx.x == 0 && nrows) atomicAdd(a.units, (unsigned long long)nrows);
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r < a.nrows; r += stride) {
+    for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r < nrows; r += stride) {
         const uint32_t* row = a.rows + r * a.L;
         uint32_t b, e;
         seg(a.g, a.st.back_part[0], row[a.st.back_col[0]], a.st.label, b, e);
@@ -149,28 +170,48 @@ __device__ __forceinline__ uint64_t block_exclusive_scan(uint64_t v, uint64_t* w
     return off + x - v;
 }
 
-__global__ __launch_bounds__(kBlock) void k_scan_reduce(const uint32_t* in, uint64_t n,
+// The scan kernels stride over tiles of kScanTile counts; n may come from
+// device memory (n_dev, clamped to n; nothing when *stop is set).
+__global__ __launch_bounds__(kBlock) void k_scan_reduce(const uint32_t* in, uint64_t n_max,
+                                                        const unsigned long long* n_dev,
+                                                        const unsigned long long* stop,
                                                         uint64_t* tile_sums) {
     __shared__ uint64_t wsum[kWavesPerBlock];
-    const uint64_t base = (uint64_t)blockIdx.x * kScanTile;
-    uint64_t s = 0;
-    for (int i = 0; i < kScanItems; ++i) {
-        uint64_t idx = base + (uint64_t)i * kBlock + threadIdx.x;
-        if (idx < n) s += in[idx];
-    }
-    s = wave_sum_u64(s);
-    if (lane_id() == 0) wsum[threadIdx.x >> 6] = s;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        uint64_t t = 0;
-        for (int i = 0; i < kWavesPerBlock; ++i) t += wsum[i];
-        tile_sums[blockIdx.x] = t;
+    const uint64_t n = dev_rows(n_max, n_dev, stop);
+    const uint64_t ntiles = (n + kScanTile - 1) / kScanTile;
+    for (uint64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const uint64_t base = t * kScanTile;
+        uint64_t s = 0;
+        for (int i = 0; i < kScanItems; ++i) {
+            uint64_t idx = base + (uint64_t)i * kBlock + threadIdx.x;
+            if (idx < n) s += in[idx];
+        }
+        s = wave_sum_u64(s);
+        if (lane_id() == 0) wsum[threadIdx.x >> 6] = s;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint64_t x = 0;
+            for (int i = 0; i < kWavesPerBlock; ++i) x += wsum[i];
+            tile_sums[t] = x;
+        }
+        __syncthreads();
     }
 }
 
 // exclusive scan of tile sums in place, one workgroup
-__global__ __launch_bounds__(kBlock) void k_scan_tiles(uint64_t* tile_sums, uint64_t ntiles) {
+__global__ __launch_bounds__(kBlock) void k_scan_tiles(uint64_t* tile_sums, uint64_t n_max,
+                                                       const unsigned long long* n_dev,
+                                                       const unsigned long long* stop,
+                                                       unsigned long long* units,
+                                                       unsigned long long* req) {
     __shared__ uint64_t wsum[kWavesPerBlock];
+    const uint64_t n = dev_rows(n_max, n_dev, stop);
+    const uint64_t ntiles = (n + kScanTile - 1) / kScanTile;
+    if (threadIdx.x == 0 && n) {
+        if (units) atomicAdd(units, (unsigned long long)n);
+        // counts read twice, sums written; tile sums read and written
+        if (req) atomicAdd(req, (unsigned long long)(16ull * n + 24ull * ntiles));
+    }
     uint64_t carry = 0;
     for (uint64_t b = 0; b < ntiles; b += kBlock) {
         uint64_t idx = b + threadIdx.x;
@@ -183,11 +224,17 @@ __global__ __launch_bounds__(kBlock) void k_scan_tiles(uint64_t* tile_sums, uint
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_scan_final(const uint32_t* in, uint64_t n,
+__global__ __launch_bounds__(kBlock) void k_scan_final(const uint32_t* in, uint64_t n_max,
+                                                       const unsigned long long* n_dev,
+                                                       const unsigned long long* stop,
                                                        const uint64_t* tile_off, uint64_t* out) {
     __shared__ uint32_t vals[kScanTile];
     __shared__ uint64_t wsum[kWavesPerBlock];
-    const uint64_t base = (uint64_t)blockIdx.x * kScanTile;
+    __shared__ uint64_t outv[kScanTile / 2];
+    const uint64_t n = dev_rows(n_max, n_dev, stop);
+    const uint64_t ntiles = (n + kScanTile - 1) / kScanTile;
+    for (uint64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const uint64_t base = t * kScanTile;
     for (int i = 0; i < kScanItems; ++i) {
         uint64_t idx = base + (uint64_t)i * kBlock + threadIdx.x;
         vals[i * kBlock + threadIdx.x] = idx < n ? in[idx] : 0u;
@@ -201,9 +248,8 @@ __global__ __launch_bounds__(kBlock) void k_scan_final(const uint32_t* in, uint6
         loc[i] = s;
     }
     uint64_t tot;
-    uint64_t ex = block_exclusive_scan(s, wsum, tot) + tile_off[blockIdx.x];
+    uint64_t ex = block_exclusive_scan(s, wsum, tot) + tile_off[t];
     // write back through LDS in two halves to keep coalesced stores
-    __shared__ uint64_t outv[kScanTile / 2];
     for (int half = 0; half < 2; ++half) {
         __syncthreads();
         const int lo = half * (kScanTile / 2);
@@ -217,6 +263,8 @@ __global__ __launch_bounds__(kBlock) void k_scan_final(const uint32_t* in, uint6
             uint64_t idx = base + lo + i;
             if (idx < n) out[idx] = outv[i];
         }
+    }
+    __syncthreads();
     }
 }
 
@@ -243,6 +291,11 @@ struct WalkArgs {
     uint32_t* gbits;
     unsigned long long* bytes;
     unsigned long long* req;   // bytes this launch reads and writes, each access once
+    unsigned long long* units; // candidates walked (profiling)
+    // extent in device memory (nullable): rows = min(*nrows_dev, nrows) and
+    // the candidates are all of theirs, [0, csum[rows - 1]); none when *stop
+    const unsigned long long* nrows_dev;
+    const unsigned long long* stop;
     gv_step_t st;
     gv_step_t tail;
     int has_tail;
@@ -258,6 +311,14 @@ __global__ __launch_bounds__(kBlock) void k_walk(WalkArgs a) {
     const unsigned wib = threadIdx.x >> 6;
     const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+    uint64_t nrows = a.nrows, c0 = a.c0, c1 = a.c1;
+    if (a.nrows_dev || a.stop) {
+        nrows = dev_rows(a.nrows, a.nrows_dev, a.stop);
+        c0 = 0;
+        c1 = nrows ? a.csum[nrows - 1] : 0;
+    }
+    if (a.units && blockIdx.x == 0 && threadIdx.x == 0 && c1 > c0)
+        atomicAdd(a.units, (unsigned long long)(c1 - c0));
     TileIn in;
     in.rows = a.rows;
     in.row_base = 0;
@@ -267,11 +328,11 @@ __global__ __launch_bounds__(kBlock) void k_walk(WalkArgs a) {
     in.anc_beg = a.anc_beg;
     in.anc_i = a.anc_i;
     in.row_lo = 0;
-    in.row_hi = a.nrows;
+    in.row_hi = nrows;
     TileOut o{a.out, a.out_cap, a.out_n, a.overflow};
     TileAcc acc;
-    for (uint64_t t0 = a.c0 + wave * kTile; t0 < a.c1; t0 += nwaves * kTile) {
-        const uint64_t t1 = t0 + kTile < a.c1 ? t0 + kTile : a.c1;
+    for (uint64_t t0 = c0 + wave * kTile; t0 < c1; t0 += nwaves * kTile) {
+        const uint64_t t1 = t0 + kTile < c1 ? t0 + kTile : c1;
         walk_tile<kCount, kIPT>(a.g, a.st, a.tail, a.has_tail != 0, in, t0, t1, o,
                                 kCount ? nullptr : stage[wib], lx[wib], a.gbits, acc);
     }
@@ -285,6 +346,10 @@ __global__ __launch_bounds__(kBlock) void k_walk(WalkArgs a) {
         acc.rq = wave_sum_u64(acc.rq);
         if (lane == 0 && acc.rq) atomicAdd(a.req, (unsigned long long)acc.rq);
     }
+}
+
+__global__ void k_zero_u64(unsigned long long* p, uint32_t n) {
+    if (threadIdx.x < n) p[threadIdx.x] = 0;
 }
 
 __global__ void k_copy_back(const uint32_t* src, uint32_t* dst, uint64_t n) {
@@ -303,7 +368,7 @@ __global__ void k_copy_back(const uint32_t* src, uint32_t* dst, uint64_t n) {
 
 namespace {
 
-int scan_counts(gv_ctx* c, const uint32_t* cnt, uint64_t n, uint64_t* csum) {
+int ensure_tiles(gv_ctx* c, uint64_t n) {
     const uint64_t ntiles = (n + kScanTile - 1) / kScanTile;
     if (ntiles > c->tiles_cap) {
         if (c->tiles) HIPCHK(hipFree(c->tiles));
@@ -315,12 +380,23 @@ int scan_counts(gv_ctx* c, const uint32_t* cnt, uint64_t n, uint64_t* csum) {
         }
         c->tiles_cap = cap;
     }
-    ProfScope ps(c, kKindScan, n);
-    if (c->prof) c->scan_req += 16ull * n + 24ull * ntiles;
-    hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)ntiles), dim3(kBlock), 0, c->stream, cnt, n,
+    return GV_OK;
+}
+
+// Inclusive scan of n counts (n_max when n_dev is null, else min(*n_dev,
+// n_max) read on the device; nothing when *stop is set).
+int scan_counts(gv_ctx* c, const uint32_t* cnt, uint64_t n_max, const unsigned long long* n_dev,
+                const unsigned long long* stop, uint64_t* csum) {
+    int rc = ensure_tiles(c, n_max);
+    if (rc != GV_OK) return rc;
+    const uint64_t ntiles = (n_max + kScanTile - 1) / kScanTile;
+    const unsigned grid = (unsigned)std::min<uint64_t>(std::max<uint64_t>(ntiles, 1), 8ull * c->num_cu);
+    ProfScope ps(c, kKindScan);
+    hipLaunchKernelGGL(k_scan_reduce, dim3(grid), dim3(kBlock), 0, c->stream, cnt, n_max, n_dev, stop,
                        c->tiles);
-    hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(kBlock), 0, c->stream, c->tiles, ntiles);
-    hipLaunchKernelGGL(k_scan_final, dim3((unsigned)ntiles), dim3(kBlock), 0, c->stream, cnt, n,
+    hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(kBlock), 0, c->stream, c->tiles, n_max, n_dev,
+                       stop, units_ptr(c, kKindScan), req_ptr(c, kKindScan));
+    hipLaunchKernelGGL(k_scan_final, dim3(grid), dim3(kBlock), 0, c->stream, cnt, n_max, n_dev, stop,
                        c->tiles, csum);
     HIPCHK(hipGetLastError());
     return GV_OK;
@@ -372,6 +448,12 @@ int gv_ctx_create(int device, gv_ctx_t* out) {
         return GV_ERR_NOMEM;
     }
     HIPCHK(hipMemset(c->acc, 0, 8 * sizeof(unsigned long long)));
+    if (hipMalloc(&c->lvl, (GV_MAXK + 1) * sizeof(unsigned long long)) != hipSuccess) {
+        hipFree(c->acc);
+        delete c;
+        return GV_ERR_NOMEM;
+    }
+    HIPCHK(hipMemset(c->lvl, 0, (GV_MAXK + 1) * sizeof(unsigned long long)));
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->num_cu = prop.multiProcessorCount;
     *out = c;
@@ -386,6 +468,8 @@ int gv_ctx_destroy(gv_ctx_t c) {
     if (c->gbits) hipFree(c->gbits);
     if (c->tiles) hipFree(c->tiles);
     if (c->preq) hipFree(c->preq);
+    if (c->punits) hipFree(c->punits);
+    if (c->lvl) hipFree(c->lvl);
     for (auto& r : c->recs) {
         hipEventDestroy(r.a);
         hipEventDestroy(r.b);
@@ -673,6 +757,7 @@ int gv_size_rows(gv_ctx_t c, gv_graph_t g, const gv_step_t* st, const uint32_t* 
     f->total = 0;
     if (nrows == 0) return GV_OK;
     SizeArgs a;
+    memset(&a, 0, sizeof(a));
     a.g = dev_view(g);
     a.rows = rows;
     a.nrows = nrows;
@@ -682,14 +767,15 @@ int gv_size_rows(gv_ctx_t c, gv_graph_t g, const gv_step_t* st, const uint32_t* 
     a.anc_i = f->anc_i;
     a.bytes = c->acc + 1;
     a.req = req_ptr(c, kKindSize);
+    a.units = units_ptr(c, kKindSize);
     a.st = *st;
     {
-        ProfScope ps(c, kKindSize, nrows);
+        ProfScope ps(c, kKindSize);
         hipLaunchKernelGGL(k_size, dim3(grid_for(nrows, kBlock, c->num_cu * 16)), dim3(kBlock), 0,
                            c->stream, a);
     }
     HIPCHK(hipGetLastError());
-    int rc = scan_counts(c, f->cnt, nrows, f->csum);
+    int rc = scan_counts(c, f->cnt, nrows, nullptr, nullptr, f->csum);
     if (rc != GV_OK) return rc;
     HIPCHK(hipMemcpyAsync(total, f->csum + (nrows - 1), sizeof(uint64_t), hipMemcpyDeviceToHost,
                           c->stream));
@@ -703,7 +789,8 @@ static int launch_walk(gv_ctx* c, WalkArgs& a, bool count) {
     const uint64_t tiles = (span + kTile - 1) / kTile;
     const int blocks = grid_for(tiles, kWavesPerBlock, c->num_cu * 32);
     a.req = req_ptr(c, count ? kKindCount : kKindExt);
-    ProfScope ps(c, count ? kKindCount : kKindExt, span);
+    a.units = units_ptr(c, count ? kKindCount : kKindExt);
+    ProfScope ps(c, count ? kKindCount : kKindExt);
     if (count)
         hipLaunchKernelGGL(k_walk<true>, dim3(blocks), dim3(kBlock), 0, c->stream, a);
     else
@@ -799,13 +886,156 @@ int gv_count_rows(gv_ctx_t c, gv_graph_t g, const gv_step_t* st, const gv_step_t
     a.count = c->acc + 0;
     a.bytes = c->acc + 1;
     a.req = req_ptr(c, kKindRows);
+    a.units = units_ptr(c, kKindRows);
     a.gbits = (g->d_gof && c->gbits && c->gbits_words * 32ull >= g->ngraphs) ? c->gbits : nullptr;
     {
-        ProfScope ps(c, kKindRows, nrows);
+        ProfScope ps(c, kKindRows);
         hipLaunchKernelGGL(k_count_rows, dim3(grid_for(nrows, kBlock, c->num_cu * 16)), dim3(kBlock),
                            0, c->stream, a);
     }
     HIPCHK(hipGetLastError());
+    return GV_OK;
+}
+
+int gv_mine_levels(gv_ctx_t c, gv_graph_t g, const gv_query_t* q, const uint32_t* start_rows,
+                   uint64_t nstart, uint32_t* const* tables, const uint64_t* table_rows,
+                   const gv_frontier_t* frontiers, int count_by_rows, uint64_t* count,
+                   uint64_t* alg_bytes, uint32_t* bits, uint32_t nwords) {
+    if (!c || !g || !q || !tables || !table_rows || !frontiers || q->k < 2 || q->k > GV_MAXK)
+        return GV_ERR_ARG;
+    if (nstart && !start_rows) return GV_ERR_ARG;
+    if (bits && nwords > c->gbits_words) return GV_ERR_ARG;
+    const uint32_t k = q->k;
+    // the widths that are materialised or sized, and their capacities
+    uint32_t wlast = 1;   // width of the counted step's input
+    for (uint32_t w = 1; w < k; ++w) {
+        const gv_step_t& st = q->step[w];
+        if (!valid_step(&st, w, g)) return GV_ERR_ARG;
+        const bool last = w + 1 == k;
+        const bool fused = w + 2 == k && q->step[w + 1].nback == 1;
+        if (fused && !valid_tail(&q->step[w + 1], w, g)) return GV_ERR_ARG;
+        const uint64_t cap = w == 1 ? nstart : table_rows[w];
+        if (w >= 2 && (!tables[w] || cap == 0)) return GV_ERR_ARG;
+        wlast = w;
+        const bool by_rows = (last || fused) && count_by_rows &&
+                             count_rows_mode(&st, fused ? &q->step[w + 1] : nullptr, w) >= 0;
+        if (!by_rows && (!frontiers[w] || frontiers[w]->cap < cap)) return GV_ERR_ARG;
+        if (last || fused) break;
+        if (!tables[w + 1]) return GV_ERR_ARG;
+    }
+    HIPCHK(hipSetDevice(c->device));
+    const DevGraph dg = dev_view(g);
+    unsigned long long* stop = c->acc + 3;
+    uint32_t* gb = (g->d_gof && c->gbits && c->gbits_words * 32ull >= g->ngraphs) ? c->gbits : nullptr;
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, c->stream, c->lvl, (uint32_t)(GV_MAXK + 1));
+    const uint32_t* rows = start_rows;
+    for (uint32_t w = 1; w <= wlast && nstart; ++w) {
+        const gv_step_t& st = q->step[w];
+        const uint64_t cap = w == 1 ? nstart : table_rows[w];
+        const unsigned long long* ndev = w == 1 ? nullptr : c->lvl + w;
+        const bool last = w + 1 == k;
+        const bool fused = w + 2 == k && q->step[w + 1].nback == 1;
+        const gv_step_t* tail = fused ? &q->step[w + 1] : nullptr;
+        if ((last || fused) && count_by_rows && count_rows_mode(&st, tail, w) >= 0) {
+            RowsArgs a;
+            memset(&a, 0, sizeof(a));
+            a.g = dg;
+            a.rows = rows;
+            a.nrows = cap;
+            a.nrows_dev = ndev;
+            a.stop = stop;
+            a.L = w;
+            a.mode = count_rows_mode(&st, tail, w);
+            a.st = st;
+            if (tail) a.tail = *tail;
+            a.count = c->acc + 0;
+            a.bytes = c->acc + 1;
+            a.req = req_ptr(c, kKindRows);
+            a.units = units_ptr(c, kKindRows);
+            a.gbits = gb;
+            ProfScope ps(c, kKindRows);
+            hipLaunchKernelGGL(k_count_rows, dim3(grid_for(cap, kBlock, c->num_cu * 16)), dim3(kBlock),
+                               0, c->stream, a);
+            HIPCHK(hipGetLastError());
+            break;
+        }
+        gv_frontier* f = frontiers[w];
+        {
+            SizeArgs a;
+            memset(&a, 0, sizeof(a));
+            a.g = dg;
+            a.rows = rows;
+            a.nrows = cap;
+            a.nrows_dev = ndev;
+            a.stop = stop;
+            a.L = w;
+            a.anc_beg = f->anc_beg;
+            a.cnt = f->cnt;
+            a.anc_i = f->anc_i;
+            a.bytes = c->acc + 1;
+            a.req = req_ptr(c, kKindSize);
+            a.units = units_ptr(c, kKindSize);
+            a.st = st;
+            ProfScope ps(c, kKindSize);
+            hipLaunchKernelGGL(k_size, dim3(grid_for(cap, kBlock, c->num_cu * 16)), dim3(kBlock), 0,
+                               c->stream, a);
+            HIPCHK(hipGetLastError());
+        }
+        int rc = scan_counts(c, f->cnt, cap, ndev, stop, f->csum);
+        if (rc != GV_OK) return rc;
+        // the frontier's host-side extents are unknown here
+        f->nrows = 0;
+        f->total = 0;
+        WalkArgs a;
+        memset(&a, 0, sizeof(a));
+        a.g = dg;
+        a.rows = rows;
+        a.nrows = cap;
+        a.nrows_dev = ndev;   // null at level 1: rows = nstart, candidates from the scan
+        a.stop = stop;
+        a.L = w;
+        a.anc_beg = f->anc_beg;
+        a.cnt = f->cnt;
+        a.csum = f->csum;
+        a.anc_i = f->anc_i;
+        a.count = c->acc + 0;
+        a.bytes = c->acc + 1;
+        a.overflow = stop;
+        a.st = st;
+        const bool counting = last || fused;
+        if (counting) {
+            if (tail) {
+                a.tail = *tail;
+                a.has_tail = 1;
+            }
+            a.gbits = gb;
+        } else {
+            a.out = tables[w + 1];
+            a.out_cap = table_rows[w + 1];
+            a.out_n = c->lvl + w + 1;
+        }
+        a.req = req_ptr(c, counting ? kKindCount : kKindExt);
+        a.units = units_ptr(c, counting ? kKindCount : kKindExt);
+        {
+            ProfScope ps(c, counting ? kKindCount : kKindExt);
+            if (counting)
+                hipLaunchKernelGGL(k_walk<true>, dim3(c->num_cu * 32), dim3(kBlock), 0, c->stream, a);
+            else
+                hipLaunchKernelGGL(k_walk<false>, dim3(c->num_cu * 32), dim3(kBlock), 0, c->stream, a);
+            HIPCHK(hipGetLastError());
+        }
+        if (counting) break;
+        rows = tables[w + 1];
+    }
+    unsigned long long h[4];
+    HIPCHK(hipMemcpyAsync(h, c->acc, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+    if (bits && nwords)
+        HIPCHK(hipMemcpyAsync(bits, c->gbits, nwords * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                              c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (h[3]) return GV_ERR_CAPACITY;
+    if (count) *count = h[0];
+    if (alg_bytes) *alg_bytes = h[1];
     return GV_OK;
 }
 
@@ -872,6 +1102,11 @@ int gv_profile_enable(gv_ctx_t c, int on) {
         if (hipMalloc(&c->preq, kKinds * sizeof(unsigned long long)) != hipSuccess) return GV_ERR_NOMEM;
         HIPCHK(hipMemset(c->preq, 0, kKinds * sizeof(unsigned long long)));
     }
+    if (on && !c->punits) {
+        if (hipMalloc(&c->punits, kKinds * sizeof(unsigned long long)) != hipSuccess)
+            return GV_ERR_NOMEM;
+        HIPCHK(hipMemset(c->punits, 0, kKinds * sizeof(unsigned long long)));
+    }
     c->prof = on != 0;
     return GV_OK;
 }
@@ -884,15 +1119,19 @@ int gv_profile_reset(gv_ctx_t c) {
         c->pool.push_back(r.b);
     }
     c->recs.clear();
-    c->scan_req = 0;
-    for (auto& u : c->units) u = 0;
     if (c->preq) HIPCHK(hipMemset(c->preq, 0, kKinds * sizeof(unsigned long long)));
+    if (c->punits) HIPCHK(hipMemset(c->punits, 0, kKinds * sizeof(unsigned long long)));
     return GV_OK;
 }
 
 int gv_profile_units(gv_ctx_t c, int kind, uint64_t* units) {
     if (!c || !units || kind < 0 || kind >= kKinds) return GV_ERR_ARG;
-    *units = c->units[kind];
+    *units = 0;
+    if (c->punits) {
+        unsigned long long h[kKinds];
+        HIPCHK(hipMemcpy(h, c->punits, sizeof(h), hipMemcpyDeviceToHost));
+        *units = h[kind];
+    }
     return GV_OK;
 }
 
@@ -910,9 +1149,7 @@ int gv_profile_read(gv_ctx_t c, int kind, uint64_t* launches, double* total_ms,
         ++n;
     }
     uint64_t rq = 0;
-    if (kind == kKindScan) {
-        rq = c->scan_req;
-    } else if (c->preq) {
+    if (c->preq) {
         unsigned long long h[kKinds];
         HIPCHK(hipMemcpy(h, c->preq, sizeof(h), hipMemcpyDeviceToHost));
         rq = h[kind];

@@ -257,6 +257,43 @@ int PlainMiner::mine(const CompiledQuery& cq, MineResult& out) {
     return mine(cq, Shard(), out);
 }
 
+// The whole query with extents kept in device memory (gv_mine_levels): one
+// synchronisation at the end.  Tables that outgrow a level are detected on the
+// device; the query is then mined again, chunked, by run().
+int PlainMiner::runLevels(const CompiledQuery& cq, const uint32_t* rows, uint64_t n,
+                          MineResult& out) {
+    if (n == 0) return GV_OK;
+    const gv_query_t& q = cq.q;
+    uint32_t* tables[GV_MAXK + 1] = {};
+    uint64_t caps[GV_MAXK + 1] = {};
+    gv_frontier_t frs[GV_MAXK + 1] = {};
+    int rc = ensureFrontier(1, n);
+    if (rc != GV_OK) return rc;
+    frs[1] = lv_[1].fr;
+    for (uint32_t w = 2; w < q.k; ++w) {
+        // the last vertex, when it has one back edge, is counted with the step
+        // before it, so width k - 1 is then never materialised
+        if (w + 1 == q.k && q.step[w].nback == 1) break;
+        rc = ensureRows(w);
+        if (rc != GV_OK) return rc;
+        tables[w] = lv_[w].rows;
+        caps[w] = lv_[w].cap;
+        frs[w] = lv_[w].fr;
+    }
+    uint32_t words = 0;
+    gv_graph_owners(g_, &words);
+    words = (words + 31) / 32;
+    rc = gv_mine_levels(ctx_, g_, &q, rows, n, tables, caps, frs, countByRows_ ? 1 : 0, nullptr,
+                        nullptr, nullptr, 0);
+    out.launches++;
+    if (rc != GV_ERR_CAPACITY) return rc;
+    // a table outgrew its level: start over, chunked
+    out.overflowRetries++;
+    rc = gv_counters_reset(ctx_, g_);
+    if (rc != GV_OK) return rc;
+    return run(q, 1, rows, n, out);
+}
+
 int PlainMiner::mine(const CompiledQuery& cq, const Shard& sh, MineResult& out) {
     out = MineResult();
     if (sh.stride == 0) return out.status = GV_ERR_ARG;
@@ -293,7 +330,7 @@ int PlainMiner::mine(const CompiledQuery& cq, const Shard& sh, MineResult& out) 
                 n = pick.size();
             }
         }
-        if (rc == GV_OK) rc = run(cq.q, 1, rows, n, out);
+        if (rc == GV_OK) rc = runLevels(cq, rows, n, out);
         if (rc != GV_OK) {
             out.status = rc;
             out.error = gv_last_error();

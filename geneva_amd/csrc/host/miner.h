@@ -50,6 +50,7 @@ struct MineResult {
     uint64_t launches = 0;       // extension/count launches
     uint64_t chunks = 0;         // extension chunks (capacity splits)
     uint64_t peakRows = 0;       // largest materialised table (rows)
+    uint64_t overflowRetries = 0;   // whole-query runs redone chunked
     std::vector<uint32_t> graphBits;
 };
 
@@ -88,6 +89,7 @@ private:
     int ensureRows(uint32_t width);
     int run(const gv_query_t& q, uint32_t width, const uint32_t* rows, uint64_t n,
             MineResult& out);
+    int runLevels(const CompiledQuery& cq, const uint32_t* rows, uint64_t n, MineResult& out);
 
     gv_ctx_t ctx_;
     gv_graph_t g_;

@@ -173,6 +173,25 @@ int gv_count_rows_mode(const gv_step_t* step, const gv_step_t* tail, uint32_t ro
 int gv_count_rows(gv_ctx_t ctx, gv_graph_t g, const gv_step_t* step, const gv_step_t* tail,
                   const uint32_t* rows_dev, uint32_t row_len, uint64_t nrows);
 
+/* A whole query on the plain path with every extent kept in device memory:
+ * the step kernels read their row counts from the previous step's output
+ * counter, so nothing is read back until the end (one synchronisation).
+ * Start rows: start_rows[0..nstart).  For each materialised width w (2 ..
+ * the counted step's input width), tables[w] holds up to table_rows[w] rows;
+ * frontiers[w] (capacity >= that width's rows; frontiers[1] >= nstart) holds
+ * the sizing arrays of step w.  count_by_rows: use gv_count_rows for a last
+ * step it applies to.  Count, bytes and graph bits accumulate into the
+ * context's accumulators (gv_counters_read; count / alg_bytes / bits, when
+ * non-null, receive them too).  GV_ERR_CAPACITY: some table outgrew its
+ * capacity; nothing was counted past that point and the accumulators must be
+ * reset before the query is mined again with the chunked step entries.
+ * Replaces the phase loop of src/gpumine.cpp:993-1061 with its per-launch
+ * read-backs. */
+int gv_mine_levels(gv_ctx_t ctx, gv_graph_t g, const gv_query_t* q, const uint32_t* start_rows,
+                   uint64_t nstart, uint32_t* const* tables, const uint64_t* table_rows,
+                   const gv_frontier_t* frontiers, int count_by_rows, uint64_t* count,
+                   uint64_t* alg_bytes, uint32_t* bits, uint32_t nwords);
+
 /* Accumulators of a context: embedding count, algorithmic bytes and matched
  * graph bitset (ngraphs bits).  reset zeroes them; read synchronises. */
 int gv_counters_reset(gv_ctx_t ctx, gv_graph_t g);

@@ -2,6 +2,7 @@
 
     python tests/golden/make_golden.py            # plans, workload 1, Enron, scaled workloads
     python tests/golden/make_golden.py --full2    # + full-size workload 2 counts (long)
+    python tests/golden/make_golden.py --only-w4  # workload 4: scaled counts, full-size shards
 
 Reference values come from the reference's own cpumine, compiled from its
 sources into oracle/_ref/cpumine (`make -C oracle ref`), run on files written
@@ -17,6 +18,8 @@ import shutil
 import subprocess
 import sys
 import tempfile
+
+import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
@@ -113,9 +116,54 @@ def full_workload2():
               open(os.path.join(HERE, "workload2_full.json"), "w"), indent=1)
 
 
+def workload4_shards(nq=32, budget_s=0.5):
+    """Full-size workload 4: every query's count is far beyond any CPU run, so
+    the oracle mines shards of 8 consecutive start vertices (indices into the
+    id-sorted start list).  Per query, shards start at the middle of the list
+    and move towards the heavy low ids (Chung-Lu weights fall with the id),
+    halving the start index until one shard takes more than budget_s."""
+    ds = configs.workload4()
+    ogw = O.OracleGraph.of(ds)
+    vl = np.asarray(ds.vlabel)
+    cases = []
+    for qi, qt in enumerate(ds.queries[:nq]):
+        v0 = int(re.search(r"svid:(\d+)", G.plan_dump(qt)).group(1))
+        l0 = [int(l.split()[2]) for l in qt.splitlines()
+              if l.startswith("v ") and int(l.split()[1]) == v0][0]
+        nstart = int((vl[1:] == l0).sum())
+        p = nstart // 2
+        shards = []
+        while p >= 8:
+            r = ogw.mine(qt, threads=os.cpu_count(), start_begin=p, start_end=p + 8)
+            shards.append({"begin": p, "end": p + 8, "count": r.count, "alg_bytes": r.alg_bytes,
+                           "oracle_seconds": round(r.seconds, 3)})
+            if r.seconds > budget_s:
+                break
+            p //= 2
+        cases.append({"query": qt, "nstart": nstart, "shards": shards})
+        print("workload4 query", qi, [(s["begin"], s["count"]) for s in shards], flush=True)
+    json.dump({"workload": 4, "scale": 1.0, "threads": os.cpu_count(), "cases": cases},
+              open(os.path.join(HERE, "workload4_shards.json"), "w"), indent=1)
+
+
+def workload4_scaled(scale=0.01, nq=24):
+    ds = configs.workload4(scale=scale, nqueries=nq)
+    ogw = O.OracleGraph.of(ds)
+    cases = []
+    for qt in ds.queries:
+        r = ogw.mine(qt, threads=os.cpu_count())
+        cases.append({"query": qt, "count": r.count, "alg_bytes": r.alg_bytes})
+        print("workload4 scaled:", r.count, round(r.seconds, 2), flush=True)
+    json.dump({"workload": 4, "scale": scale, "kw": {"nqueries": nq}, "cases": cases},
+              open(os.path.join(HERE, "workload4_scaled.json"), "w"), indent=1)
+
+
 def main():
     if "--only-full2" in sys.argv:
         return full_workload2()
+    if "--only-w4" in sys.argv:
+        workload4_scaled()
+        return workload4_shards()
     full2 = "--full2" in sys.argv
     if not os.path.exists(O.REF_CPUMINE):
         sys.exit("oracle/_ref/cpumine missing: make -C oracle ref")
