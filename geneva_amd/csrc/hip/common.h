@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <map>
@@ -589,6 +590,7 @@ struct gv_graph {
     uint32_t nlabels = 0;
     uint32_t nparts = 0;
     uint64_t nnbr = 0;
+    uint32_t max_deg = 0;
     uint64_t dev_bytes = 0;
     std::vector<uint32_t> label_values;       // dense -> original
     std::vector<uint32_t> part_edge_label;    // partition -> edge label
@@ -682,6 +684,35 @@ bool valid_step(const gv_step_t* s, uint32_t row_len, const gv_graph* g) {
     for (int i = 0; i < s->ncoll; ++i)
         if (s->coll_col[i] >= row_len) return false;
     if (s->restrict_col != GV_NONE && s->restrict_col >= row_len) return false;
+    return true;
+}
+
+// leaf groups over a core of d->width columns; every group's C(s, m) must be
+// exact in 64 bits: max_deg^m < 2^64
+bool valid_leaves(const gv_leaves_t* d, const gv_graph* g) {
+    if (!d || d->width < 1 || d->k > GV_MAXK || d->width >= d->k) return false;
+    if (d->ngroups < 1 || d->ngroups > GV_MAX_LEAF_GROUPS) return false;
+    uint32_t total = 0;
+    for (uint32_t i = 0; i < d->ngroups; ++i) {
+        const gv_leaf_group_t& gr = d->group[i];
+        if (gr.anchor_col >= d->width || gr.part >= std::max<uint32_t>(g->nparts, 1) ||
+            gr.label >= g->nlabels || gr.m < 1 || gr.ncoll >= GV_MAXK)
+            return false;
+        if (gr.restrict_col != GV_NONE && gr.restrict_col >= d->width) return false;
+        for (uint32_t j = 0; j < gr.ncoll; ++j)
+            if (gr.coll_col[j] >= d->width) return false;
+        double bits = gr.m * std::log2((double)g->max_deg + 1.0);
+        if (bits >= 63.0) return false;
+        total += gr.m;
+    }
+    if (total != d->k - d->width) return false;
+    uint32_t seen[GV_MAX_LEAF_GROUPS] = {};
+    for (uint32_t j = 0; j < total; ++j) {
+        if (d->group_of[j] >= d->ngroups) return false;
+        seen[d->group_of[j]]++;
+    }
+    for (uint32_t i = 0; i < d->ngroups; ++i)
+        if (seen[i] != d->group[i].m) return false;
     return true;
 }
 

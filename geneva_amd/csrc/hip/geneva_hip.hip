@@ -146,6 +146,114 @@ __global__ __launch_bounds__(kBlock) void k_count_rows(RowsArgs a) {
 }
 
 // ---------------------------------------------------------------------------
+// count by leaf groups: one thread per core row; the row's embeddings are the
+// product over groups of C(s_g, m_g) (gv_leaves_t), and its algorithmic bytes
+// follow the rows the enumeration would have held at every width: with j_g
+// leaves of group g placed, there are prod_g C(s_g, j_g) of them.
+struct LeavesArgs {
+    DevGraph g;
+    const uint32_t* rows;
+    uint64_t nrows;
+    const unsigned long long* nrows_dev;   // as in SizeArgs
+    const unsigned long long* stop;
+    gv_leaves_t d;
+    unsigned long long* count;
+    unsigned long long* bytes;
+    unsigned long long* req;
+    unsigned long long* units;
+    uint32_t* gbits;
+};
+
+// C(s, m) for s^m < 2^64 (checked by the caller against the largest degree)
+__device__ __forceinline__ uint64_t binom_small(uint64_t s, uint32_t m) {
+    if (m > s) return 0;
+    uint64_t num = 1, den = 1;
+    for (uint32_t i = 0; i < m; ++i) {
+        num *= s - i;
+        den *= i + 1;
+    }
+    return num / den;
+}
+
+__global__ __launch_bounds__(kBlock) void k_count_leaves(LeavesArgs a) {
+    constexpr int G = GV_MAX_LEAF_GROUPS;
+    uint64_t cnt = 0, by = 0, rq = 0;
+    uint32_t last_gid = 0xffffffffu;
+    const uint32_t L0 = a.d.width, ng = a.d.ngroups, nleaf = a.d.k - a.d.width;
+    const uint64_t nrows = dev_rows(a.nrows, a.nrows_dev, a.stop);
+    if (a.units && blockIdx.x == 0 && threadIdx.x == 0 && nrows) atomicAdd(a.units, (unsigned long long)nrows);
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r < nrows; r += stride) {
+        const uint32_t* row = a.rows + r * L0;
+        uint32_t sv[G], len[G];
+        rq += 4ull * L0;
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            sv[g] = 0;
+            len[g] = 0;
+            if (g >= (int)ng) continue;
+            const gv_leaf_group_t& gr = a.d.group[g];
+            uint32_t b, e;
+            seg(a.g, gr.part, row[gr.anchor_col], gr.label, b, e);
+            const uint32_t bound = gr.restrict_col != GV_NONE ? row[gr.restrict_col] : 0xffffffffu;
+            const uint32_t cut = bound == 0xffffffffu ? e : lower_bound_u32(a.g.nbr, b, e, bound);
+            rq += 8ull + (bound != 0xffffffffu ? 4ull * probes(e - b) : 0ull);
+            uint32_t s = cut - b;
+            for (uint32_t j = 0; j < gr.ncoll; ++j) {
+                const uint32_t x = row[gr.coll_col[j]];
+                if (x >= bound) continue;
+                rq += 4ull * probes(cut - b);
+                if (contains_u32(a.g.nbr, b, cut, x)) --s;
+            }
+            sv[g] = s;
+            len[g] = e - b;
+        }
+        // rows of every width, leaf by leaf in column order
+        uint32_t jg[G];
+#pragma unroll
+        for (int g = 0; g < G; ++g) jg[g] = 0;
+        uint64_t rows_cur = 1;
+        for (uint32_t j = 0; j < nleaf; ++j) {
+            const uint32_t L = L0 + j;
+            const uint32_t gj = a.d.group_of[j];
+            uint32_t lj = 0;
+#pragma unroll
+            for (int g = 0; g < G; ++g)
+                if (g == (int)gj) {
+                    lj = len[g];
+                    jg[g] += 1;
+                }
+            by += rows_cur * (4ull * L + 8ull + 4ull * lj);
+            uint64_t next = 1;
+#pragma unroll
+            for (int g = 0; g < G; ++g)
+                if (g < (int)ng) next *= binom_small(sv[g], jg[g]);
+            if (L + 1 < a.d.k) by += 4ull * (L + 1) * next;
+            rows_cur = next;
+        }
+        const uint64_t n = rows_cur;   // all leaves placed
+        cnt += n;
+        if (n && a.gbits) {
+            const uint32_t gid = a.g.gof[row[0]];
+            if (gid != last_gid) {
+                atomicOr(&a.gbits[gid >> 5], 1u << (gid & 31));
+                last_gid = gid;
+            }
+        }
+    }
+    cnt = wave_sum_u64(cnt);
+    by = wave_sum_u64(by);
+    if (lane_id() == 0) {
+        if (cnt) atomicAdd(a.count, (unsigned long long)cnt);
+        if (by) atomicAdd(a.bytes, (unsigned long long)by);
+    }
+    if (a.req) {
+        rq = wave_sum_u64(rq);
+        if (lane_id() == 0 && rq) atomicAdd(a.req, (unsigned long long)rq);
+    }
+}
+
+// ---------------------------------------------------------------------------
 // grid-level inclusive scan of uint32 counts into uint64 sums (reduce, scan of
 // tile sums, scan + add)
 
@@ -602,6 +710,13 @@ int gv_graph_create(gv_ctx_t c, const uint32_t* vLabel, uint32_t nblocks,
             }
         }
     }
+    // largest neighbour list (any partition): bounds every label segment
+    uint32_t max_deg = 0;
+    for (uint32_t p = 0; p < std::max<uint32_t>(nblocks, 1); ++p) {
+        const uint32_t* lo = loff.data() + (uint64_t)p * nv1 * nl1;
+        for (uint64_t v = 1; v < nv1; ++v)
+            max_deg = std::max(max_deg, lo[v * nl1 + NL] - lo[v * nl1]);
+    }
     // start tables per label
     std::vector<uint64_t> lvoff(nl1, 0);
     for (uint64_t v = 1; v < nv1; ++v)
@@ -619,6 +734,7 @@ int gv_graph_create(gv_ctx_t c, const uint32_t* vLabel, uint32_t nblocks,
     g->nlabels = NL;
     g->nparts = nblocks;
     g->nnbr = nbr.size();
+    g->max_deg = max_deg;
     g->label_values = vals;
     g->part_edge_label.assign(block_edge_labels, block_edge_labels + nblocks);
     g->lvoff = lvoff;
@@ -699,6 +815,12 @@ int gv_graph_info(gv_graph_t g, uint64_t* nv1, uint32_t* nl, uint32_t* np, uint6
     if (np) *np = g->nparts;
     if (nnbr) *nnbr = g->nnbr;
     if (bytes) *bytes = g->dev_bytes;
+    return GV_OK;
+}
+
+int gv_graph_max_degree(gv_graph_t g, uint32_t* d) {
+    if (!g || !d) return GV_ERR_ARG;
+    *d = g->max_deg;
     return GV_OK;
 }
 
@@ -865,6 +987,37 @@ int gv_count_rows_mode(const gv_step_t* st, const gv_step_t* tail, uint32_t L) {
     return count_rows_mode(st, tail, L);
 }
 
+static void leaves_args(LeavesArgs& a, gv_ctx* c, gv_graph* g, const gv_leaves_t* d,
+                        const uint32_t* rows, uint64_t nrows) {
+    memset(&a, 0, sizeof(a));
+    a.g = dev_view(g);
+    a.rows = rows;
+    a.nrows = nrows;
+    a.d = *d;
+    a.count = c->acc + 0;
+    a.bytes = c->acc + 1;
+    a.req = req_ptr(c, kKindRows);
+    a.units = units_ptr(c, kKindRows);
+    a.gbits = (g->d_gof && c->gbits && c->gbits_words * 32ull >= g->ngraphs) ? c->gbits : nullptr;
+}
+
+int gv_count_leaves(gv_ctx_t c, gv_graph_t g, const gv_leaves_t* d, const uint32_t* rows,
+                    uint64_t nrows) {
+    if (!c || !g || !valid_leaves(d, g)) return GV_ERR_ARG;
+    if (nrows == 0) return GV_OK;
+    if (!rows) return GV_ERR_ARG;
+    HIPCHK(hipSetDevice(c->device));
+    LeavesArgs a;
+    leaves_args(a, c, g, d, rows, nrows);
+    {
+        ProfScope ps(c, kKindRows);
+        hipLaunchKernelGGL(k_count_leaves, dim3(grid_for(nrows, kBlock, c->num_cu * 16)), dim3(kBlock),
+                           0, c->stream, a);
+    }
+    HIPCHK(hipGetLastError());
+    return GV_OK;
+}
+
 int gv_count_rows(gv_ctx_t c, gv_graph_t g, const gv_step_t* st, const gv_step_t* tail,
                   const uint32_t* rows, uint32_t L, uint64_t nrows) {
     if (!c || !g || !valid_step(st, L, g)) return GV_ERR_ARG;
@@ -899,10 +1052,11 @@ int gv_count_rows(gv_ctx_t c, gv_graph_t g, const gv_step_t* st, const gv_step_t
 
 int gv_mine_levels(gv_ctx_t c, gv_graph_t g, const gv_query_t* q, const uint32_t* start_rows,
                    uint64_t nstart, uint32_t* const* tables, const uint64_t* table_rows,
-                   const gv_frontier_t* frontiers, int count_by_rows, uint64_t* count,
-                   uint64_t* alg_bytes, uint32_t* bits, uint32_t nwords) {
+                   const gv_frontier_t* frontiers, int count_by_rows, const gv_leaves_t* leaves,
+                   uint64_t* count, uint64_t* alg_bytes, uint32_t* bits, uint32_t nwords) {
     if (!c || !g || !q || !tables || !table_rows || !frontiers || q->k < 2 || q->k > GV_MAXK)
         return GV_ERR_ARG;
+    if (leaves && (!valid_leaves(leaves, g) || leaves->k != q->k)) return GV_ERR_ARG;
     if (nstart && !start_rows) return GV_ERR_ARG;
     if (bits && nwords > c->gbits_words) return GV_ERR_ARG;
     const uint32_t k = q->k;
@@ -917,6 +1071,7 @@ int gv_mine_levels(gv_ctx_t c, gv_graph_t g, const gv_query_t* q, const uint32_t
         const uint64_t cap = w == 1 ? nstart : table_rows[w];
         if (w >= 2 && (!tables[w] || cap == 0)) return GV_ERR_ARG;
         wlast = w;
+        if (leaves && w == leaves->width) break;
         const bool by_rows = (last || fused) && count_by_rows &&
                              count_rows_mode(&st, fused ? &q->step[w + 1] : nullptr, w) >= 0;
         if (!by_rows && (!frontiers[w] || frontiers[w]->cap < cap)) return GV_ERR_ARG;
@@ -936,6 +1091,17 @@ int gv_mine_levels(gv_ctx_t c, gv_graph_t g, const gv_query_t* q, const uint32_t
         const bool last = w + 1 == k;
         const bool fused = w + 2 == k && q->step[w + 1].nback == 1;
         const gv_step_t* tail = fused ? &q->step[w + 1] : nullptr;
+        if (leaves && w == leaves->width) {
+            LeavesArgs a;
+            leaves_args(a, c, g, leaves, rows, cap);
+            a.nrows_dev = ndev;
+            a.stop = stop;
+            ProfScope ps(c, kKindRows);
+            hipLaunchKernelGGL(k_count_leaves, dim3(grid_for(cap, kBlock, c->num_cu * 16)), dim3(kBlock),
+                               0, c->stream, a);
+            HIPCHK(hipGetLastError());
+            break;
+        }
         if ((last || fused) && count_by_rows && count_rows_mode(&st, tail, w) >= 0) {
             RowsArgs a;
             memset(&a, 0, sizeof(a));

@@ -1,6 +1,9 @@
 // Plain mining path over the C ABI.
 #include "miner.h"
 
+#include <cmath>
+#include <cstring>
+
 #include <algorithm>
 #include <sstream>
 
@@ -257,6 +260,63 @@ int PlainMiner::mine(const CompiledQuery& cq, MineResult& out) {
     return mine(cq, Shard(), out);
 }
 
+// Leaf groups over the columns width .. k-1 (gv_leaves_t): every vertex there
+// hangs by one edge from a core column; vertices sharing anchor, partition and
+// label form a group whose first member may be restricted to a core column
+// and whose later members are each restricted to the member before (the
+// planner's chain); groups carry pairwise different labels.  C(s, m) must be
+// exact in 64 bits for s up to the largest degree.
+static bool leafGroups(const gv_query_t& q, uint32_t width, uint32_t maxDeg, gv_leaves_t& d) {
+    memset(&d, 0, sizeof(d));
+    d.width = width;
+    d.k = q.k;
+    std::vector<std::vector<uint32_t>> members(GV_MAX_LEAF_GROUPS);
+    for (uint32_t t = width; t < q.k; ++t) {
+        const gv_step_t& s = q.step[t];
+        if (s.nback != 1 || s.back_col[0] >= width) return false;
+        int gi = -1;
+        for (uint32_t i = 0; i < d.ngroups; ++i) {
+            if (d.group[i].label != s.label) continue;
+            if (d.group[i].anchor_col != s.back_col[0] || d.group[i].part != s.back_part[0]) return false;
+            gi = (int)i;
+        }
+        std::vector<uint32_t> core, leaf;
+        for (uint32_t j = 0; j < s.ncoll; ++j) (s.coll_col[j] < width ? core : leaf).push_back(s.coll_col[j]);
+        if (gi < 0) {
+            if (d.ngroups == GV_MAX_LEAF_GROUPS || !leaf.empty()) return false;
+            if (s.restrict_col != GV_NONE && s.restrict_col >= width) return false;
+            gv_leaf_group_t& gr = d.group[d.ngroups];
+            gr.anchor_col = s.back_col[0];
+            gr.part = s.back_part[0];
+            gr.label = s.label;
+            gr.m = 1;
+            gr.restrict_col = s.restrict_col;
+            gr.ncoll = (uint8_t)core.size();
+            for (size_t j = 0; j < core.size(); ++j) gr.coll_col[j] = (uint8_t)core[j];
+            gi = (int)d.ngroups++;
+        } else {
+            gv_leaf_group_t& gr = d.group[gi];
+            if (core.size() != gr.ncoll) return false;
+            for (size_t j = 0; j < core.size(); ++j)
+                if (core[j] != gr.coll_col[j]) return false;
+            if (leaf != members[gi] || s.restrict_col != members[gi].back()) return false;
+            gr.m++;
+        }
+        members[gi].push_back(t);
+        d.group_of[t - width] = (uint8_t)gi;
+    }
+    for (uint32_t i = 0; i < d.ngroups; ++i)
+        if (d.group[i].m * std::log2((double)maxDeg + 1.0) >= 63.0) return false;
+    return d.ngroups > 0;
+}
+
+// The narrowest core whose remaining vertices are leaf groups.
+bool leafPlan(const gv_query_t& q, uint32_t maxDeg, gv_leaves_t& d) {
+    for (uint32_t w = 1; w < q.k; ++w)
+        if (leafGroups(q, w, maxDeg, d)) return true;
+    return false;
+}
+
 // The whole query with extents kept in device memory (gv_mine_levels): one
 // synchronisation at the end.  Tables that outgrow a level are detected on the
 // device; the query is then mined again, chunked, by run().
@@ -270,10 +330,16 @@ int PlainMiner::runLevels(const CompiledQuery& cq, const uint32_t* rows, uint64_
     int rc = ensureFrontier(1, n);
     if (rc != GV_OK) return rc;
     frs[1] = lv_[1].fr;
+    uint32_t maxDeg = 0;
+    gv_graph_max_degree(g_, &maxDeg);
+    gv_leaves_t leaves;
+    const bool useLeaves = countByRows_ && leafPlan(q, maxDeg, leaves);
     for (uint32_t w = 2; w < q.k; ++w) {
         // the last vertex, when it has one back edge, is counted with the step
-        // before it, so width k - 1 is then never materialised
+        // before it, so width k - 1 is then never materialised; leaf groups
+        // are counted on their core
         if (w + 1 == q.k && q.step[w].nback == 1) break;
+        if (useLeaves && w > leaves.width) break;
         rc = ensureRows(w);
         if (rc != GV_OK) return rc;
         tables[w] = lv_[w].rows;
@@ -283,8 +349,8 @@ int PlainMiner::runLevels(const CompiledQuery& cq, const uint32_t* rows, uint64_
     uint32_t words = 0;
     gv_graph_owners(g_, &words);
     words = (words + 31) / 32;
-    rc = gv_mine_levels(ctx_, g_, &q, rows, n, tables, caps, frs, countByRows_ ? 1 : 0, nullptr,
-                        nullptr, nullptr, 0);
+    rc = gv_mine_levels(ctx_, g_, &q, rows, n, tables, caps, frs, countByRows_ ? 1 : 0,
+                        useLeaves ? &leaves : nullptr, nullptr, nullptr, nullptr, 0);
     out.launches++;
     if (rc != GV_ERR_CAPACITY) return rc;
     // a table outgrew its level: start over, chunked

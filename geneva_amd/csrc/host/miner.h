@@ -54,6 +54,10 @@ struct MineResult {
     std::vector<uint32_t> graphBits;
 };
 
+// The narrowest core (gv_leaves_t::width) whose remaining vertices form leaf
+// groups that gv_count_leaves counts per core row; false if there is none.
+bool leafPlan(const gv_query_t& q, uint32_t maxDeg, gv_leaves_t& d);
+
 class PlainMiner {
 public:
     // levelBytes: device memory per table level (rows + sizing arrays).

@@ -59,6 +59,33 @@ typedef struct {
     uint8_t coll_col[GV_MAXK];
 } gv_step_t;
 
+/* Leaves of a query: the vertices in columns `width` .. k-1 each hang by a
+ * single edge from a core column (< width).  Leaves with the same anchor
+ * column, partition and label form a group; groups have pairwise different
+ * labels, the first leaf of a group may be restricted to a core column and
+ * every later one is restricted to the group's previous leaf (the chain the
+ * planner's automorphism breaking gives).  A core row then has
+ * prod_g C(s_g, m_g) embeddings, s_g = the group's segment below the bound
+ * minus the core values of its label found there. */
+typedef struct {
+    uint8_t anchor_col;          /* core column the group's leaves hang from */
+    uint8_t part;                /* edge-label partition of their edges */
+    uint8_t m;                   /* leaves in the group */
+    uint8_t restrict_col;        /* first leaf < row[restrict_col]; GV_NONE = none */
+    uint32_t label;              /* dense vertex label of the leaves */
+    uint8_t ncoll;               /* core columns holding the same label */
+    uint8_t coll_col[GV_MAXK];
+} gv_leaf_group_t;
+
+#define GV_MAX_LEAF_GROUPS 8
+typedef struct {
+    uint32_t width;              /* core width */
+    uint32_t k;                  /* query vertices */
+    uint32_t ngroups;
+    uint8_t group_of[GV_MAXK];   /* group of the leaf in column width + j */
+    gv_leaf_group_t group[GV_MAX_LEAF_GROUPS];
+} gv_leaves_t;
+
 /* A compiled query: k vertices in match order; step[t] adds column t. */
 typedef struct {
     uint32_t k;
@@ -110,6 +137,8 @@ int gv_graph_part_index(gv_graph_t g, uint32_t edge_label, uint32_t* part);
  * device bytes held. */
 int gv_graph_info(gv_graph_t g, uint64_t* nv1, uint32_t* nlabels, uint32_t* nparts,
                   uint64_t* nnbr, uint64_t* dev_bytes);
+/* Largest neighbour list of any vertex (bounds every label segment). */
+int gv_graph_max_degree(gv_graph_t g, uint32_t* max_degree);
 /* Label statistics of partition `part`: vertices per dense label
  * (nlabels entries) and directed edges per (source label, neighbour label)
  * (nlabels*nlabels entries, row-major).  Either pointer may be null. */
@@ -174,6 +203,7 @@ int gv_count_rows(gv_ctx_t ctx, gv_graph_t g, const gv_step_t* step, const gv_st
                   const uint32_t* rows_dev, uint32_t row_len, uint64_t nrows);
 
 /* A whole query on the plain path with every extent kept in device memory:
+ * (leaves, when non-null, ends it at width leaves->width with gv_count_leaves)
  * the step kernels read their row counts from the previous step's output
  * counter, so nothing is read back until the end (one synchronisation).
  * Start rows: start_rows[0..nstart).  For each materialised width w (2 ..
@@ -189,8 +219,19 @@ int gv_count_rows(gv_ctx_t ctx, gv_graph_t g, const gv_step_t* step, const gv_st
  * read-backs. */
 int gv_mine_levels(gv_ctx_t ctx, gv_graph_t g, const gv_query_t* q, const uint32_t* start_rows,
                    uint64_t nstart, uint32_t* const* tables, const uint64_t* table_rows,
-                   const gv_frontier_t* frontiers, int count_by_rows, uint64_t* count,
-                   uint64_t* alg_bytes, uint32_t* bits, uint32_t nwords);
+                   const gv_frontier_t* frontiers, int count_by_rows, const gv_leaves_t* leaves,
+                   uint64_t* count, uint64_t* alg_bytes, uint32_t* bits, uint32_t nwords);
+
+/* Count-only final phases for a query whose last vertices are leaf groups
+ * (gv_leaves_t): one thread per core row (rows of d->width columns), no
+ * candidate walk and nothing materialised past the core.  Counts, algorithmic
+ * bytes (those of every row the enumeration would hold) and graph bits
+ * accumulate like gv_count_emb's.  GV_ERR_ARG when the descriptor is not
+ * valid for this graph (including max_degree^m >= 2^63 for some group).
+ * Replaces the isLastPhase extension phases of src/gpumine.cpp:993-1061 for
+ * such queries.  Asynchronous. */
+int gv_count_leaves(gv_ctx_t ctx, gv_graph_t g, const gv_leaves_t* d, const uint32_t* rows_dev,
+                    uint64_t nrows);
 
 /* Accumulators of a context: embedding count, algorithmic bytes and matched
  * graph bitset (ngraphs bits).  reset zeroes them; read synchronises. */

@@ -131,15 +131,21 @@ def test_bad_query_raises(w1):
 
 
 ROW_COUNT_QUERIES = [
-    # (query, mode of the counted step): one edge (last step), a path whose
-    # last vertex hangs off an earlier column, and stars whose last two leaves
-    # share label and centre (pairs below v in one segment)
+    # (query, kind): 0-2 the row-count modes of the counted step (one edge; a
+    # path whose last vertex hangs off an earlier column; stars whose last two
+    # leaves share label and centre), 3 leaf groups counted on their core
     ("v 0 1\nv 1 2\ne 0 1 1\n", 0),
     ("v 0 1\nv 1 1\ne 0 1 1\n", 0),
     ("v 0 1\nv 1 2\nv 2 3\nv 3 1\ne 0 1 1\ne 1 2 1\ne 2 3 1\n", 1),
     ("v 0 1\nv 1 2\nv 2 2\nv 3 2\ne 0 1 1\ne 0 2 1\ne 0 3 1\n", 2),
     ("v 0 1\nv 1 1\nv 2 1\nv 3 1\nv 4 1\ne 0 1 1\ne 0 2 1\ne 0 3 1\ne 0 4 1\n", 2),
     ("v 0 1\nv 1 2\nv 2 1\nv 3 1\ne 0 1 1\ne 1 2 1\ne 1 3 1\n", 2),
+    # leaf groups on a core: two same-label pairs on one centre, a double
+    # star, three same-label leaves, a leaf whose label is the centre's
+    ("v 0 1\nv 1 2\nv 2 2\nv 3 3\nv 4 3\ne 0 1 1\ne 0 2 1\ne 0 3 1\ne 0 4 1\n", 3),
+    ("v 0 1\nv 1 2\nv 2 3\nv 3 3\nv 4 1\nv 5 2\ne 0 1 1\ne 0 2 1\ne 0 3 1\ne 1 4 1\ne 1 5 1\n", 3),
+    ("v 0 2\nv 1 1\nv 2 1\nv 3 1\ne 0 1 1\ne 0 2 1\ne 0 3 1\n", 3),
+    ("v 0 1\nv 1 1\nv 2 2\nv 3 1\ne 0 1 1\ne 0 2 1\ne 0 3 1\n", 3),
 ]
 
 

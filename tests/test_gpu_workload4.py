@@ -52,6 +52,8 @@ def test_workload4_shards_partition_the_count(require_gpu):
 
 
 def test_workload4_full_size_shards(require_gpu):
+    if not os.path.exists(os.path.join(GOLDEN, "workload4_shards.json")):
+        pytest.skip("workload4_shards.json not generated (make_golden.py --only-w4)")
     gold = _load("workload4_shards.json")
     ds = configs.workload4()
     eng = G.Engine(ds)
