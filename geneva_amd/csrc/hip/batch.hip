@@ -25,6 +25,8 @@ struct BDesc {               // written by the host at admission
     gv_query_t q;
     uint32_t start_off;      // offset of label0's start rows in lverts
     uint32_t start_n;
+    uint32_t has_leaves;     // leaves: counted per row at width leaves.width
+    gv_leaves_t leaves;
 };
 
 struct BState {              // dynamic, polled by the host (mirrors gv_slot_info_t)
@@ -36,7 +38,8 @@ struct BState {              // dynamic, polled by the host (mirrors gv_slot_inf
     unsigned long long bytes;
     // internal
     uint32_t sel;            // half holding the input table
-    uint32_t mode;           // this iteration: 0 idle, 1 extend, 2 count, 3 count + tail
+    uint32_t mode;           // this iteration: 0 idle, 1 extend, 2 count, 3 count + tail,
+                             // 4 leaf groups (counted while sizing)
     int32_t rmode;           // count modes: count_rows_mode of the step, or -1 (walk)
     unsigned long long out_rows;
     unsigned long long overflow;
@@ -131,7 +134,8 @@ __global__ __launch_bounds__(1024) void k_b_prep(BArgs a) {
                 const bool last = S.width + 1 == D.q.k;
                 const bool tail = S.width + 2 == D.q.k && D.q.step[S.width + 1].nback == 1;
                 mode = last ? 2 : (tail ? 3 : 1);
-                S.rmode = mode == 1 ? -1
+                if (D.has_leaves && S.width == D.leaves.width) mode = 4;
+                S.rmode = mode == 1 || mode == 4 ? -1
                                     : count_rows_mode(&D.q.step[S.width],
                                                       mode == 3 ? &D.q.step[S.width + 1] : nullptr,
                                                       S.width);
@@ -202,6 +206,25 @@ __global__ __launch_bounds__(kBlock) void k_b_size(BArgs a) {
         const uint64_t r = S.row_end - rows + local;
         const gv_step_t& st = D.q.step[L];
         const uint32_t* row = slot_rows(a, s, S, D) + local * L;
+        if (S.mode == 4) {
+            // leaf groups on this core row: counted here, no candidates (the
+            // row's own read is part of the leaf bytes)
+            uint64_t by = 0;
+            const uint64_t n = leaf_row_count(a.g, D.leaves, row, by, rq);
+            a.cnt[r] = 0;
+            a.anc_beg[r] = 0;
+            a.anc_i[r] = 0;
+            cacc += n;
+            acc += by;
+            if (n && a.g.gof) {
+                const uint32_t gid = a.g.gof[row[0]];
+                if (gid != last_gid) {
+                    atomicOr(&a.gbits[(uint64_t)s * a.gwords + (gid >> 5)], 1u << (gid & 31));
+                    last_gid = gid;
+                }
+            }
+            continue;
+        }
         uint32_t best_b = 0, best_len = 0xffffffffu, best_i = 0;
         uint64_t by = 4ull * L;
         for (uint32_t i = 0; i < st.nback; ++i) {
@@ -417,7 +440,7 @@ __global__ __launch_bounds__(1024) void k_b_post(BArgs a) {
     if (s >= a.nslots) return;
     BState& S = a.st[s];
     if (S.mode == 0) return;
-    if (S.mode >= 2) {
+    if (S.mode >= 2) {   // counted (count, count + tail, leaf groups)
         S.state = GV_SLOT_DONE;
         S.width = a.desc[s].q.k;
     } else if (S.overflow) {
@@ -657,8 +680,14 @@ int gv_batch_set_tune(gv_batch_t b, const gv_tune_t* t) {
 }
 
 int gv_batch_admit(gv_batch_t b, const gv_query_t* q, uint64_t tag, uint32_t* slot) {
+    return gv_batch_admit_leaves(b, q, nullptr, tag, slot);
+}
+
+int gv_batch_admit_leaves(gv_batch_t b, const gv_query_t* q, const gv_leaves_t* leaves, uint64_t tag,
+                          uint32_t* slot) {
     if (!b || !q || !slot || q->k < 2 || q->k > GV_MAXK || q->label0 >= b->g->nlabels)
         return GV_ERR_ARG;
+    if (leaves && (!valid_leaves(leaves, b->g) || leaves->k != q->k)) return GV_ERR_ARG;
     for (uint32_t t = 1; t < q->k; ++t) {
         const gv_step_t& s = q->step[t];
         if (s.nback < 1 || s.nback >= GV_MAXK || s.label >= b->g->nlabels || s.ncoll >= GV_MAXK)
@@ -683,6 +712,10 @@ int gv_batch_admit(gv_batch_t b, const gv_query_t* q, uint64_t tag, uint32_t* sl
     d.q = *q;
     d.start_off = (uint32_t)b->g->lvoff[q->label0];
     d.start_n = (uint32_t)(b->g->lvoff[q->label0 + 1] - b->g->lvoff[q->label0]);
+    if (leaves) {
+        d.has_leaves = 1;
+        d.leaves = *leaves;
+    }
     BState& st = b->h_st[s];
     memset(&st, 0, sizeof(st));
     st.tag = tag;

@@ -304,6 +304,73 @@ __device__ __forceinline__ uint64_t row_count(const DevGraph& g, const gv_step_t
     return s;
 }
 
+// C(s, m) for s^m < 2^64 (leaf groups are checked against the largest degree)
+__device__ __forceinline__ uint64_t binom_small(uint64_t s, uint32_t m) {
+    if (m > s) return 0;
+    uint64_t num = 1, den = 1;
+    for (uint32_t i = 0; i < m; ++i) {
+        num *= s - i;
+        den *= i + 1;
+    }
+    return num / den;
+}
+
+// One core row of a leaf-group query (gv_leaves_t): the row's embeddings,
+// prod_g C(s_g, m_g); adds to `by` the algorithmic bytes of every row the
+// enumeration would hold from the core onwards (with j_g leaves of group g
+// placed there are prod_g C(s_g, j_g) rows), to `rq` the requested bytes.
+__device__ __forceinline__ uint64_t leaf_row_count(const DevGraph& g, const gv_leaves_t& d,
+                                                   const uint32_t* row, uint64_t& by,
+                                                   uint64_t& rq) {
+    constexpr int G = GV_MAX_LEAF_GROUPS;
+    const uint32_t L0 = d.width, ng = d.ngroups, nleaf = d.k - d.width;
+    uint32_t sv[G], len[G];
+#pragma unroll
+    for (int q = 0; q < G; ++q) {
+        sv[q] = 0;
+        len[q] = 0;
+        if (q >= (int)ng) continue;
+        const gv_leaf_group_t& gr = d.group[q];
+        uint32_t b, e;
+        seg(g, gr.part, row[gr.anchor_col], gr.label, b, e);
+        const uint32_t bound = gr.restrict_col != GV_NONE ? row[gr.restrict_col] : 0xffffffffu;
+        const uint32_t cut = bound == 0xffffffffu ? e : lower_bound_u32(g.nbr, b, e, bound);
+        rq += 8ull + (bound != 0xffffffffu ? 4ull * probes(e - b) : 0ull);
+        uint32_t s = cut - b;
+        for (uint32_t j = 0; j < gr.ncoll; ++j) {
+            const uint32_t x = row[gr.coll_col[j]];
+            if (x >= bound) continue;
+            rq += 4ull * probes(cut - b);
+            if (contains_u32(g.nbr, b, cut, x)) --s;
+        }
+        sv[q] = s;
+        len[q] = e - b;
+    }
+    uint32_t jg[G];
+#pragma unroll
+    for (int q = 0; q < G; ++q) jg[q] = 0;
+    uint64_t rows_cur = 1;
+    for (uint32_t j = 0; j < nleaf; ++j) {
+        const uint32_t L = L0 + j;
+        const uint32_t gj = d.group_of[j];
+        uint32_t lj = 0;
+#pragma unroll
+        for (int q = 0; q < G; ++q)
+            if (q == (int)gj) {
+                lj = len[q];
+                jg[q] += 1;
+            }
+        by += rows_cur * (4ull * L + 8ull + 4ull * lj);
+        uint64_t next = 1;
+#pragma unroll
+        for (int q = 0; q < G; ++q)
+            if (q < (int)ng) next *= binom_small(sv[q], jg[q]);
+        if (L + 1 < d.k) by += 4ull * (L + 1) * next;
+        rows_cur = next;
+    }
+    return rows_cur;
+}
+
 __device__ __forceinline__ unsigned lane_id() { return __lane_id(); }
 
 

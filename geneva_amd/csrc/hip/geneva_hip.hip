@@ -164,74 +164,16 @@ struct LeavesArgs {
     uint32_t* gbits;
 };
 
-// C(s, m) for s^m < 2^64 (checked by the caller against the largest degree)
-__device__ __forceinline__ uint64_t binom_small(uint64_t s, uint32_t m) {
-    if (m > s) return 0;
-    uint64_t num = 1, den = 1;
-    for (uint32_t i = 0; i < m; ++i) {
-        num *= s - i;
-        den *= i + 1;
-    }
-    return num / den;
-}
-
 __global__ __launch_bounds__(kBlock) void k_count_leaves(LeavesArgs a) {
-    constexpr int G = GV_MAX_LEAF_GROUPS;
     uint64_t cnt = 0, by = 0, rq = 0;
     uint32_t last_gid = 0xffffffffu;
-    const uint32_t L0 = a.d.width, ng = a.d.ngroups, nleaf = a.d.k - a.d.width;
     const uint64_t nrows = dev_rows(a.nrows, a.nrows_dev, a.stop);
     if (a.units && blockIdx.x == 0 && threadIdx.x == 0 && nrows) atomicAdd(a.units, (unsigned long long)nrows);
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r < nrows; r += stride) {
-        const uint32_t* row = a.rows + r * L0;
-        uint32_t sv[G], len[G];
-        rq += 4ull * L0;
-#pragma unroll
-        for (int g = 0; g < G; ++g) {
-            sv[g] = 0;
-            len[g] = 0;
-            if (g >= (int)ng) continue;
-            const gv_leaf_group_t& gr = a.d.group[g];
-            uint32_t b, e;
-            seg(a.g, gr.part, row[gr.anchor_col], gr.label, b, e);
-            const uint32_t bound = gr.restrict_col != GV_NONE ? row[gr.restrict_col] : 0xffffffffu;
-            const uint32_t cut = bound == 0xffffffffu ? e : lower_bound_u32(a.g.nbr, b, e, bound);
-            rq += 8ull + (bound != 0xffffffffu ? 4ull * probes(e - b) : 0ull);
-            uint32_t s = cut - b;
-            for (uint32_t j = 0; j < gr.ncoll; ++j) {
-                const uint32_t x = row[gr.coll_col[j]];
-                if (x >= bound) continue;
-                rq += 4ull * probes(cut - b);
-                if (contains_u32(a.g.nbr, b, cut, x)) --s;
-            }
-            sv[g] = s;
-            len[g] = e - b;
-        }
-        // rows of every width, leaf by leaf in column order
-        uint32_t jg[G];
-#pragma unroll
-        for (int g = 0; g < G; ++g) jg[g] = 0;
-        uint64_t rows_cur = 1;
-        for (uint32_t j = 0; j < nleaf; ++j) {
-            const uint32_t L = L0 + j;
-            const uint32_t gj = a.d.group_of[j];
-            uint32_t lj = 0;
-#pragma unroll
-            for (int g = 0; g < G; ++g)
-                if (g == (int)gj) {
-                    lj = len[g];
-                    jg[g] += 1;
-                }
-            by += rows_cur * (4ull * L + 8ull + 4ull * lj);
-            uint64_t next = 1;
-#pragma unroll
-            for (int g = 0; g < G; ++g)
-                if (g < (int)ng) next *= binom_small(sv[g], jg[g]);
-            if (L + 1 < a.d.k) by += 4ull * (L + 1) * next;
-            rows_cur = next;
-        }
-        const uint64_t n = rows_cur;   // all leaves placed
+        const uint32_t* row = a.rows + r * a.d.width;
+        rq += 4ull * a.d.width;
+        const uint64_t n = leaf_row_count(a.g, a.d, row, by, rq);
         cnt += n;
         if (n && a.gbits) {
             const uint32_t gid = a.g.gof[row[0]];

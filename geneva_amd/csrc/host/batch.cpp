@@ -33,6 +33,7 @@ uint64_t estimatePeakBytes(const CompiledQuery& cq, const GraphStats& st) {
         const bool last = t + 1 == q.k;
         const bool fused = t + 2 == q.k && q.step[t + 1].nback == 1;
         if (last || fused) break;
+        if (cq.hasLeaves && t >= cq.leaves.width) break;   // counted on the core
         const gv_step_t& s = q.step[t];
         double best = 1e300;
         for (uint32_t i = 0; i < s.nback; ++i) {
@@ -93,7 +94,9 @@ int BatchMiner::run(const std::vector<CompiledQuery>& qs, std::vector<MineResult
         uint32_t maxk = 0;
         while (!pending.empty()) {
             uint32_t slot = 0;
-            rc = gv_batch_admit(b_, &qs[pending.front()].q, pending.front(), &slot);
+            const CompiledQuery& cq = qs[pending.front()];
+            rc = gv_batch_admit_leaves(b_, &cq.q, cq.hasLeaves ? &cq.leaves : nullptr, pending.front(),
+                                       &slot);
             if (rc == GV_ERR_CAPACITY) break;
             if (rc != GV_OK) return rc;
             slotQuery[slot] = pending.front();

@@ -22,7 +22,8 @@ struct GraphStats {
 // table is the label's vertex count, each extension multiplies by the average
 // number of neighbours of the new vertex's label seen from its sparsest back
 // neighbour's label.  The count-only last step (and a single-edge last step
-// fused into the one before it) materialises nothing.
+// fused into the one before it, or trailing leaf groups counted on their core)
+// materialises nothing.
 uint64_t estimatePeakBytes(const CompiledQuery& cq, const GraphStats& st);
 
 struct BatchOptions {

@@ -136,6 +136,9 @@ bool compilePlan(const std::vector<Phase>& mo, const std::map<uint, uint>& restr
         for (uint32_t j = 0; j < t; ++j)
             if (queryLabel.at(out.order[j]) == queryLabel.at(u)) s.coll_col[s.ncoll++] = (uint8_t)j;
     }
+    uint32_t maxDeg = 0;
+    gv_graph_max_degree(g, &maxDeg);
+    out.hasLeaves = !out.empty && leafPlan(q, maxDeg, out.leaves);
     return true;
 }
 
@@ -330,10 +333,8 @@ int PlainMiner::runLevels(const CompiledQuery& cq, const uint32_t* rows, uint64_
     int rc = ensureFrontier(1, n);
     if (rc != GV_OK) return rc;
     frs[1] = lv_[1].fr;
-    uint32_t maxDeg = 0;
-    gv_graph_max_degree(g_, &maxDeg);
-    gv_leaves_t leaves;
-    const bool useLeaves = countByRows_ && leafPlan(q, maxDeg, leaves);
+    const bool useLeaves = countByRows_ && cq.hasLeaves;
+    const gv_leaves_t& leaves = cq.leaves;
     for (uint32_t w = 2; w < q.k; ++w) {
         // the last vertex, when it has one back edge, is counted with the step
         // before it, so width k - 1 is then never materialised; leaf groups

@@ -26,6 +26,8 @@ struct CompiledQuery {
     std::vector<uint32_t> order;      // query vertex of each column
     uint32_t dvPhases = 0;            // two-vertex extension phases in the plan
     uint32_t nphases = 0;
+    bool hasLeaves = false;           // trailing leaf groups (leafPlan) for this graph
+    gv_leaves_t leaves{};
 };
 
 // Plan `text` with the DV planner (or the SV planner) and lower the plan: the

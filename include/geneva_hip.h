@@ -283,6 +283,10 @@ int gv_batch_set_tune(gv_batch_t b, const gv_tune_t* tune);
 /* Place a compiled query in a free slot (stream-ordered; effective at the next
  * iteration).  GV_ERR_CAPACITY when no slot is free. */
 int gv_batch_admit(gv_batch_t b, const gv_query_t* q, uint64_t tag, uint32_t* slot);
+/* As gv_batch_admit; leaves (nullable, see gv_leaves_t) makes the slot count
+ * its leaf groups per core row when its table reaches leaves->width. */
+int gv_batch_admit_leaves(gv_batch_t b, const gv_query_t* q, const gv_leaves_t* leaves,
+                          uint64_t tag, uint32_t* slot);
 /* Launch `iterations` iterations (asynchronous). */
 int gv_batch_run(gv_batch_t b, uint32_t iterations);
 /* Copy every slot's state to info[0..slots) (synchronises). */
