@@ -47,7 +47,7 @@ struct BState {              // dynamic, polled by the host (mirrors gv_slot_inf
     uint64_t cand_base;      // first candidate of this slot in the iteration's space
     uint64_t cand_end;
     uint64_t tile_end;       // end of this slot's tiles (tiles never straddle slots)
-    uint64_t rtile_end;      // end of this slot's sizing tiles (kBlock rows each)
+    uint64_t rtile_end;      // end of this slot's sizing tiles (kSizeTile rows each)
 };
 
 struct BArgs {
@@ -157,8 +157,8 @@ __global__ __launch_bounds__(1024) void k_b_prep(BArgs a) {
     }
     ex = block_scan_1024(rows, wsum, tot);
     const uint64_t total_rows = tot;
-    // sizing tiles: kBlock rows of one slot each
-    const uint64_t nrt = (rows + kBlock - 1) / kBlock;
+    // sizing tiles: kSizeTile rows of one slot each
+    const uint64_t nrt = (rows + kSizeTile - 1) / kSizeTile;
     uint64_t rtot;
     const uint64_t rex = block_scan_1024(nrt, wsum, rtot);
     if (s < a.nslots) {
@@ -200,66 +200,73 @@ __global__ __launch_bounds__(kBlock) void k_b_size(BArgs a) {
         const BDesc& D = a.desc[s];
         const uint32_t L = S.width;
         const uint64_t rows = L == 1 ? D.start_n : S.rows;
-        const uint64_t nrt = (rows + kBlock - 1) / kBlock;
-        const uint64_t local = (RT - (rtile_end[s] - nrt)) * kBlock + threadIdx.x;
-        if (local >= rows) continue;
-        const uint64_t r = S.row_end - rows + local;
+        const uint64_t nrt = (rows + kSizeTile - 1) / kSizeTile;
+        const uint64_t tile0 = (RT - (rtile_end[s] - nrt)) * kSizeTile;
+        const uint64_t rbase = S.row_end - rows;   // slot's first row in the iteration space
         const gv_step_t& st = D.q.step[L];
-        const uint32_t* row = slot_rows(a, s, S, D) + local * L;
+        const uint32_t* trows = slot_rows(a, s, S, D);
+        uint64_t li[kSizeRows];
+        bool ok[kSizeRows];
+#pragma unroll
+        for (int k = 0; k < kSizeRows; ++k) {
+            li[k] = tile0 + (uint64_t)k * kBlock + threadIdx.x;
+            ok[k] = li[k] < rows;
+        }
         if (S.mode == 4) {
-            // leaf groups on this core row: counted here, no candidates (the
+            // leaf groups on these core rows: counted here, no candidates (a
             // row's own read is part of the leaf bytes)
-            uint64_t by = 0;
-            const uint64_t n = leaf_row_count(a.g, D.leaves, row, by, rq);
-            a.cnt[r] = 0;
-            a.anc_beg[r] = 0;
-            a.anc_i[r] = 0;
-            cacc += n;
-            acc += by;
-            if (n && a.g.gof) {
-                const uint32_t gid = a.g.gof[row[0]];
-                if (gid != last_gid) {
-                    atomicOr(&a.gbits[(uint64_t)s * a.gwords + (gid >> 5)], 1u << (gid & 31));
-                    last_gid = gid;
+#pragma unroll
+            for (int k = 0; k < kSizeRows; ++k) {
+                if (!ok[k]) continue;
+                const uint64_t r = rbase + li[k];
+                const uint32_t* row = trows + li[k] * L;
+                uint64_t by = 0;
+                const uint64_t n = leaf_row_count(a.g, D.leaves, row, by, rq);
+                a.cnt[r] = 0;
+                a.anc_beg[r] = 0;
+                a.anc_i[r] = 0;
+                cacc += n;
+                acc += by;
+                if (n && a.g.gof) {
+                    const uint32_t gid = a.g.gof[row[0]];
+                    if (gid != last_gid) {
+                        atomicOr(&a.gbits[(uint64_t)s * a.gwords + (gid >> 5)], 1u << (gid & 31));
+                        last_gid = gid;
+                    }
                 }
             }
             continue;
         }
-        uint32_t best_b = 0, best_len = 0xffffffffu, best_i = 0;
-        uint64_t by = 4ull * L;
-        for (uint32_t i = 0; i < st.nback; ++i) {
-            uint32_t b, e;
-            seg(a.g, st.back_part[i], row[st.back_col[i]], st.label, b, e);
-            const uint32_t len = e - b;
-            by += 8ull + 4ull * len;
-            if (len < best_len) {
-                best_len = len;
-                best_b = b;
-                best_i = i;
-            }
-        }
-        a.anc_beg[r] = best_b;
-        a.anc_i[r] = (uint8_t)best_i;
-        if (S.rmode >= 0) {
-            // counted here, per row: no candidates are left for the walk
-            const gv_step_t& tl = D.q.step[L + 1 < GV_MAXK ? L + 1 : GV_MAXK - 1];
-            const uint64_t n = row_count(a.g, st, tl, S.rmode, L, row, best_b, best_b + best_len, by, rq);
-            a.cnt[r] = 0;
-            cacc += n;
-            if (n && a.g.gof) {
-                const uint32_t gid = a.g.gof[row[0]];
-                if (gid != last_gid) {
-                    atomicOr(&a.gbits[(uint64_t)s * a.gwords + (gid >> 5)], 1u << (gid & 31));
-                    last_gid = gid;
-                }
-            }
-        } else {
-            const uint32_t bound = st.restrict_col != GV_NONE ? row[st.restrict_col] : 0xffffffffu;
-            a.cnt[r] = bound == 0xffffffffu
-                           ? best_len
-                           : lower_bound_u32(a.g.nbr, best_b, best_b + best_len, bound) - best_b;
-        }
+        uint32_t beg[kSizeRows], len[kSizeRows], cnt[kSizeRows], ai[kSizeRows];
+        uint64_t by = 0;
+        size_rows<kSizeRows>(a.g, st, trows, L, li, ok, beg, len, cnt, ai, by, rq);
         acc += by;
+#pragma unroll
+        for (int k = 0; k < kSizeRows; ++k) {
+            if (!ok[k]) continue;
+            const uint64_t r = rbase + li[k];
+            a.anc_beg[r] = beg[k];
+            a.anc_i[r] = (uint8_t)ai[k];
+            if (S.rmode >= 0) {
+                // counted here, per row: no candidates are left for the walk
+                const gv_step_t& tl = D.q.step[L + 1 < GV_MAXK ? L + 1 : GV_MAXK - 1];
+                uint64_t tby = 0;
+                const uint64_t n = row_count(a.g, st, tl, S.rmode, L, trows + li[k] * L, beg[k],
+                                             beg[k] + len[k], tby, rq);
+                acc += tby;
+                a.cnt[r] = 0;
+                cacc += n;
+                if (n && a.g.gof) {
+                    const uint32_t gid = a.g.gof[trows[li[k] * L]];
+                    if (gid != last_gid) {
+                        atomicOr(&a.gbits[(uint64_t)s * a.gwords + (gid >> 5)], 1u << (gid & 31));
+                        last_gid = gid;
+                    }
+                }
+            } else {
+                a.cnt[r] = cnt[k];
+            }
+        }
     }
     flush();
 }

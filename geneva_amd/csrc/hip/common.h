@@ -35,6 +35,8 @@ constexpr int kWavesPerBlock = kBlock / 64;
 constexpr int kIPT = 4;                 // candidates per lane per tile
 constexpr uint64_t kTile = 64 * kIPT;   // candidates per wave tile
 constexpr int kScanItems = 16;          // per thread in the scan
+constexpr int kSizeRows = 4;            // rows per thread in a sizing pass
+constexpr uint64_t kSizeTile = (uint64_t)kBlock * kSizeRows;   // rows per sizing tile
 constexpr int kScanTile = kBlock * kScanItems;
 
 struct DevGraph {
@@ -369,6 +371,86 @@ __device__ __forceinline__ uint64_t leaf_row_count(const DevGraph& g, const gv_l
         rows_cur = next;
     }
     return rows_cur;
+}
+
+// Sizing of R rows per thread (rows ri[k] of a table of width L, those with
+// ok[k]), their memory accesses interleaved so that R independent loads are in
+// flight at each level (row values, label-segment offsets, restriction
+// probes).  Per row: the shortest back-neighbour segment (anchor index ai,
+// start beg, length len) and its length below the restriction bound (cnt).
+// Adds the algorithmic bytes of the row's step (row read and every required
+// list) to `by` and the requested bytes to `rq`.
+template <int R>
+__device__ __forceinline__ void size_rows(const DevGraph& g, const gv_step_t& st,
+                                          const uint32_t* rows, uint32_t L, const uint64_t (&ri)[R],
+                                          const bool (&ok)[R], uint32_t (&beg)[R],
+                                          uint32_t (&len)[R], uint32_t (&cnt)[R],
+                                          uint32_t (&ai)[R], uint64_t& by, uint64_t& rq) {
+    uint32_t bound[R];
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+        beg[k] = 0;
+        len[k] = 0xffffffffu;
+        ai[k] = 0;
+        bound[k] = 0xffffffffu;
+    }
+    if (st.restrict_col != GV_NONE) {
+#pragma unroll
+        for (int k = 0; k < R; ++k)
+            if (ok[k]) bound[k] = rows[ri[k] * L + st.restrict_col];
+    }
+    for (uint32_t i = 0; i < st.nback; ++i) {
+        uint32_t w[R];
+#pragma unroll
+        for (int k = 0; k < R; ++k) w[k] = ok[k] ? rows[ri[k] * L + st.back_col[i]] : 0u;
+        uint32_t b[R], e[R];
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+            b[k] = e[k] = 0;
+            if (ok[k]) seg(g, st.back_part[i], w[k], st.label, b[k], e[k]);
+        }
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+            if (!ok[k]) continue;
+            const uint32_t l = e[k] - b[k];
+            by += 8ull + 4ull * l;
+            if (l < len[k]) {
+                len[k] = l;
+                beg[k] = b[k];
+                ai[k] = i;
+            }
+        }
+    }
+    // lower bound of the restriction in the anchor segment, R searches in step
+    uint32_t lo[R], n[R];
+    bool any = false;
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+        lo[k] = beg[k];
+        n[k] = (ok[k] && bound[k] != 0xffffffffu) ? len[k] : 0u;
+        any |= n[k] > 0;
+        if (ok[k]) {
+            by += 4ull * L;
+            rq += 4ull * L + 8ull * st.nback + (bound[k] != 0xffffffffu ? 4ull * probes(len[k]) : 0ull) + 9ull;
+        }
+    }
+    while (any) {
+        any = false;
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+            if (n[k] == 0) continue;
+            const uint32_t h = n[k] >> 1;
+            if (g.nbr[lo[k] + h] < bound[k]) {
+                lo[k] += h + 1;
+                n[k] -= h + 1;
+            } else {
+                n[k] = h;
+            }
+            any |= n[k] > 0;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < R; ++k) cnt[k] = bound[k] == 0xffffffffu ? len[k] : lo[k] - beg[k];
 }
 
 __device__ __forceinline__ unsigned lane_id() { return __lane_id(); }

@@ -59,30 +59,24 @@ __global__ __launch_bounds__(kBlock) void k_size(SizeArgs a) {
     uint64_t local = 0, rq = 0;
     const uint64_t nrows = dev_rows(a.nrows, a.nrows_dev, a.stop);
     if (a.units && blockIdx.x == 0 && threadIdx.x == 0 && nrows) atomicAdd(a.units, (unsigned long long)nrows);
-    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r < nrows; r += stride) {
-        const uint32_t* row = a.rows + r * a.L;
-        uint32_t best_b = 0, best_len = 0xffffffffu, best_i = 0;
-        local += 4ull * a.L;
-        for (uint32_t i = 0; i < a.st.nback; ++i) {
-            uint32_t b, e;
-            seg(a.g, a.st.back_part[i], row[a.st.back_col[i]], a.st.label, b, e);
-            uint32_t len = e - b;
-            local += 8ull + 4ull * len;
-            if (len < best_len) {
-                best_len = len;
-                best_b = b;
-                best_i = i;
-            }
+    const uint64_t ntiles = (nrows + kSizeTile - 1) / kSizeTile;
+    for (uint64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        uint64_t ri[kSizeRows];
+        bool ok[kSizeRows];
+#pragma unroll
+        for (int k = 0; k < kSizeRows; ++k) {
+            ri[k] = t * kSizeTile + (uint64_t)k * kBlock + threadIdx.x;
+            ok[k] = ri[k] < nrows;
         }
-        uint32_t bound = a.st.restrict_col != GV_NONE ? row[a.st.restrict_col] : 0xffffffffu;
-        uint32_t cut = bound == 0xffffffffu
-                           ? best_len
-                           : lower_bound_u32(a.g.nbr, best_b, best_b + best_len, bound) - best_b;
-        a.anc_beg[r] = best_b;
-        a.cnt[r] = cut;
-        a.anc_i[r] = (uint8_t)best_i;
-        rq += 4ull * a.L + 8ull * a.st.nback + (bound != 0xffffffffu ? 4ull * probes(best_len) : 0ull) + 9ull;
+        uint32_t beg[kSizeRows], len[kSizeRows], cnt[kSizeRows], ai[kSizeRows];
+        size_rows<kSizeRows>(a.g, a.st, a.rows, a.L, ri, ok, beg, len, cnt, ai, local, rq);
+#pragma unroll
+        for (int k = 0; k < kSizeRows; ++k) {
+            if (!ok[k]) continue;
+            a.anc_beg[ri[k]] = beg[k];
+            a.cnt[ri[k]] = cnt[k];
+            a.anc_i[ri[k]] = (uint8_t)ai[k];
+        }
     }
     local = wave_sum_u64(local);
     if (lane_id() == 0 && local) atomicAdd(a.bytes, (unsigned long long)local);
@@ -835,7 +829,7 @@ int gv_size_rows(gv_ctx_t c, gv_graph_t g, const gv_step_t* st, const uint32_t* 
     a.st = *st;
     {
         ProfScope ps(c, kKindSize);
-        hipLaunchKernelGGL(k_size, dim3(grid_for(nrows, kBlock, c->num_cu * 16)), dim3(kBlock), 0,
+        hipLaunchKernelGGL(k_size, dim3(grid_for(nrows, kSizeTile, c->num_cu * 16)), dim3(kBlock), 0,
                            c->stream, a);
     }
     HIPCHK(hipGetLastError());
@@ -1085,7 +1079,7 @@ int gv_mine_levels(gv_ctx_t c, gv_graph_t g, const gv_query_t* q, const uint32_t
             a.units = units_ptr(c, kKindSize);
             a.st = st;
             ProfScope ps(c, kKindSize);
-            hipLaunchKernelGGL(k_size, dim3(grid_for(cap, kBlock, c->num_cu * 16)), dim3(kBlock), 0,
+            hipLaunchKernelGGL(k_size, dim3(grid_for(cap, kSizeTile, c->num_cu * 16)), dim3(kBlock), 0,
                                c->stream, a);
             HIPCHK(hipGetLastError());
         }
