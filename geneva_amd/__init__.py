@@ -311,8 +311,11 @@ class Engine:
         if rc:
             raise _err("count_by_rows", rc)
 
-    def profile(self, on=True):
-        rc = lib().gh_engine_profile(self._h, int(on))
+    def profile(self, on=True, count_bytes=True):
+        """Per-kernel profiling: HIP events around every launch, and with
+        count_bytes the kernels' requested bytes and work units (counting
+        variants of the kernels)."""
+        rc = lib().gh_engine_profile(self._h, (2 if count_bytes else 1) if on else 0)
         if rc:
             raise _err("profile", rc)
         lib().gh_engine_profile_reset(self._h)

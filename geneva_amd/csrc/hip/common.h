@@ -36,6 +36,7 @@ constexpr int kIPT = 4;                 // candidates per lane per tile
 constexpr uint64_t kTile = 64 * kIPT;   // candidates per wave tile
 constexpr int kScanItems = 16;          // per thread in the scan
 constexpr int kSizeRows = 4;            // rows per thread in a sizing pass
+constexpr uint32_t kWin = 256;          // LDS window of a back-neighbour list per wave
 constexpr uint64_t kSizeTile = (uint64_t)kBlock * kSizeRows;   // rows per sizing tile
 constexpr int kScanTile = kBlock * kScanItems;
 
@@ -118,6 +119,44 @@ __device__ __forceinline__ uint64_t wave_find_row(const uint64_t* csum, uint64_t
         lo = lo + (uint64_t)(j - 1) * step + 1;   // answer > probe j - 1
     }
     return lo;
+}
+
+// First index i in [lo, hi) of the sorted a with a[i] >= x (hi if none), found
+// by the whole wave (uniform arguments and result): each round probes 64
+// evenly spaced entries with one load per lane.
+__device__ __forceinline__ uint32_t wave_lower_bound(const uint32_t* a, uint32_t lo, uint32_t hi,
+                                                     uint32_t x) {
+    const unsigned lane = __lane_id();
+    while (lo < hi) {   // the answer lies in [lo, hi]
+        const uint32_t n = hi - lo;
+        const uint32_t step = (n + 63) >> 6;
+        const uint32_t p = lo + lane * step;
+        const uint64_t m = __ballot(lane * step < n && a[p] >= x);
+        if (m == 0) {
+            const uint32_t last = (n - 1) / step < 63 ? (n - 1) / step : 63;
+            lo = lo + last * step + 1;
+            continue;
+        }
+        const unsigned j = __builtin_ctzll(m);
+        if (j == 0) return lo;
+        hi = lo + j * step;
+        lo = lo + (j - 1) * step + 1;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ bool lds_contains(const uint32_t* w, uint32_t n0, uint32_t x) {
+    uint32_t lo = 0, n = n0;
+    while (n > 0) {
+        const uint32_t h = n >> 1;
+        if (w[lo + h] < x) {
+            lo += h + 1;
+            n -= h + 1;
+        } else {
+            n = h;
+        }
+    }
+    return lo < n0 && w[lo] == x;
 }
 
 // Count of the last vertex when it has a single back edge (step `t`, rows of
@@ -507,8 +546,8 @@ template <bool kCount, int IPT>
 __device__ __forceinline__ void walk_tile(const DevGraph& g, const gv_step_t& st,
                                           const gv_step_t& tail, bool has_tail, const TileIn& in,
                                           uint64_t t0, uint64_t t1, const TileOut& o,
-                                          uint32_t* stage, uint64_t* lx, uint32_t* gbits,
-                                          TileAcc& acc) {
+                                          uint32_t* stage, uint64_t* lx, uint32_t* win,
+                                          uint32_t* gbits, TileAcc& acc) {
     const unsigned lane = __lane_id();
     const uint32_t L = in.L, W = L + 1;
     const uint64_t r_lo = wave_find_row(in.csum, in.row_lo, in.row_hi, t0);
@@ -557,6 +596,38 @@ __device__ __forceinline__ void walk_tile(const DevGraph& g, const gv_step_t& st
             ++nob;
         }
         if (lane == 0) acc.rq += 4ull * L + 17ull + 8ull * nob;
+        // the tile's candidates are consecutive entries of the anchor segment,
+        // so they span [vmin, vmax]: the part of each other list in that range
+        // is loaded into LDS once and searched there
+        uint32_t n0 = 0, n1 = 0;
+        bool w0 = false, w1 = false;
+        if (nob > 0) {
+            const uint32_t vmin = g.nbr[abeg + (uint32_t)(t0 - first)];
+            const uint32_t vmax = g.nbr[abeg + (uint32_t)(t1 - 1 - first)];
+            const uint32_t l0 = wave_lower_bound(g.nbr, ob0, oe0, vmin);
+            const uint32_t h0 = wave_lower_bound(g.nbr, l0, oe0, vmax + 1);
+            n0 = h0 - l0;
+            w0 = n0 <= kWin;
+            if (lane == 0) acc.rq += 8ull * probes(oe0 - ob0);
+            if (w0) {
+                for (uint32_t i = lane; i < n0; i += 64) win[i] = g.nbr[l0 + i];
+                if (lane == 0) acc.rq += 4ull * n0;
+            }
+            if (nob > 1) {
+                const uint32_t l1 = wave_lower_bound(g.nbr, ob1, oe1, vmin);
+                const uint32_t h1 = wave_lower_bound(g.nbr, l1, oe1, vmax + 1);
+                n1 = h1 - l1;
+                w1 = n1 <= kWin;
+                if (lane == 0) acc.rq += 8ull * probes(oe1 - ob1);
+                if (w1) {
+                    for (uint32_t i = lane; i < n1; i += 64) win[kWin + i] = g.nbr[l1 + i];
+                    if (lane == 0) acc.rq += 4ull * n1;
+                }
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        }
 #pragma unroll
         for (int it = 0; it < IPT; ++it) {
             const uint64_t c = t0 + (uint64_t)it * 64 + lane;
@@ -567,12 +638,20 @@ __device__ __forceinline__ void walk_tile(const DevGraph& g, const gv_step_t& st
                 acc.rq += 4;
                 ok = v != cv0 && v != cv1 && v != cv2;
                 if (ok && nob > 0) {
-                    acc.rq += 4ull * probes(oe0 - ob0);
-                    ok = contains_u32(g.nbr, ob0, oe0, v);
+                    if (w0) {
+                        ok = lds_contains(win, n0, v);
+                    } else {
+                        acc.rq += 4ull * probes(oe0 - ob0);
+                        ok = contains_u32(g.nbr, ob0, oe0, v);
+                    }
                 }
                 if (ok && nob > 1) {
-                    acc.rq += 4ull * probes(oe1 - ob1);
-                    ok = contains_u32(g.nbr, ob1, oe1, v);
+                    if (w1) {
+                        ok = lds_contains(win + kWin, n1, v);
+                    } else {
+                        acc.rq += 4ull * probes(oe1 - ob1);
+                        ok = contains_u32(g.nbr, ob1, oe1, v);
+                    }
                 }
             }
             okk[it] = ok;
@@ -580,6 +659,8 @@ __device__ __forceinline__ void walk_tile(const DevGraph& g, const gv_step_t& st
             rr[it] = r;
             six[it] = shared ? (uint32_t)(c - first) : ~0u;
         }
+        // the windows are rewritten by the next tile
+        if (nob > 0) __builtin_amdgcn_wave_barrier();
     } else {
         // rows r_lo..r_hi hold the tile: when they fit, stage their candidate
         // starts in LDS (lx[i] = first candidate of row r_lo + i, lx[nr] = end)
@@ -720,6 +801,7 @@ struct gv_ctx {
     int num_cu = 256;
     // profiling: HIP events around every launch, per kernel kind
     bool prof = false;
+    bool prof_bytes = false;
     struct Rec {
         int kind;
         hipEvent_t a, b;
@@ -810,11 +892,11 @@ struct ProfScope {
 };
 
 unsigned long long* req_ptr(gv_ctx* c, int kind) {
-    return (c->prof && c->preq) ? c->preq + kind : nullptr;
+    return (c->prof_bytes && c->preq) ? c->preq + kind : nullptr;
 }
 
 unsigned long long* units_ptr(gv_ctx* c, int kind) {
-    return (c->prof && c->punits) ? c->punits + kind : nullptr;
+    return (c->prof_bytes && c->punits) ? c->punits + kind : nullptr;
 }
 
 int grid_for(uint64_t items, uint64_t per_block, int cap_blocks) {

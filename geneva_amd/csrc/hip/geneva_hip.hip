@@ -55,10 +55,13 @@ __device__ __forceinline__ uint64_t dev_rows(uint64_t n, const unsigned long lon
     return d < n ? d : n;
 }
 
+// kProf: requested bytes and units are accumulated (profiling builds of the
+// launch); otherwise that arithmetic is dead and compiled out.
+template <bool kProf>
 __global__ __launch_bounds__(kBlock) void k_size(SizeArgs a) {
     uint64_t local = 0, rq = 0;
     const uint64_t nrows = dev_rows(a.nrows, a.nrows_dev, a.stop);
-    if (a.units && blockIdx.x == 0 && threadIdx.x == 0 && nrows) atomicAdd(a.units, (unsigned long long)nrows);
+    if (kProf && a.units && blockIdx.x == 0 && threadIdx.x == 0 && nrows) atomicAdd(a.units, (unsigned long long)nrows);
     const uint64_t ntiles = (nrows + kSizeTile - 1) / kSizeTile;
     for (uint64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
         uint64_t ri[kSizeRows];
@@ -80,7 +83,7 @@ __global__ __launch_bounds__(kBlock) void k_size(SizeArgs a) {
     }
     local = wave_sum_u64(local);
     if (lane_id() == 0 && local) atomicAdd(a.bytes, (unsigned long long)local);
-    if (a.req) {
+    if (kProf && a.req) {
         rq = wave_sum_u64(rq);
         if (lane_id() == 0 && rq) atomicAdd(a.req, (unsigned long long)rq);
     }
@@ -105,11 +108,12 @@ struct RowsArgs {
     uint32_t* gbits;
 };
 
+template <bool kProf>
 __global__ __launch_bounds__(kBlock) void k_count_rows(RowsArgs a) {
     uint64_t cnt = 0, by = 0, rq = 0;
     uint32_t last_gid = 0xffffffffu;
     const uint64_t nrows = dev_rows(a.nrows, a.nrows_dev, a.stop);
-    if (a.units && blockIdx.x == 0 && threadIdx.x == 0 && nrows) atomicAdd(a.units, (unsigned long long)nrows);
+    if (kProf && a.units && blockIdx.x == 0 && threadIdx.x == 0 && nrows) atomicAdd(a.units, (unsigned long long)nrows);
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r < nrows; r += stride) {
         const uint32_t* row = a.rows + r * a.L;
@@ -133,7 +137,7 @@ __global__ __launch_bounds__(kBlock) void k_count_rows(RowsArgs a) {
         if (cnt) atomicAdd(a.count, (unsigned long long)cnt);
         if (by) atomicAdd(a.bytes, (unsigned long long)by);
     }
-    if (a.req) {
+    if (kProf && a.req) {
         rq = wave_sum_u64(rq);
         if (lane_id() == 0 && rq) atomicAdd(a.req, (unsigned long long)rq);
     }
@@ -158,11 +162,12 @@ struct LeavesArgs {
     uint32_t* gbits;
 };
 
+template <bool kProf>
 __global__ __launch_bounds__(kBlock) void k_count_leaves(LeavesArgs a) {
     uint64_t cnt = 0, by = 0, rq = 0;
     uint32_t last_gid = 0xffffffffu;
     const uint64_t nrows = dev_rows(a.nrows, a.nrows_dev, a.stop);
-    if (a.units && blockIdx.x == 0 && threadIdx.x == 0 && nrows) atomicAdd(a.units, (unsigned long long)nrows);
+    if (kProf && a.units && blockIdx.x == 0 && threadIdx.x == 0 && nrows) atomicAdd(a.units, (unsigned long long)nrows);
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r < nrows; r += stride) {
         const uint32_t* row = a.rows + r * a.d.width;
@@ -183,7 +188,7 @@ __global__ __launch_bounds__(kBlock) void k_count_leaves(LeavesArgs a) {
         if (cnt) atomicAdd(a.count, (unsigned long long)cnt);
         if (by) atomicAdd(a.bytes, (unsigned long long)by);
     }
-    if (a.req) {
+    if (kProf && a.req) {
         rq = wave_sum_u64(rq);
         if (lane_id() == 0 && rq) atomicAdd(a.req, (unsigned long long)rq);
     }
@@ -347,10 +352,11 @@ struct WalkArgs {
 
 
 
-template <bool kCount>
+template <bool kCount, bool kProf>
 __global__ __launch_bounds__(kBlock) void k_walk(WalkArgs a) {
     __shared__ uint32_t stage[kWavesPerBlock][64 * GV_MAXK];
     __shared__ uint64_t lx[kWavesPerBlock][kTile + 1];
+    __shared__ uint32_t win[kWavesPerBlock][2 * kWin];
     const unsigned lane = lane_id();
     const unsigned wib = threadIdx.x >> 6;
     const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
@@ -361,7 +367,7 @@ __global__ __launch_bounds__(kBlock) void k_walk(WalkArgs a) {
         c0 = 0;
         c1 = nrows ? a.csum[nrows - 1] : 0;
     }
-    if (a.units && blockIdx.x == 0 && threadIdx.x == 0 && c1 > c0)
+    if (kProf && a.units && blockIdx.x == 0 && threadIdx.x == 0 && c1 > c0)
         atomicAdd(a.units, (unsigned long long)(c1 - c0));
     TileIn in;
     in.rows = a.rows;
@@ -378,7 +384,7 @@ __global__ __launch_bounds__(kBlock) void k_walk(WalkArgs a) {
     for (uint64_t t0 = c0 + wave * kTile; t0 < c1; t0 += nwaves * kTile) {
         const uint64_t t1 = t0 + kTile < c1 ? t0 + kTile : c1;
         walk_tile<kCount, kIPT>(a.g, a.st, a.tail, a.has_tail != 0, in, t0, t1, o,
-                                kCount ? nullptr : stage[wib], lx[wib], a.gbits, acc);
+                                kCount ? nullptr : stage[wib], lx[wib], win[wib], a.gbits, acc);
     }
     if (kCount) {
         acc.count = wave_sum_u64(acc.count);
@@ -386,7 +392,7 @@ __global__ __launch_bounds__(kBlock) void k_walk(WalkArgs a) {
     }
     acc.bytes = wave_sum_u64(acc.bytes);
     if (lane == 0 && acc.bytes) atomicAdd(a.bytes, (unsigned long long)acc.bytes);
-    if (a.req) {
+    if (kProf && a.req) {
         acc.rq = wave_sum_u64(acc.rq);
         if (lane == 0 && acc.rq) atomicAdd(a.req, (unsigned long long)acc.rq);
     }
@@ -829,8 +835,12 @@ int gv_size_rows(gv_ctx_t c, gv_graph_t g, const gv_step_t* st, const uint32_t* 
     a.st = *st;
     {
         ProfScope ps(c, kKindSize);
-        hipLaunchKernelGGL(k_size, dim3(grid_for(nrows, kSizeTile, c->num_cu * 16)), dim3(kBlock), 0,
-                           c->stream, a);
+        if (c->prof_bytes)
+            hipLaunchKernelGGL(k_size<true>, dim3(grid_for(nrows, kSizeTile, c->num_cu * 16)),
+                               dim3(kBlock), 0, c->stream, a);
+        else
+            hipLaunchKernelGGL(k_size<false>, dim3(grid_for(nrows, kSizeTile, c->num_cu * 16)),
+                               dim3(kBlock), 0, c->stream, a);
     }
     HIPCHK(hipGetLastError());
     int rc = scan_counts(c, f->cnt, nrows, nullptr, nullptr, f->csum);
@@ -849,10 +859,13 @@ static int launch_walk(gv_ctx* c, WalkArgs& a, bool count) {
     a.req = req_ptr(c, count ? kKindCount : kKindExt);
     a.units = units_ptr(c, count ? kKindCount : kKindExt);
     ProfScope ps(c, count ? kKindCount : kKindExt);
-    if (count)
-        hipLaunchKernelGGL(k_walk<true>, dim3(blocks), dim3(kBlock), 0, c->stream, a);
-    else
-        hipLaunchKernelGGL(k_walk<false>, dim3(blocks), dim3(kBlock), 0, c->stream, a);
+    if (count) {
+        if (c->prof_bytes) hipLaunchKernelGGL((k_walk<true, true>), dim3(blocks), dim3(kBlock), 0, c->stream, a);
+        else hipLaunchKernelGGL((k_walk<true, false>), dim3(blocks), dim3(kBlock), 0, c->stream, a);
+    } else {
+        if (c->prof_bytes) hipLaunchKernelGGL((k_walk<false, true>), dim3(blocks), dim3(kBlock), 0, c->stream, a);
+        else hipLaunchKernelGGL((k_walk<false, false>), dim3(blocks), dim3(kBlock), 0, c->stream, a);
+    }
     HIPCHK(hipGetLastError());
     return GV_OK;
 }
@@ -947,8 +960,12 @@ int gv_count_leaves(gv_ctx_t c, gv_graph_t g, const gv_leaves_t* d, const uint32
     leaves_args(a, c, g, d, rows, nrows);
     {
         ProfScope ps(c, kKindRows);
-        hipLaunchKernelGGL(k_count_leaves, dim3(grid_for(nrows, kBlock, c->num_cu * 16)), dim3(kBlock),
-                           0, c->stream, a);
+        if (c->prof_bytes)
+            hipLaunchKernelGGL(k_count_leaves<true>, dim3(grid_for(nrows, kBlock, c->num_cu * 16)),
+                               dim3(kBlock), 0, c->stream, a);
+        else
+            hipLaunchKernelGGL(k_count_leaves<false>, dim3(grid_for(nrows, kBlock, c->num_cu * 16)),
+                               dim3(kBlock), 0, c->stream, a);
     }
     HIPCHK(hipGetLastError());
     return GV_OK;
@@ -979,8 +996,12 @@ int gv_count_rows(gv_ctx_t c, gv_graph_t g, const gv_step_t* st, const gv_step_t
     a.gbits = (g->d_gof && c->gbits && c->gbits_words * 32ull >= g->ngraphs) ? c->gbits : nullptr;
     {
         ProfScope ps(c, kKindRows);
-        hipLaunchKernelGGL(k_count_rows, dim3(grid_for(nrows, kBlock, c->num_cu * 16)), dim3(kBlock),
-                           0, c->stream, a);
+        if (c->prof_bytes)
+            hipLaunchKernelGGL(k_count_rows<true>, dim3(grid_for(nrows, kBlock, c->num_cu * 16)),
+                               dim3(kBlock), 0, c->stream, a);
+        else
+            hipLaunchKernelGGL(k_count_rows<false>, dim3(grid_for(nrows, kBlock, c->num_cu * 16)),
+                               dim3(kBlock), 0, c->stream, a);
     }
     HIPCHK(hipGetLastError());
     return GV_OK;
@@ -1033,8 +1054,12 @@ int gv_mine_levels(gv_ctx_t c, gv_graph_t g, const gv_query_t* q, const uint32_t
             a.nrows_dev = ndev;
             a.stop = stop;
             ProfScope ps(c, kKindRows);
-            hipLaunchKernelGGL(k_count_leaves, dim3(grid_for(cap, kBlock, c->num_cu * 16)), dim3(kBlock),
-                               0, c->stream, a);
+            if (c->prof_bytes)
+                hipLaunchKernelGGL(k_count_leaves<true>, dim3(grid_for(cap, kBlock, c->num_cu * 16)),
+                                   dim3(kBlock), 0, c->stream, a);
+            else
+                hipLaunchKernelGGL(k_count_leaves<false>, dim3(grid_for(cap, kBlock, c->num_cu * 16)),
+                                   dim3(kBlock), 0, c->stream, a);
             HIPCHK(hipGetLastError());
             break;
         }
@@ -1056,8 +1081,12 @@ int gv_mine_levels(gv_ctx_t c, gv_graph_t g, const gv_query_t* q, const uint32_t
             a.units = units_ptr(c, kKindRows);
             a.gbits = gb;
             ProfScope ps(c, kKindRows);
-            hipLaunchKernelGGL(k_count_rows, dim3(grid_for(cap, kBlock, c->num_cu * 16)), dim3(kBlock),
-                               0, c->stream, a);
+            if (c->prof_bytes)
+                hipLaunchKernelGGL(k_count_rows<true>, dim3(grid_for(cap, kBlock, c->num_cu * 16)),
+                                   dim3(kBlock), 0, c->stream, a);
+            else
+                hipLaunchKernelGGL(k_count_rows<false>, dim3(grid_for(cap, kBlock, c->num_cu * 16)),
+                                   dim3(kBlock), 0, c->stream, a);
             HIPCHK(hipGetLastError());
             break;
         }
@@ -1079,8 +1108,12 @@ int gv_mine_levels(gv_ctx_t c, gv_graph_t g, const gv_query_t* q, const uint32_t
             a.units = units_ptr(c, kKindSize);
             a.st = st;
             ProfScope ps(c, kKindSize);
-            hipLaunchKernelGGL(k_size, dim3(grid_for(cap, kSizeTile, c->num_cu * 16)), dim3(kBlock), 0,
-                               c->stream, a);
+            if (c->prof_bytes)
+                hipLaunchKernelGGL(k_size<true>, dim3(grid_for(cap, kSizeTile, c->num_cu * 16)),
+                                   dim3(kBlock), 0, c->stream, a);
+            else
+                hipLaunchKernelGGL(k_size<false>, dim3(grid_for(cap, kSizeTile, c->num_cu * 16)),
+                                   dim3(kBlock), 0, c->stream, a);
             HIPCHK(hipGetLastError());
         }
         int rc = scan_counts(c, f->cnt, cap, ndev, stop, f->csum);
@@ -1120,10 +1153,15 @@ int gv_mine_levels(gv_ctx_t c, gv_graph_t g, const gv_query_t* q, const uint32_t
         a.units = units_ptr(c, counting ? kKindCount : kKindExt);
         {
             ProfScope ps(c, counting ? kKindCount : kKindExt);
-            if (counting)
-                hipLaunchKernelGGL(k_walk<true>, dim3(c->num_cu * 32), dim3(kBlock), 0, c->stream, a);
+            const dim3 grid(c->num_cu * 32);
+            if (counting && c->prof_bytes)
+                hipLaunchKernelGGL((k_walk<true, true>), grid, dim3(kBlock), 0, c->stream, a);
+            else if (counting)
+                hipLaunchKernelGGL((k_walk<true, false>), grid, dim3(kBlock), 0, c->stream, a);
+            else if (c->prof_bytes)
+                hipLaunchKernelGGL((k_walk<false, true>), grid, dim3(kBlock), 0, c->stream, a);
             else
-                hipLaunchKernelGGL(k_walk<false>, dim3(c->num_cu * 32), dim3(kBlock), 0, c->stream, a);
+                hipLaunchKernelGGL((k_walk<false, false>), grid, dim3(kBlock), 0, c->stream, a);
             HIPCHK(hipGetLastError());
         }
         if (counting) break;
@@ -1209,7 +1247,10 @@ int gv_profile_enable(gv_ctx_t c, int on) {
             return GV_ERR_NOMEM;
         HIPCHK(hipMemset(c->punits, 0, kKinds * sizeof(unsigned long long)));
     }
+    // 1: HIP events around every launch; 2: also requested bytes and units,
+    // accumulated by the kernels' counting variants
     c->prof = on != 0;
+    c->prof_bytes = on >= 2;
     return GV_OK;
 }
 

@@ -301,13 +301,15 @@ int gv_batch_release(gv_batch_t b, uint32_t slot, uint32_t* graph_bits, uint32_t
 /* Slots, rows a slot table holds at a given width, free slots. */
 int gv_batch_info(gv_batch_t b, uint32_t* slots, uint64_t* slot_bytes, uint32_t* free_slots);
 
-/* Per-kernel profiling on the context's stream.  While enabled, every launch
- * is bracketed by HIP events and each kernel kind counts the bytes its
- * algorithm reads and writes (each access once).  Kinds: 0 sizing (k_size),
- * 1 scan (k_scan_*), 2 extension (k_walk, writing rows), 3 counting (k_walk,
- * last step), 4 batch iteration (one graph launch).  read synchronises and
- * returns the launch count, the summed device milliseconds and the requested
- * bytes since the last reset. */
+/* Per-kernel profiling on the context's stream.  on = 1: every launch is
+ * bracketed by HIP events; on = 2: in addition each kernel kind counts the
+ * bytes its algorithm reads and writes (each access once) and its work units
+ * (launch variants that carry that arithmetic); 0: off.  Kinds: 0 sizing
+ * (k_size), 1 scan (k_scan_*), 2 extension (k_walk, writing rows), 3 counting
+ * (k_walk, last step), 4 batch iteration (one graph launch), 5 per-row
+ * counting (k_count_rows, k_count_leaves).  read synchronises and returns the
+ * launch count, the summed device milliseconds and the requested bytes since
+ * the last reset. */
 #define GV_KIND_SIZE 0
 #define GV_KIND_SCAN 1
 #define GV_KIND_EXTEND 2
