@@ -136,7 +136,8 @@ def batched_and_served(args):
         serv.append({"offered_qps": qps, "requests": n,
                      "batched": {"queries_per_s": round(n / sb["wall_s"], 1), "p50_ms": round(pct(lb, 50), 3),
                                  "p99_ms": round(pct(lb, 99), 3), "max_in_flight": int(sb["max_concurrent"]),
-                                 "iterations": int(sb["iterations"]), "to_plain": int(sb["overflowed"])},
+                                 "iterations": int(sb["iterations"]), "to_plain": int(sb["overflowed"]),
+                                 "direct_plain": int(sb["direct"])},
                      "plain": {"queries_per_s": round(n / sp["wall_s"], 1), "p50_ms": round(pct(lp, 50), 3),
                                "p99_ms": round(pct(lp, 99), 3)},
                      "counts_equal": cb == cp})

@@ -297,12 +297,12 @@ class Engine:
         arr = (C.c_double * max(n, 1))(*arrival_s)
         lat = (C.c_double * max(n, 1))()
         cnt = (C.c_uint64 * max(n, 1))()
-        st = (C.c_double * 4)()
+        st = (C.c_double * 5)()
         rc = lib().gh_engine_serve(self._h, self._cstrs(queries), arr, n, 0 if batched else 1,
                                    mem_budget, lat, cnt, st)
         if rc:
             raise _err("serve", rc)
-        stats = dict(zip(("wall_s", "iterations", "max_concurrent", "overflowed"), list(st)))
+        stats = dict(zip(("wall_s", "iterations", "max_concurrent", "overflowed", "direct"), list(st)))
         return list(lat)[:n], list(cnt)[:n], stats
 
     def count_by_rows(self, on=True):

@@ -82,14 +82,16 @@ int autotune(BatchMiner& bm, const std::vector<CompiledQuery>& sample, TuneResul
 // arrived, while slots are free and the summed peak-table estimate of the
 // running requests stays under memBudget, then runs one iteration, in which
 // every running request advances by one vertex; requests leave as they
-// finish.  Plain: requests are mined one at a time in arrival order.
+// finish; a request that arrives alone while nothing runs is mined on the plain
+// path directly.  Plain: requests are mined one at a time in arrival order.
 struct ServeResult {
     std::vector<double> latencyMs;   // arrival to result, per request
     std::vector<uint64_t> counts;
     double wallS = 0;                // first arrival to last result
     uint64_t iterations = 0;
     uint64_t maxConcurrent = 0;
-    uint64_t overflowed = 0;
+    uint64_t overflowed = 0;     // left the batch engine for the plain path
+    uint64_t direct = 0;         // arrived alone at an idle engine: plain path
 };
 int serveBatched(BatchMiner& bm, const std::vector<CompiledQuery>& reqs,
                  const std::vector<double>& arrivalS, uint64_t memBudget, const GraphStats& gs,

@@ -479,6 +479,7 @@ int gh_engine_serve(gh_engine_t e, const char* const* queries, const double* arr
         stats[1] = (double)sr.iterations;
         stats[2] = (double)sr.maxConcurrent;
         stats[3] = (double)sr.overflowed;
+        stats[4] = (double)sr.direct;
     }
     return GV_OK;
 }

@@ -42,8 +42,9 @@ int serveBatched(BatchMiner& bm, const std::vector<CompiledQuery>& reqs,
     const Clock::time_point t0 = Clock::now();
     // a request that cannot stay on the batch engine is mined on the plain
     // path at once (the loop waits for it, as a real server thread would)
-    auto plain = [&](size_t r) -> int {
-        ++overflowed;
+    auto plain = [&](size_t r, bool direct) -> int {
+        if (direct) ++out.direct;
+        else ++overflowed;
         PlainMiner* pm = bm.fallback();
         if (!pm) return GV_ERR_CAPACITY;
         MineResult mr;
@@ -59,6 +60,16 @@ int serveBatched(BatchMiner& bm, const std::vector<CompiledQuery>& reqs,
     gv_batch_t b = bm.handle();
     while (finished < n) {
         const double now = since(t0);
+        // an idle engine and a lone arrived request: the plain path answers it
+        // in one launch sequence instead of one iteration per vertex
+        if (running == 0 && next < n && arrivalS[next] <= now && !reqs[next].empty &&
+            (next + 1 >= n || arrivalS[next + 1] > now) && bm.fallback()) {
+            rc = plain(next, true);
+            if (rc != GV_OK) return rc;
+            ++finished;
+            ++next;
+            continue;
+        }
         // admit what has arrived, in arrival order, while a slot is free and
         // the memory bound holds (a lone request is always admitted; one whose
         // estimate exceeds a slot table goes straight to the plain path)
@@ -70,7 +81,7 @@ int serveBatched(BatchMiner& bm, const std::vector<CompiledQuery>& reqs,
                 continue;
             }
             if (est[next] > slotRowsBytes) {
-                rc = plain(next);
+                rc = plain(next, false);
                 if (rc != GV_OK) return rc;
                 ++finished;
                 ++next;
@@ -114,7 +125,7 @@ int serveBatched(BatchMiner& bm, const std::vector<CompiledQuery>& reqs,
                 inUse -= est[r];
                 --running;
                 if (!done) {
-                    rc = plain(r);
+                    rc = plain(r, false);
                     if (rc != GV_OK) return rc;
                 }
                 ++finished;

@@ -106,9 +106,10 @@ int gh_engine_autotune(gh_engine_t e, const char* const* queries, uint32_t n, ui
 /* Serve n requests arriving at arrival_s[i] seconds (non-decreasing).
  * mode 0: continuous batching with admission under mem_budget bytes of
  * estimated peak tables; mode 1: plain path, one request at a time.
- * latency_ms and counts receive n entries; stats (nullable, 4 doubles): wall
+ * latency_ms and counts receive n entries; stats (nullable, 5 doubles): wall
  * seconds, iterations, most requests in flight, requests re-run on the plain
- * path. */
+ * path, requests that arrived alone at an idle engine and went to the plain
+ * path directly. */
 int gh_engine_serve(gh_engine_t e, const char* const* queries, const double* arrival_s, uint32_t n,
                     int mode, uint64_t mem_budget, double* latency_ms, uint64_t* counts,
                     double* stats);

@@ -3,6 +3,7 @@
     python tests/golden/make_golden.py            # plans, workload 1, Enron, scaled workloads
     python tests/golden/make_golden.py --full2    # + full-size workload 2 counts (long)
     python tests/golden/make_golden.py --only-w4  # workload 4: scaled counts, full-size shards
+                                                  # (--shards-only: the shards alone)
 
 Reference values come from the reference's own cpumine, compiled from its
 sources into oracle/_ref/cpumine (`make -C oracle ref`), run on files written
@@ -18,6 +19,7 @@ import shutil
 import subprocess
 import sys
 import tempfile
+import time
 
 import numpy as np
 
@@ -116,33 +118,65 @@ def full_workload2():
               open(os.path.join(HERE, "workload2_full.json"), "w"), indent=1)
 
 
-def workload4_shards(nq=32, budget_s=0.5):
+def _shard_worker(og, qt, b, e, conn):
+    r = og.mine(qt, threads=1, start_begin=b, start_end=e)
+    conn.send((r.count, r.alg_bytes, r.seconds))
+    conn.close()
+
+
+def workload4_shards(nq=64, cap_s=30.0, procs=8):
     """Full-size workload 4: every query's count is far beyond any CPU run, so
     the oracle mines shards of 8 consecutive start vertices (indices into the
-    id-sorted start list).  Per query, shards start at the middle of the list
-    and move towards the heavy low ids (Chung-Lu weights fall with the id),
-    halving the start index until one shard takes more than budget_s."""
+    id-sorted start list): per query, at the middle of the list and at every
+    halving of that index towards the heavy low ids (Chung-Lu weights fall with
+    the id).  Shards run single-threaded in forked workers, `procs` at a time;
+    a shard that takes more than cap_s seconds is dropped."""
+    import multiprocessing as mp
+    ctx = mp.get_context("fork")
     ds = configs.workload4()
     ogw = O.OracleGraph.of(ds)
     vl = np.asarray(ds.vlabel)
-    cases = []
+    tasks = []
     for qi, qt in enumerate(ds.queries[:nq]):
         v0 = int(re.search(r"svid:(\d+)", G.plan_dump(qt)).group(1))
         l0 = [int(l.split()[2]) for l in qt.splitlines()
               if l.startswith("v ") and int(l.split()[1]) == v0][0]
         nstart = int((vl[1:] == l0).sum())
         p = nstart // 2
-        shards = []
         while p >= 8:
-            r = ogw.mine(qt, threads=os.cpu_count(), start_begin=p, start_end=p + 8)
-            shards.append({"begin": p, "end": p + 8, "count": r.count, "alg_bytes": r.alg_bytes,
-                           "oracle_seconds": round(r.seconds, 3)})
-            if r.seconds > budget_s:
-                break
+            tasks.append((qi, p))
             p //= 2
-        cases.append({"query": qt, "nstart": nstart, "shards": shards})
+    results = {}
+    running = []
+    pending = list(tasks)
+    while pending or running:
+        while pending and len(running) < procs:
+            qi, p = pending.pop(0)
+            a, b = ctx.Pipe(duplex=False)
+            pr = ctx.Process(target=_shard_worker, args=(ogw, ds.queries[qi], p, p + 8, b))
+            pr.start()
+            b.close()
+            running.append((qi, p, pr, a, time.time()))
+        still = []
+        for qi, p, pr, a, t0 in running:
+            if a.poll():
+                cnt, by, secs = a.recv()
+                pr.join()
+                results[(qi, p)] = {"begin": p, "end": p + 8, "count": cnt, "alg_bytes": by,
+                                    "oracle_seconds": round(secs, 3)}
+            elif time.time() - t0 > cap_s:
+                pr.kill()
+                pr.join()
+            else:
+                still.append((qi, p, pr, a, t0))
+        running = still
+        time.sleep(0.01)
+    cases = []
+    for qi, qt in enumerate(ds.queries[:nq]):
+        shards = [results[k] for k in sorted(results, key=lambda k: -k[1]) if k[0] == qi]
+        cases.append({"query": qt, "shards": shards})
         print("workload4 query", qi, [(s["begin"], s["count"]) for s in shards], flush=True)
-    json.dump({"workload": 4, "scale": 1.0, "threads": os.cpu_count(), "cases": cases},
+    json.dump({"workload": 4, "scale": 1.0, "threads": 1, "cap_seconds": cap_s, "cases": cases},
               open(os.path.join(HERE, "workload4_shards.json"), "w"), indent=1)
 
 
@@ -162,7 +196,8 @@ def main():
     if "--only-full2" in sys.argv:
         return full_workload2()
     if "--only-w4" in sys.argv:
-        workload4_scaled()
+        if "--shards-only" not in sys.argv:
+            workload4_scaled()
         return workload4_shards()
     full2 = "--full2" in sys.argv
     if not os.path.exists(O.REF_CPUMINE):
