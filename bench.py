@@ -197,9 +197,7 @@ def main():
             r = eng.mine(q)
             log(f"warmup {w} query {i}: count={r.count} device_ms={r.ms:.2f}")
 
-    # timed steps: every launch bracketed by HIP events on the context's stream
-    # (lean kernel variants, no byte accounting)
-    eng.profile(True, count_bytes=False)
+    # timed steps: nothing but the mining (lean kernels, no events)
     results = []
     t2 = time.perf_counter()
     for s in range(args.steps):
@@ -207,8 +205,16 @@ def main():
         for q in queries:
             step.append(eng.mine(q))
         results.append(step)
-        log(f"step {s}: {sum(r.count for r in step)} embeddings")
     t3 = time.perf_counter()
+    for s, step in enumerate(results):
+        log(f"step {s}: {sum(r.count for r in step)} embeddings")
+    # the same steps again with every launch bracketed by HIP events on the
+    # context's stream: per-kernel device time (the events add their own gaps
+    # between launches, not to the kernels)
+    eng.profile(True, count_bytes=False)
+    for s in range(args.steps):
+        for q in queries:
+            eng.mine(q)
     prof = eng.profile_read()
     # one more step with the counting kernel variants: requested bytes and work
     # units per step of every kernel kind (the work is the same every step)

@@ -398,8 +398,15 @@ __global__ __launch_bounds__(kBlock) void k_walk(WalkArgs a) {
     }
 }
 
-__global__ void k_zero_u64(unsigned long long* p, uint32_t n) {
-    if (threadIdx.x < n) p[threadIdx.x] = 0;
+// zero a query's accumulators, level counters and graph bitset in one launch
+__global__ void k_zero_query(unsigned long long* acc, uint32_t nacc, unsigned long long* lvl,
+                             uint32_t nlvl, uint32_t* bits, uint32_t nbits) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (blockIdx.x == 0) {
+        if (threadIdx.x < nacc) acc[threadIdx.x] = 0;
+        if (threadIdx.x < nlvl) lvl[threadIdx.x] = 0;
+    }
+    for (uint32_t i = t; i < nbits; i += gridDim.x * blockDim.x) bits[i] = 0;
 }
 
 __global__ void k_copy_back(const uint32_t* src, uint32_t* dst, uint64_t n) {
@@ -417,6 +424,20 @@ __global__ void k_copy_back(const uint32_t* src, uint32_t* dst, uint64_t n) {
 }  // namespace
 
 namespace {
+
+// the context's graph bitset holds at least the graph's owners
+int ensure_gbits(gv_ctx* c, const gv_graph* g) {
+    if (!g || !g->ngraphs) return GV_OK;
+    const uint32_t words = (g->ngraphs + 31) / 32;
+    if (words <= c->gbits_words) return GV_OK;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (c->gbits) HIPCHK(hipFree(c->gbits));
+    c->gbits = nullptr;
+    c->gbits_words = 0;
+    if (hipMalloc(&c->gbits, words * sizeof(uint32_t)) != hipSuccess) return GV_ERR_NOMEM;
+    c->gbits_words = words;
+    return GV_OK;
+}
 
 int ensure_tiles(gv_ctx* c, uint64_t n) {
     const uint64_t ntiles = (n + kScanTile - 1) / kScanTile;
@@ -1015,7 +1036,7 @@ int gv_mine_levels(gv_ctx_t c, gv_graph_t g, const gv_query_t* q, const uint32_t
         return GV_ERR_ARG;
     if (leaves && (!valid_leaves(leaves, g) || leaves->k != q->k)) return GV_ERR_ARG;
     if (nstart && !start_rows) return GV_ERR_ARG;
-    if (bits && nwords > c->gbits_words) return GV_ERR_ARG;
+    if (bits && nwords > (g->ngraphs + 31) / 32) return GV_ERR_ARG;
     const uint32_t k = q->k;
     // the widths that are materialised or sized, and their capacities
     uint32_t wlast = 1;   // width of the counted step's input
@@ -1036,10 +1057,17 @@ int gv_mine_levels(gv_ctx_t c, gv_graph_t g, const gv_query_t* q, const uint32_t
         if (!tables[w + 1]) return GV_ERR_ARG;
     }
     HIPCHK(hipSetDevice(c->device));
+    int rc0 = ensure_gbits(c, g);
+    if (rc0 != GV_OK) return rc0;
     const DevGraph dg = dev_view(g);
     unsigned long long* stop = c->acc + 3;
     uint32_t* gb = (g->d_gof && c->gbits && c->gbits_words * 32ull >= g->ngraphs) ? c->gbits : nullptr;
-    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, c->stream, c->lvl, (uint32_t)(GV_MAXK + 1));
+    {
+        const uint32_t words = g->ngraphs ? (g->ngraphs + 31) / 32 : 0;
+        hipLaunchKernelGGL(k_zero_query, dim3(std::max<uint32_t>(1, std::min<uint32_t>((words + 255) / 256, 1024))),
+                           dim3(256), 0, c->stream, c->acc, 8u, c->lvl, (uint32_t)(GV_MAXK + 1),
+                           gb, gb ? words : 0u);
+    }
     const uint32_t* rows = start_rows;
     for (uint32_t w = 1; w <= wlast && nstart; ++w) {
         const gv_step_t& st = q->step[w];
@@ -1184,15 +1212,8 @@ int gv_counters_reset(gv_ctx_t c, gv_graph_t g) {
     HIPCHK(hipSetDevice(c->device));
     HIPCHK(hipMemsetAsync(c->acc, 0, 8 * sizeof(unsigned long long), c->stream));
     if (g && g->ngraphs) {
-        uint32_t words = (g->ngraphs + 31) / 32;
-        if (words > c->gbits_words) {
-            HIPCHK(hipStreamSynchronize(c->stream));
-            if (c->gbits) HIPCHK(hipFree(c->gbits));
-            c->gbits = nullptr;
-            c->gbits_words = 0;
-            if (hipMalloc(&c->gbits, words * sizeof(uint32_t)) != hipSuccess) return GV_ERR_NOMEM;
-            c->gbits_words = words;
-        }
+        int rc = ensure_gbits(c, g);
+        if (rc != GV_OK) return rc;
         HIPCHK(hipMemsetAsync(c->gbits, 0, c->gbits_words * sizeof(uint32_t), c->stream));
     }
     return GV_OK;

@@ -350,25 +350,30 @@ int PlainMiner::runLevels(const CompiledQuery& cq, const uint32_t* rows, uint64_
     uint32_t words = 0;
     gv_graph_owners(g_, &words);
     words = (words + 31) / 32;
+    uint32_t* bits = out.graphBits.empty() ? nullptr : out.graphBits.data();
+    const uint32_t nwords = (uint32_t)out.graphBits.size();
     rc = gv_mine_levels(ctx_, g_, &q, rows, n, tables, caps, frs, countByRows_ ? 1 : 0,
-                        useLeaves ? &leaves : nullptr, nullptr, nullptr, nullptr, 0);
+                        useLeaves ? &leaves : nullptr, &out.count, &out.algBytes, bits, nwords);
     out.launches++;
     if (rc != GV_ERR_CAPACITY) return rc;
     // a table outgrew its level: start over, chunked
     out.overflowRetries++;
     rc = gv_counters_reset(ctx_, g_);
-    if (rc != GV_OK) return rc;
-    return run(q, 1, rows, n, out);
+    if (rc == GV_OK) rc = run(q, 1, rows, n, out);
+    if (rc == GV_OK) rc = gv_counters_read(ctx_, &out.count, &out.algBytes, bits, nwords);
+    return rc;
 }
 
 int PlainMiner::mine(const CompiledQuery& cq, const Shard& sh, MineResult& out) {
     out = MineResult();
     if (sh.stride == 0) return out.status = GV_ERR_ARG;
-    uint64_t nv1 = 0;
-    gv_graph_info(g_, &nv1, nullptr, nullptr, nullptr, nullptr);
-    int rc = gv_counters_reset(ctx_, g_);
-    if (rc != GV_OK) return out.status = rc;
+    uint32_t ngraphs = 0;
+    gv_graph_owners(g_, &ngraphs);
+    const uint32_t words = (ngraphs + 31) / 32;
+    out.graphBits.assign(words, 0);
+    int rc = GV_OK;
     gv_mark(ctx_, 0);
+    bool counted = false;   // out holds the results already
     if (!cq.empty) {
         const uint32_t* rows = nullptr;
         uint64_t n = 0;
@@ -397,7 +402,10 @@ int PlainMiner::mine(const CompiledQuery& cq, const Shard& sh, MineResult& out) 
                 n = pick.size();
             }
         }
-        if (rc == GV_OK) rc = runLevels(cq, rows, n, out);
+        if (rc == GV_OK && n) {
+            rc = runLevels(cq, rows, n, out);
+            counted = rc == GV_OK;
+        }
         if (rc != GV_OK) {
             out.status = rc;
             out.error = gv_last_error();
@@ -408,14 +416,13 @@ int PlainMiner::mine(const CompiledQuery& cq, const Shard& sh, MineResult& out) 
     float ms = 0;
     gv_elapsed_ms(ctx_, 0, 1, &ms);
     out.ms = ms;
-    uint32_t ngraphs = 0;
-    gv_graph_owners(g_, &ngraphs);
-    const uint32_t words = (ngraphs + 31) / 32;
-    out.graphBits.assign(words, 0);
-    rc = gv_counters_read(ctx_, &out.count, &out.algBytes, words ? out.graphBits.data() : nullptr,
-                          words);
-    out.status = rc;
-    return rc;
+    if (!counted) {
+        // empty query or no start vertex: nothing was counted
+        out.count = 0;
+        out.algBytes = 0;
+    }
+    out.status = GV_OK;
+    return GV_OK;
 }
 
 int PlainMiner::mineText(const std::string& text, bool singleV, MineResult& out) {
