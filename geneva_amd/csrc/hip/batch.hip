@@ -649,10 +649,10 @@ int gv_batch_create(gv_ctx_t c, gv_graph_t g, uint32_t slots, uint64_t slot_byte
         return GV_ERR_NOMEM;
     }
     memset(b->h_zero, 0, sizeof(BState));
-    HIPCHK(hipMemset(b->d_desc, 0, sizeof(BDesc) * slots));
-    HIPCHK(hipMemset(b->d_st, 0, sizeof(BState) * slots));
-    HIPCHK(hipMemset(b->d_glob, 0, 8 * 4));
-    HIPCHK(hipMemset(b->d_gbits, 0, 4ull * b->gwords * slots));
+    HIPCHK(hipMemsetAsync(b->d_desc, 0, sizeof(BDesc) * slots, c->stream));
+    HIPCHK(hipMemsetAsync(b->d_st, 0, sizeof(BState) * slots, c->stream));
+    HIPCHK(hipMemsetAsync(b->d_glob, 0, 8 * 4, c->stream));
+    HIPCHK(hipMemsetAsync(b->d_gbits, 0, 4ull * b->gwords * slots, c->stream));
     int rc = capture(b);
     if (rc != GV_OK) {
         gv_batch_destroy(b);

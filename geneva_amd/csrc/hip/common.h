@@ -813,6 +813,11 @@ struct gv_ctx {
     // rows of each level's table when a whole query runs with extents in
     // device memory (gv_mine_levels)
     unsigned long long* lvl = nullptr;
+    // per-vertex weights of a pendant chain (gv_mine_levels), nv1 entries
+    unsigned long long* chainw = nullptr;
+    uint32_t* chainneed = nullptr;        // chainw[y] is needed when == chain_epoch
+    uint64_t chainw_cap = 0;
+    uint32_t chain_epoch = 0;
 };
 
 struct gv_graph {

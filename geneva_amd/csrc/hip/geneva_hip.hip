@@ -195,6 +195,131 @@ __global__ __launch_bounds__(kBlock) void k_count_leaves(LeavesArgs a) {
 }
 
 // ---------------------------------------------------------------------------
+// pendant chain: the last two query vertices u1, u2 hang as a path y - u1 - u2
+// off core column a, carry labels no other query vertex has and are not
+// restricted, so no constraint ties them to the rest of the row: a core row
+// has S(y) = sum over x in N_l1(y) of |N_l2(x)| embeddings.  S is computed once
+// per distinct anchor vertex y (marked by the core rows, then one wave per
+// marked vertex, lanes over its segment) instead of once per core row.
+struct ChainArgs {
+    DevGraph g;
+    const uint32_t* verts;      // vertices of the anchor's label
+    uint64_t nverts;
+    uint32_t l1, p1, l2, p2;
+    unsigned long long* w;      // S by vertex id
+    const uint32_t* need;       // need[y] == epoch: S(y) is read by some core row
+    uint32_t epoch;
+};
+
+__global__ __launch_bounds__(kBlock) void k_chain_mark(const uint32_t* rows, uint64_t nrows_max,
+                                                       const unsigned long long* nrows_dev,
+                                                       const unsigned long long* stop, uint32_t L,
+                                                       uint32_t acol, uint32_t* need, uint32_t epoch) {
+    const uint64_t nrows = dev_rows(nrows_max, nrows_dev, stop);
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r < nrows; r += stride)
+        need[rows[r * L + acol]] = epoch;
+}
+
+__global__ __launch_bounds__(kBlock) void k_chain_weights(ChainArgs a) {
+    const unsigned lane = lane_id();
+    const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+    // 64 vertices per wave: lanes test the marks, the wave sums each marked
+    // vertex's segment together
+    for (uint64_t base = wave * 64; base < a.nverts; base += nwaves * 64) {
+        const uint64_t i = base + lane;
+        const uint32_t yl = i < a.nverts ? a.verts[i] : 0u;
+        uint64_t m = __ballot(i < a.nverts && a.need[yl] == a.epoch);
+        while (m) {
+            const unsigned j = __builtin_ctzll(m);
+            m &= m - 1;
+            const uint32_t y = __shfl(yl, j, 64);
+            uint32_t b, e;
+            seg(a.g, a.p1, y, a.l1, b, e);
+            uint64_t s = 0;
+            for (uint32_t k = b + lane; k < e; k += 64) {
+                uint32_t xb, xe;
+                seg(a.g, a.p2, a.g.nbr[k], a.l2, xb, xe);
+                s += xe - xb;
+            }
+            s = wave_sum_u64(s);
+            if (lane == 0) a.w[y] = s;
+        }
+    }
+}
+
+struct ChainCountArgs {
+    DevGraph g;
+    const uint32_t* rows;
+    uint64_t nrows;
+    const unsigned long long* nrows_dev;   // as in SizeArgs
+    const unsigned long long* stop;
+    uint32_t L, a, l1, p1;
+    const unsigned long long* w;
+    unsigned long long* count;
+    unsigned long long* bytes;
+    unsigned long long* req;
+    unsigned long long* units;
+    uint32_t* gbits;
+};
+
+// Per core row: S(row[a]); algorithmic bytes of the rows the enumeration would
+// hold: the core row (4L + 8 + 4 P1), its P1 written extensions and their
+// reads (P1 (8(L+1) + 8)) and the last vertex's lists (4 S), P1 = |N_l1(y)|.
+template <bool kProf>
+__global__ __launch_bounds__(kBlock) void k_count_chain(ChainCountArgs a) {
+    uint64_t cnt = 0, by = 0, rq = 0;
+    uint32_t last_gid = 0xffffffffu;
+    const uint64_t nrows = dev_rows(a.nrows, a.nrows_dev, a.stop);
+    if (kProf && a.units && blockIdx.x == 0 && threadIdx.x == 0 && nrows)
+        atomicAdd(a.units, (unsigned long long)nrows);
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r < nrows; r += stride) {
+        const uint32_t* row = a.rows + r * a.L;
+        const uint32_t y = row[a.a];
+        uint32_t b, e;
+        seg(a.g, a.p1, y, a.l1, b, e);
+        const uint64_t p1 = e - b, s = a.w[y];
+        by += 4ull * a.L + 8ull + 4ull * p1 + p1 * (8ull * (a.L + 1) + 8ull) + 4ull * s;
+        rq += 4ull * a.L + 16ull;
+        cnt += s;
+        if (s && a.gbits) {
+            const uint32_t gid = a.g.gof[row[0]];
+            if (gid != last_gid) {
+                atomicOr(&a.gbits[gid >> 5], 1u << (gid & 31));
+                last_gid = gid;
+            }
+        }
+    }
+    cnt = wave_sum_u64(cnt);
+    by = wave_sum_u64(by);
+    if (lane_id() == 0) {
+        if (cnt) atomicAdd(a.count, (unsigned long long)cnt);
+        if (by) atomicAdd(a.bytes, (unsigned long long)by);
+    }
+    if (kProf && a.req) {
+        rq = wave_sum_u64(rq);
+        if (lane_id() == 0 && rq) atomicAdd(a.req, (unsigned long long)rq);
+    }
+}
+
+constexpr uint32_t kChainMinDegree = 256;
+
+// The pendant chain of the last two steps of q (counted step w = k - 2),
+// or false.
+bool chain_tail(const gv_query_t* q, uint32_t w) {
+    if (q->k < 3 || w + 2 != q->k) return false;
+    const gv_step_t& u1 = q->step[w];
+    const gv_step_t& u2 = q->step[w + 1];
+    // single back edges, u1 into the core, u2 onto u1; no restriction; no
+    // earlier vertex with either label (u2's list would include u1 if they
+    // shared one)
+    return u1.nback == 1 && u1.back_col[0] < w && u1.ncoll == 0 && u1.restrict_col == GV_NONE &&
+           u2.nback == 1 && u2.back_col[0] == w && u2.ncoll == 0 && u2.restrict_col == GV_NONE;
+}
+
+// ---------------------------------------------------------------------------
 // grid-level inclusive scan of uint32 counts into uint64 sums (reduce, scan of
 // tile sums, scan + add)
 
@@ -518,13 +643,13 @@ int gv_ctx_create(int device, gv_ctx_t* out) {
         delete c;
         return GV_ERR_NOMEM;
     }
-    HIPCHK(hipMemset(c->acc, 0, 8 * sizeof(unsigned long long)));
+    HIPCHK(hipMemsetAsync(c->acc, 0, 8 * sizeof(unsigned long long), c->stream));
     if (hipMalloc(&c->lvl, (GV_MAXK + 1) * sizeof(unsigned long long)) != hipSuccess) {
         hipFree(c->acc);
         delete c;
         return GV_ERR_NOMEM;
     }
-    HIPCHK(hipMemset(c->lvl, 0, (GV_MAXK + 1) * sizeof(unsigned long long)));
+    HIPCHK(hipMemsetAsync(c->lvl, 0, (GV_MAXK + 1) * sizeof(unsigned long long), c->stream));
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->num_cu = prop.multiProcessorCount;
     *out = c;
@@ -541,6 +666,8 @@ int gv_ctx_destroy(gv_ctx_t c) {
     if (c->preq) hipFree(c->preq);
     if (c->punits) hipFree(c->punits);
     if (c->lvl) hipFree(c->lvl);
+    if (c->chainw) hipFree(c->chainw);
+    if (c->chainneed) hipFree(c->chainneed);
     for (auto& r : c->recs) {
         hipEventDestroy(r.a);
         hipEventDestroy(r.b);
@@ -1091,6 +1218,76 @@ int gv_mine_levels(gv_ctx_t c, gv_graph_t g, const gv_query_t* q, const uint32_t
             HIPCHK(hipGetLastError());
             break;
         }
+        // memoised chain weights pay when anchors repeat over many rows, i.e.
+        // on graphs with hubs; on small-degree graphs the walk is cheaper
+        if (fused && count_by_rows && g->max_deg > kChainMinDegree && chain_tail(q, w)) {
+            // weights of every vertex of the anchor's label, then one pass
+            // over the core rows
+            const uint32_t acol = st.back_col[0];
+            const uint32_t alab = acol == 0 ? q->label0 : q->step[acol].label;
+            if (c->chainw_cap < g->nv1) {
+                HIPCHK(hipStreamSynchronize(c->stream));
+                if (c->chainw) HIPCHK(hipFree(c->chainw));
+                if (c->chainneed) HIPCHK(hipFree(c->chainneed));
+                c->chainw = nullptr;
+                c->chainneed = nullptr;
+                c->chainw_cap = 0;
+                if (hipMalloc(&c->chainw, g->nv1 * sizeof(unsigned long long)) != hipSuccess ||
+                    hipMalloc(&c->chainneed, g->nv1 * sizeof(uint32_t)) != hipSuccess)
+                    return GV_ERR_NOMEM;
+                HIPCHK(hipMemsetAsync(c->chainneed, 0, g->nv1 * sizeof(uint32_t), c->stream));
+                c->chainw_cap = g->nv1;
+                c->chain_epoch = 0;
+            }
+            if (++c->chain_epoch == 0) {   // marks of 2^32 queries ago could match
+                HIPCHK(hipStreamSynchronize(c->stream));
+                HIPCHK(hipMemsetAsync(c->chainneed, 0, c->chainw_cap * sizeof(uint32_t), c->stream));
+                c->chain_epoch = 1;
+            }
+            ChainArgs ca;
+            ca.g = dg;
+            ca.verts = g->d_lverts + g->lvoff[alab];
+            ca.nverts = g->lvoff[alab + 1] - g->lvoff[alab];
+            ca.l1 = st.label;
+            ca.p1 = st.back_part[0];
+            ca.l2 = q->step[w + 1].label;
+            ca.p2 = q->step[w + 1].back_part[0];
+            ca.w = c->chainw;
+            ca.need = c->chainneed;
+            ca.epoch = c->chain_epoch;
+            ChainCountArgs a;
+            memset(&a, 0, sizeof(a));
+            a.g = dg;
+            a.rows = rows;
+            a.nrows = cap;
+            a.nrows_dev = ndev;
+            a.stop = stop;
+            a.L = w;
+            a.a = acol;
+            a.l1 = st.label;
+            a.p1 = st.back_part[0];
+            a.w = c->chainw;
+            a.count = c->acc + 0;
+            a.bytes = c->acc + 1;
+            a.req = req_ptr(c, kKindRows);
+            a.units = units_ptr(c, kKindRows);
+            a.gbits = gb;
+            ProfScope ps(c, kKindRows);
+            hipLaunchKernelGGL(k_chain_mark, dim3(grid_for(cap, kBlock, c->num_cu * 16)), dim3(kBlock), 0,
+                               c->stream, rows, cap, ndev, stop, w, acol, c->chainneed, c->chain_epoch);
+            if (ca.nverts)
+                hipLaunchKernelGGL(k_chain_weights,
+                                   dim3(grid_for(ca.nverts, kBlock, c->num_cu * 16)),
+                                   dim3(kBlock), 0, c->stream, ca);
+            if (c->prof_bytes)
+                hipLaunchKernelGGL(k_count_chain<true>, dim3(grid_for(cap, kBlock, c->num_cu * 16)),
+                                   dim3(kBlock), 0, c->stream, a);
+            else
+                hipLaunchKernelGGL(k_count_chain<false>, dim3(grid_for(cap, kBlock, c->num_cu * 16)),
+                                   dim3(kBlock), 0, c->stream, a);
+            HIPCHK(hipGetLastError());
+            break;
+        }
         if ((last || fused) && count_by_rows && count_rows_mode(&st, tail, w) >= 0) {
             RowsArgs a;
             memset(&a, 0, sizeof(a));
@@ -1261,12 +1458,12 @@ int gv_profile_enable(gv_ctx_t c, int on) {
     if (!c) return GV_ERR_ARG;
     if (on && !c->preq) {
         if (hipMalloc(&c->preq, kKinds * sizeof(unsigned long long)) != hipSuccess) return GV_ERR_NOMEM;
-        HIPCHK(hipMemset(c->preq, 0, kKinds * sizeof(unsigned long long)));
+        HIPCHK(hipMemsetAsync(c->preq, 0, kKinds * sizeof(unsigned long long), c->stream));
     }
     if (on && !c->punits) {
         if (hipMalloc(&c->punits, kKinds * sizeof(unsigned long long)) != hipSuccess)
             return GV_ERR_NOMEM;
-        HIPCHK(hipMemset(c->punits, 0, kKinds * sizeof(unsigned long long)));
+        HIPCHK(hipMemsetAsync(c->punits, 0, kKinds * sizeof(unsigned long long), c->stream));
     }
     // 1: HIP events around every launch; 2: also requested bytes and units,
     // accumulated by the kernels' counting variants
@@ -1283,8 +1480,8 @@ int gv_profile_reset(gv_ctx_t c) {
         c->pool.push_back(r.b);
     }
     c->recs.clear();
-    if (c->preq) HIPCHK(hipMemset(c->preq, 0, kKinds * sizeof(unsigned long long)));
-    if (c->punits) HIPCHK(hipMemset(c->punits, 0, kKinds * sizeof(unsigned long long)));
+    if (c->preq) HIPCHK(hipMemsetAsync(c->preq, 0, kKinds * sizeof(unsigned long long), c->stream));
+    if (c->punits) HIPCHK(hipMemsetAsync(c->punits, 0, kKinds * sizeof(unsigned long long), c->stream));
     return GV_OK;
 }
 

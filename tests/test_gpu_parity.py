@@ -133,7 +133,8 @@ def test_bad_query_raises(w1):
 ROW_COUNT_QUERIES = [
     # (query, kind): 0-2 the row-count modes of the counted step (one edge; a
     # path whose last vertex hangs off an earlier column; stars whose last two
-    # leaves share label and centre), 3 leaf groups counted on their core
+    # leaves share label and centre), 3 leaf groups counted on their core, 4
+    # pendant chains counted from per-vertex weights
     ("v 0 1\nv 1 2\ne 0 1 1\n", 0),
     ("v 0 1\nv 1 1\ne 0 1 1\n", 0),
     ("v 0 1\nv 1 2\nv 2 3\nv 3 1\ne 0 1 1\ne 1 2 1\ne 2 3 1\n", 1),
@@ -146,6 +147,11 @@ ROW_COUNT_QUERIES = [
     ("v 0 1\nv 1 2\nv 2 3\nv 3 3\nv 4 1\nv 5 2\ne 0 1 1\ne 0 2 1\ne 0 3 1\ne 1 4 1\ne 1 5 1\n", 3),
     ("v 0 2\nv 1 1\nv 2 1\nv 3 1\ne 0 1 1\ne 0 2 1\ne 0 3 1\n", 3),
     ("v 0 1\nv 1 1\nv 2 2\nv 3 1\ne 0 1 1\ne 0 2 1\ne 0 3 1\n", 3),
+    # pendant chains: a triangle and a 4-cycle with a two-edge tail of labels
+    # no other vertex has
+    ("v 0 1\nv 1 1\nv 2 1\nv 3 2\nv 4 3\ne 0 1 1\ne 1 2 1\ne 0 2 1\ne 2 3 1\ne 3 4 1\n", 4),
+    ("v 0 1\nv 1 1\nv 2 1\nv 3 1\nv 4 2\nv 5 3\ne 0 1 1\ne 1 2 1\ne 2 3 1\ne 3 0 1\ne 0 4 1\n"
+     "e 4 5 1\n", 4),
 ]
 
 
