@@ -239,7 +239,8 @@ __global__ __launch_bounds__(kBlock) void k_b_size(BArgs a) {
         }
         uint32_t beg[kSizeRows], len[kSizeRows], cnt[kSizeRows], ai[kSizeRows];
         uint64_t by = 0;
-        size_rows<kSizeRows>(a.g, st, trows, L, li, ok, beg, len, cnt, ai, by, rq);
+        size_rows<kSizeRows>(a.g, st, trows, L, li, ok, beg, len, cnt, ai, by, rq,
+                             L == 1 ? (int64_t)D.start_off : (int64_t)-1);
         acc += by;
 #pragma unroll
         for (int k = 0; k < kSizeRows; ++k) {

@@ -43,6 +43,11 @@ constexpr int kScanTile = kBlock * kScanItems;
 struct DevGraph {
     const uint32_t* loff;   // [npart][nv1][nl1]
     const uint32_t* nbr;
+    // start planes: sp[part][l][pos] = loff[part][lverts[pos]][l], so a start
+    // table's rows (consecutive positions of lverts) read their label
+    // offsets as consecutive words of two planes
+    const uint32_t* sp;
+    uint64_t sp_n;          // entries of lverts (plane length)
     const uint32_t* gof;    // owner graph per vertex (nullable)
     uint64_t part_stride;   // nv1 * nl1
     uint32_t nl1;           // dense labels + 1
@@ -419,12 +424,16 @@ __device__ __forceinline__ uint64_t leaf_row_count(const DevGraph& g, const gv_l
 // start beg, length len) and its length below the restriction bound (cnt).
 // Adds the algorithmic bytes of the row's step (row read and every required
 // list) to `by` and the requested bytes to `rq`.
+// start_pos >= 0: the table is the start table at lverts position start_pos
+// (width 1), whose row r reads its offsets from the start planes.
 template <int R>
 __device__ __forceinline__ void size_rows(const DevGraph& g, const gv_step_t& st,
                                           const uint32_t* rows, uint32_t L, const uint64_t (&ri)[R],
                                           const bool (&ok)[R], uint32_t (&beg)[R],
                                           uint32_t (&len)[R], uint32_t (&cnt)[R],
-                                          uint32_t (&ai)[R], uint64_t& by, uint64_t& rq) {
+                                          uint32_t (&ai)[R], uint64_t& by, uint64_t& rq,
+                                          int64_t start_pos) {
+    const bool planes = start_pos >= 0 && L == 1 && g.sp;
     uint32_t bound[R];
 #pragma unroll
     for (int k = 0; k < R; ++k) {
@@ -439,14 +448,24 @@ __device__ __forceinline__ void size_rows(const DevGraph& g, const gv_step_t& st
             if (ok[k]) bound[k] = rows[ri[k] * L + st.restrict_col];
     }
     for (uint32_t i = 0; i < st.nback; ++i) {
-        uint32_t w[R];
-#pragma unroll
-        for (int k = 0; k < R; ++k) w[k] = ok[k] ? rows[ri[k] * L + st.back_col[i]] : 0u;
         uint32_t b[R], e[R];
+        if (planes) {
+            const uint32_t* p0 =
+                g.sp + ((uint64_t)st.back_part[i] * g.nl1 + st.label) * g.sp_n + (uint64_t)start_pos;
 #pragma unroll
-        for (int k = 0; k < R; ++k) {
-            b[k] = e[k] = 0;
-            if (ok[k]) seg(g, st.back_part[i], w[k], st.label, b[k], e[k]);
+            for (int k = 0; k < R; ++k) {
+                b[k] = ok[k] ? p0[ri[k]] : 0u;
+                e[k] = ok[k] ? p0[g.sp_n + ri[k]] : 0u;
+            }
+        } else {
+            uint32_t w[R];
+#pragma unroll
+            for (int k = 0; k < R; ++k) w[k] = ok[k] ? rows[ri[k] * L + st.back_col[i]] : 0u;
+#pragma unroll
+            for (int k = 0; k < R; ++k) {
+                b[k] = e[k] = 0;
+                if (ok[k]) seg(g, st.back_part[i], w[k], st.label, b[k], e[k]);
+            }
         }
 #pragma unroll
         for (int k = 0; k < R; ++k) {
@@ -834,6 +853,8 @@ struct gv_graph {
     std::vector<uint64_t> pair_edges;         // [part][NL][NL] directed edges by label pair
     uint32_t* d_loff = nullptr;
     uint32_t* d_nbr = nullptr;
+    uint32_t* d_sp = nullptr;
+    uint64_t sp_n = 0;
     uint32_t* d_lverts = nullptr;
     uint32_t* d_gof = nullptr;
     uint32_t ngraphs = 0;
@@ -857,6 +878,8 @@ DevGraph dev_view(const gv_graph* g) {
     DevGraph d;
     d.loff = g->d_loff;
     d.nbr = g->d_nbr;
+    d.sp = g->d_sp;
+    d.sp_n = g->sp_n;
     d.gof = g->d_gof;
     d.nl1 = g->nlabels + 1;
     d.part_stride = g->nv1 * (uint64_t)(g->nlabels + 1);

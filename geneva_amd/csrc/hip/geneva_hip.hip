@@ -43,6 +43,7 @@ struct SizeArgs {
     // when *stop is set
     const unsigned long long* nrows_dev;
     const unsigned long long* stop;
+    int64_t start_pos;         // >= 0: rows are the start table at this lverts position
     gv_step_t st;
 };
 
@@ -72,7 +73,7 @@ __global__ __launch_bounds__(kBlock) void k_size(SizeArgs a) {
             ok[k] = ri[k] < nrows;
         }
         uint32_t beg[kSizeRows], len[kSizeRows], cnt[kSizeRows], ai[kSizeRows];
-        size_rows<kSizeRows>(a.g, a.st, a.rows, a.L, ri, ok, beg, len, cnt, ai, local, rq);
+        size_rows<kSizeRows>(a.g, a.st, a.rows, a.L, ri, ok, beg, len, cnt, ai, local, rq, a.start_pos);
 #pragma unroll
         for (int k = 0; k < kSizeRows; ++k) {
             if (!ok[k]) continue;
@@ -550,6 +551,13 @@ __global__ void k_copy_back(const uint32_t* src, uint32_t* dst, uint64_t n) {
 
 namespace {
 
+// lverts position of a width-1 table that lies inside lverts (a start table
+// or a contiguous shard of one), else -1
+int64_t start_pos_of(const gv_graph* g, const uint32_t* rows, uint32_t L) {
+    if (L != 1 || !g->d_lverts || rows < g->d_lverts || rows >= g->d_lverts + g->sp_n) return -1;
+    return (int64_t)(rows - g->d_lverts);
+}
+
 // the context's graph bitset holds at least the graph's owners
 int ensure_gbits(gv_ctx* c, const gv_graph* g) {
     if (!g || !g->ngraphs) return GV_OK;
@@ -818,9 +826,20 @@ int gv_graph_create(gv_ctx_t c, const uint32_t* vLabel, uint32_t nblocks,
         for (uint64_t v = 1; v < nv1; ++v)
             if (dl[v] != 0xffffffffu) lverts[pos[dl[v]]++] = (uint32_t)v;
     }
+    // start planes (label-major offsets in lverts order)
+    const uint64_t spn = lverts.size();
+    std::vector<uint32_t> sp((uint64_t)std::max<uint32_t>(nblocks, 1) * nl1 * spn + 1, 0u);
+    for (uint32_t p = 0; p < std::max<uint32_t>(nblocks, 1); ++p) {
+        const uint32_t* lo = loff.data() + (uint64_t)p * nv1 * nl1;
+        for (uint32_t l = 0; l < nl1; ++l) {
+            uint32_t* plane = sp.data() + ((uint64_t)p * nl1 + l) * spn;
+            for (uint64_t i = 0; i < spn; ++i) plane[i] = lo[(uint64_t)lverts[i] * nl1 + l];
+        }
+    }
     gv_graph* g = new gv_graph();
     g->ctx = c;
     g->nv1 = nv1;
+    g->sp_n = spn;
     g->nlabels = NL;
     g->nparts = nblocks;
     g->nnbr = nbr.size();
@@ -841,6 +860,7 @@ int gv_graph_create(gv_ctx_t c, const uint32_t* vLabel, uint32_t nblocks,
     HIPCHK(hipSetDevice(c->device));
     up(&g->d_loff, loff.data(), loff.size() * sizeof(uint32_t));
     up(&g->d_nbr, nbr.data(), nbr.size() * sizeof(uint32_t));
+    up(&g->d_sp, sp.data(), sp.size() * sizeof(uint32_t));
     up(&g->d_lverts, lverts.data(), lverts.size() * sizeof(uint32_t));
     if (rc != GV_OK) {
         gv_graph_destroy(g);
@@ -855,6 +875,7 @@ int gv_graph_destroy(gv_graph_t g) {
     if (g->ctx) hipSetDevice(g->ctx->device);
     if (g->d_loff) hipFree(g->d_loff);
     if (g->d_nbr) hipFree(g->d_nbr);
+    if (g->d_sp) hipFree(g->d_sp);
     if (g->d_lverts) hipFree(g->d_lverts);
     if (g->d_gof) hipFree(g->d_gof);
     delete g;
@@ -981,6 +1002,7 @@ int gv_size_rows(gv_ctx_t c, gv_graph_t g, const gv_step_t* st, const uint32_t* 
     a.req = req_ptr(c, kKindSize);
     a.units = units_ptr(c, kKindSize);
     a.st = *st;
+    a.start_pos = start_pos_of(g, rows, L);
     {
         ProfScope ps(c, kKindSize);
         if (c->prof_bytes)
@@ -1332,6 +1354,7 @@ int gv_mine_levels(gv_ctx_t c, gv_graph_t g, const gv_query_t* q, const uint32_t
             a.req = req_ptr(c, kKindSize);
             a.units = units_ptr(c, kKindSize);
             a.st = st;
+            a.start_pos = start_pos_of(g, rows, w);
             ProfScope ps(c, kKindSize);
             if (c->prof_bytes)
                 hipLaunchKernelGGL(k_size<true>, dim3(grid_for(cap, kSizeTile, c->num_cu * 16)),
