@@ -243,7 +243,7 @@ def main():
     # accounting step
     name = max(prof.items(), key=lambda kv: kv[1][1])[0]
     launches, total_ms = prof[name][0], prof[name][1]
-    step_launches, _, req_step, units_step = acct[name]
+    step_launches, _, req_step, units_step, b_step = acct[name]
     avg_ms = total_ms / max(launches, 1)
     ms_step = total_ms / args.steps
     achieved = req_step / (ms_step / 1e3) / 1e9 if ms_step > 0 else 0.0
@@ -256,9 +256,14 @@ def main():
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                 "traffic": traffic, "launches": launches, "avg_launch_ms": round(avg_ms, 4),
                 "alg_bytes_per_launch": req_step // max(step_launches, 1),
-                "units_per_launch": units_step // max(step_launches, 1)}
+                "units_per_launch": units_step // max(step_launches, 1),
+                # the reference figure B (every required list charged in full,
+                # including rows counted without enumeration) for comparison
+                "b_bytes_per_launch": b_step // max(step_launches, 1),
+                "b_rate_gbps": round(b_step / (ms_step / 1e3) / 1e9, 1) if ms_step > 0 else 0.0}
     kernels = {k: {"launches": v[0], "ms": round(v[1], 3), "req_bytes_per_step": acct[k][2],
-                   "units_per_step": acct[k][3]} for k, v in prof.items()}
+                   "units_per_step": acct[k][3], "b_bytes_per_step": acct[k][4]}
+               for k, v in prof.items()}
 
     eng.close()
     batched = served = None

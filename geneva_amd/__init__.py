@@ -80,6 +80,7 @@ def lib():
         "gh_engine_profile_reset": (i32, [vp]),
         "gh_engine_profile_read": (i32, [vp, i32, C.POINTER(u64), C.POINTER(dbl), C.POINTER(u64)]),
         "gh_engine_profile_units": (i32, [vp, i32, C.POINTER(u64)]),
+        "gh_engine_profile_alg_bytes": (i32, [vp, i32, C.POINTER(u64)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -321,17 +322,19 @@ class Engine:
         lib().gh_engine_profile_reset(self._h)
 
     def profile_read(self):
-        """{kernel: (launches, total device ms, requested bytes, units)} since the last
-        reset; units are rows (k_size, k_scan, k_count_rows) or candidates (k_walk)."""
+        """{kernel: (launches, total device ms, requested bytes, units, algorithmic
+        bytes B)} since the last reset; units are rows (k_size, k_scan,
+        k_count_rows) or candidates (k_walk)."""
         out = {}
         for k, name in self.KERNELS.items():
             n, ms, rq = C.c_uint64(0), C.c_double(0), C.c_uint64(0)
             rc = lib().gh_engine_profile_read(self._h, k, C.byref(n), C.byref(ms), C.byref(rq))
             if rc:
                 raise _err("profile_read", rc)
-            u = C.c_uint64(0)
+            u, b = C.c_uint64(0), C.c_uint64(0)
             lib().gh_engine_profile_units(self._h, k, C.byref(u))
-            out[name] = (n.value, ms.value, rq.value, u.value)
+            lib().gh_engine_profile_alg_bytes(self._h, k, C.byref(b))
+            out[name] = (n.value, ms.value, rq.value, u.value, b.value)
         return out
 
     def mine(self, query_text, single_v=False, start_begin=0, start_end=0, stride=1):

@@ -83,7 +83,10 @@ __global__ __launch_bounds__(kBlock) void k_size(SizeArgs a) {
         }
     }
     local = wave_sum_u64(local);
-    if (lane_id() == 0 && local) atomicAdd(a.bytes, (unsigned long long)local);
+    if (lane_id() == 0 && local) {
+        atomicAdd(a.bytes, (unsigned long long)local);
+        if (kProf && a.units) atomicAdd(a.units + kKinds, (unsigned long long)local);
+    }
     if (kProf && a.req) {
         rq = wave_sum_u64(rq);
         if (lane_id() == 0 && rq) atomicAdd(a.req, (unsigned long long)rq);
@@ -136,7 +139,10 @@ __global__ __launch_bounds__(kBlock) void k_count_rows(RowsArgs a) {
     by = wave_sum_u64(by);
     if (lane_id() == 0) {
         if (cnt) atomicAdd(a.count, (unsigned long long)cnt);
-        if (by) atomicAdd(a.bytes, (unsigned long long)by);
+        if (by) {
+            atomicAdd(a.bytes, (unsigned long long)by);
+            if (kProf && a.units) atomicAdd(a.units + kKinds, (unsigned long long)by);
+        }
     }
     if (kProf && a.req) {
         rq = wave_sum_u64(rq);
@@ -187,7 +193,10 @@ __global__ __launch_bounds__(kBlock) void k_count_leaves(LeavesArgs a) {
     by = wave_sum_u64(by);
     if (lane_id() == 0) {
         if (cnt) atomicAdd(a.count, (unsigned long long)cnt);
-        if (by) atomicAdd(a.bytes, (unsigned long long)by);
+        if (by) {
+            atomicAdd(a.bytes, (unsigned long long)by);
+            if (kProf && a.units) atomicAdd(a.units + kKinds, (unsigned long long)by);
+        }
     }
     if (kProf && a.req) {
         rq = wave_sum_u64(rq);
@@ -297,7 +306,10 @@ __global__ __launch_bounds__(kBlock) void k_count_chain(ChainCountArgs a) {
     by = wave_sum_u64(by);
     if (lane_id() == 0) {
         if (cnt) atomicAdd(a.count, (unsigned long long)cnt);
-        if (by) atomicAdd(a.bytes, (unsigned long long)by);
+        if (by) {
+            atomicAdd(a.bytes, (unsigned long long)by);
+            if (kProf && a.units) atomicAdd(a.units + kKinds, (unsigned long long)by);
+        }
     }
     if (kProf && a.req) {
         rq = wave_sum_u64(rq);
@@ -517,7 +529,10 @@ __global__ __launch_bounds__(kBlock) void k_walk(WalkArgs a) {
         if (lane == 0 && acc.count) atomicAdd(a.count, (unsigned long long)acc.count);
     }
     acc.bytes = wave_sum_u64(acc.bytes);
-    if (lane == 0 && acc.bytes) atomicAdd(a.bytes, (unsigned long long)acc.bytes);
+    if (lane == 0 && acc.bytes) {
+        atomicAdd(a.bytes, (unsigned long long)acc.bytes);
+        if (kProf && a.units) atomicAdd(a.units + kKinds, (unsigned long long)acc.bytes);
+    }
     if (kProf && a.req) {
         acc.rq = wave_sum_u64(acc.rq);
         if (lane == 0 && acc.rq) atomicAdd(a.req, (unsigned long long)acc.rq);
@@ -1484,9 +1499,10 @@ int gv_profile_enable(gv_ctx_t c, int on) {
         HIPCHK(hipMemsetAsync(c->preq, 0, kKinds * sizeof(unsigned long long), c->stream));
     }
     if (on && !c->punits) {
-        if (hipMalloc(&c->punits, kKinds * sizeof(unsigned long long)) != hipSuccess)
+        // [kind]: work units, [kKinds + kind]: algorithmic bytes B
+        if (hipMalloc(&c->punits, 2 * kKinds * sizeof(unsigned long long)) != hipSuccess)
             return GV_ERR_NOMEM;
-        HIPCHK(hipMemsetAsync(c->punits, 0, kKinds * sizeof(unsigned long long), c->stream));
+        HIPCHK(hipMemsetAsync(c->punits, 0, 2 * kKinds * sizeof(unsigned long long), c->stream));
     }
     // 1: HIP events around every launch; 2: also requested bytes and units,
     // accumulated by the kernels' counting variants
@@ -1504,7 +1520,7 @@ int gv_profile_reset(gv_ctx_t c) {
     }
     c->recs.clear();
     if (c->preq) HIPCHK(hipMemsetAsync(c->preq, 0, kKinds * sizeof(unsigned long long), c->stream));
-    if (c->punits) HIPCHK(hipMemsetAsync(c->punits, 0, kKinds * sizeof(unsigned long long), c->stream));
+    if (c->punits) HIPCHK(hipMemsetAsync(c->punits, 0, 2 * kKinds * sizeof(unsigned long long), c->stream));
     return GV_OK;
 }
 
@@ -1512,9 +1528,22 @@ int gv_profile_units(gv_ctx_t c, int kind, uint64_t* units) {
     if (!c || !units || kind < 0 || kind >= kKinds) return GV_ERR_ARG;
     *units = 0;
     if (c->punits) {
-        unsigned long long h[kKinds];
-        HIPCHK(hipMemcpy(h, c->punits, sizeof(h), hipMemcpyDeviceToHost));
+        unsigned long long h[2 * kKinds];
+        HIPCHK(hipMemcpyAsync(h, c->punits, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
         *units = h[kind];
+    }
+    return GV_OK;
+}
+
+int gv_profile_alg_bytes(gv_ctx_t c, int kind, uint64_t* bytes) {
+    if (!c || !bytes || kind < 0 || kind >= kKinds) return GV_ERR_ARG;
+    *bytes = 0;
+    if (c->punits) {
+        unsigned long long h[2 * kKinds];
+        HIPCHK(hipMemcpyAsync(h, c->punits, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        *bytes = h[kKinds + kind];
     }
     return GV_OK;
 }

@@ -499,4 +499,8 @@ int gh_engine_profile_units(gh_engine_t e, int kind, uint64_t* units) {
     return e ? gv_profile_units(e->ctx, kind, units) : GV_ERR_ARG;
 }
 
+int gh_engine_profile_alg_bytes(gh_engine_t e, int kind, uint64_t* bytes) {
+    return e ? gv_profile_alg_bytes(e->ctx, kind, bytes) : GV_ERR_ARG;
+}
+
 }  // extern "C"

@@ -323,6 +323,9 @@ int gv_profile_read(gv_ctx_t ctx, int kind, uint64_t* launches, double* total_ms
 /* Work units of a kind since the last reset: rows (size, scan, rows kinds) or
  * candidates (extend, count); 0 for batch iterations (extents on the device). */
 int gv_profile_units(gv_ctx_t ctx, int kind, uint64_t* units);
+/* Algorithmic bytes B (the reference figure, accumulated exactly as for the
+ * counters) of a kind since the last reset, with on = 2. */
+int gv_profile_alg_bytes(gv_ctx_t ctx, int kind, uint64_t* bytes);
 
 /* Timing on the context's stream (HIP events): record a mark, and the
  * milliseconds between two marks (synchronises on the second). */

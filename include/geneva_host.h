@@ -121,6 +121,7 @@ int gh_engine_profile_reset(gh_engine_t e);
 int gh_engine_profile_read(gh_engine_t e, int kind, uint64_t* launches, double* total_ms,
                            uint64_t* req_bytes);
 int gh_engine_profile_units(gh_engine_t e, int kind, uint64_t* units);
+int gh_engine_profile_alg_bytes(gh_engine_t e, int kind, uint64_t* bytes);
 
 #ifdef __cplusplus
 }
