@@ -315,23 +315,25 @@ __global__ __launch_bounds__(kBlock) void k_b_scan_tiles(BArgs a) {
 }
 
 __global__ __launch_bounds__(kBlock) void k_b_scan_final(BArgs a) {
-    __shared__ uint32_t vals[kScanTile];
+    // the counts and the sums share one padded tile: the counts are all read
+    // (and the block scan has synchronised) before the first sum is written
+    __shared__ uint64_t outv[kScanTilePadded / 2];
+    uint32_t* vals = reinterpret_cast<uint32_t*>(outv);
     __shared__ uint64_t wsum[kWavesPerBlock];
-    __shared__ uint64_t outv[kScanTile / 2];
     const uint64_t n = a.glob[0];
     const uint64_t ntiles = (n + kScanTile - 1) / kScanTile;
     for (uint64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
         const uint64_t base = t * kScanTile;
         for (int i = 0; i < kScanItems; ++i) {
             const uint64_t idx = base + (uint64_t)i * kBlock + threadIdx.x;
-            vals[i * kBlock + threadIdx.x] = idx < n ? a.cnt[idx] : 0u;
+            vals[lds_pad16(i * kBlock + threadIdx.x)] = idx < n ? a.cnt[idx] : 0u;
         }
         __syncthreads();
         uint64_t loc[kScanItems];
         uint64_t s = 0;
 #pragma unroll
         for (int i = 0; i < kScanItems; ++i) {
-            s += vals[threadIdx.x * kScanItems + i];
+            s += vals[lds_pad16(threadIdx.x * kScanItems + i)];
             loc[i] = s;
         }
         uint64_t tot;
@@ -342,12 +344,12 @@ __global__ __launch_bounds__(kBlock) void k_b_scan_final(BArgs a) {
 #pragma unroll
             for (int i = 0; i < kScanItems; ++i) {
                 const int p = threadIdx.x * kScanItems + i - lo;
-                if (p >= 0 && p < kScanTile / 2) outv[p] = ex + loc[i];
+                if (p >= 0 && p < kScanTile / 2) outv[lds_pad16(p)] = ex + loc[i];
             }
             __syncthreads();
             for (int i = threadIdx.x; i < kScanTile / 2; i += kBlock) {
                 const uint64_t idx = base + lo + i;
-                if (idx < n) a.csum[idx] = outv[i];
+                if (idx < n) a.csum[idx] = outv[lds_pad16(i)];
             }
         }
         __syncthreads();

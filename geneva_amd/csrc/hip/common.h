@@ -39,6 +39,11 @@ constexpr int kSizeRows = 4;            // rows per thread in a sizing pass
 constexpr uint32_t kWin = 256;          // LDS window of a back-neighbour list per wave
 constexpr uint64_t kSizeTile = (uint64_t)kBlock * kSizeRows;   // rows per sizing tile
 constexpr int kScanTile = kBlock * kScanItems;
+// The scan reads its LDS tile thread-major (thread t takes words 16t..16t+15),
+// so one pad word after every 16 puts the 64 lanes of a wave on 64 different
+// banks (17t + i is distinct mod 64 and mod 32).
+__host__ __device__ constexpr int lds_pad16(int e) { return e + (e >> 4); }
+constexpr int kScanTilePadded = kScanTile + kScanTile / 16;
 
 struct DevGraph {
     const uint32_t* loff;   // [npart][nv1][nl1]

@@ -415,23 +415,25 @@ __global__ __launch_bounds__(kBlock) void k_scan_final(const uint32_t* in, uint6
                                                        const unsigned long long* n_dev,
                                                        const unsigned long long* stop,
                                                        const uint64_t* tile_off, uint64_t* out) {
-    __shared__ uint32_t vals[kScanTile];
+    // the counts and the sums share one padded tile: the counts are all read
+    // (and the block scan has synchronised) before the first sum is written
+    __shared__ uint64_t outv[kScanTilePadded / 2];
+    uint32_t* vals = reinterpret_cast<uint32_t*>(outv);
     __shared__ uint64_t wsum[kWavesPerBlock];
-    __shared__ uint64_t outv[kScanTile / 2];
     const uint64_t n = dev_rows(n_max, n_dev, stop);
     const uint64_t ntiles = (n + kScanTile - 1) / kScanTile;
     for (uint64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
     const uint64_t base = t * kScanTile;
     for (int i = 0; i < kScanItems; ++i) {
         uint64_t idx = base + (uint64_t)i * kBlock + threadIdx.x;
-        vals[i * kBlock + threadIdx.x] = idx < n ? in[idx] : 0u;
+        vals[lds_pad16(i * kBlock + threadIdx.x)] = idx < n ? in[idx] : 0u;
     }
     __syncthreads();
     uint64_t loc[kScanItems];
     uint64_t s = 0;
 #pragma unroll
     for (int i = 0; i < kScanItems; ++i) {
-        s += vals[threadIdx.x * kScanItems + i];
+        s += vals[lds_pad16(threadIdx.x * kScanItems + i)];
         loc[i] = s;
     }
     uint64_t tot;
@@ -443,12 +445,12 @@ __global__ __launch_bounds__(kBlock) void k_scan_final(const uint32_t* in, uint6
 #pragma unroll
         for (int i = 0; i < kScanItems; ++i) {
             int p = threadIdx.x * kScanItems + i - lo;
-            if (p >= 0 && p < kScanTile / 2) outv[p] = ex + loc[i];
+            if (p >= 0 && p < kScanTile / 2) outv[lds_pad16(p)] = ex + loc[i];
         }
         __syncthreads();
         for (int i = threadIdx.x; i < kScanTile / 2; i += kBlock) {
             uint64_t idx = base + lo + i;
-            if (idx < n) out[idx] = outv[i];
+            if (idx < n) out[idx] = outv[lds_pad16(i)];
         }
     }
     __syncthreads();
