@@ -355,15 +355,19 @@ __device__ __forceinline__ uint64_t row_count(const DevGraph& g, const gv_step_t
     return s;
 }
 
-// C(s, m) for s^m < 2^64 (leaf groups are checked against the largest degree)
-__device__ __forceinline__ uint64_t binom_small(uint64_t s, uint32_t m) {
-    if (m > s) return 0;
-    uint64_t num = 1, den = 1;
-    for (uint32_t i = 0; i < m; ++i) {
-        num *= s - i;
-        den *= i + 1;
-    }
-    return num / den;
+// x / d for a d >= 1 that divides x: the power of two 2^k of d by a shift, its
+// odd part o by multiplying with the inverse of o modulo 2^64 (exact division
+// by multiplication, Granlund & Montgomery, PLDI 1994, section 9).  The
+// inverse comes from Newton's iteration inv <- inv (2 - o inv), which doubles
+// the number of correct low bits, starting from 3 (o o = 1 mod 8 for odd o).
+// Exact when x / d < 2^(64 - k).
+__host__ __device__ __forceinline__ uint64_t div_exact(uint64_t x, uint32_t d) {
+    const uint32_t k = __builtin_ctz(d);
+    const uint64_t o = d >> k;
+    uint64_t inv = o;
+#pragma unroll
+    for (int i = 0; i < 5; ++i) inv *= 2 - o * inv;
+    return (x >> k) * inv;
 }
 
 // One core row of a leaf-group query (gv_leaves_t): the row's embeddings,
@@ -397,25 +401,33 @@ __device__ __forceinline__ uint64_t leaf_row_count(const DevGraph& g, const gv_l
         sv[q] = s;
         len[q] = e - b;
     }
-    uint32_t jg[G];
+    // c[q] = C(s_q, t_q) with t_q leaves of group q placed, updated one leaf at
+    // a time as C(s, t + 1) = C(s, t) (s - t) / (t + 1).  The product before
+    // the division is C(s, t + 1) (t + 1) <= s^(t + 1) < 2^63 (the groups are
+    // checked against the largest degree), so it does not wrap and the quotient
+    // is below 2^(63 - k) for t + 1 = 2^k o.
+    uint64_t c[G];
+    uint32_t t[G];
 #pragma unroll
-    for (int q = 0; q < G; ++q) jg[q] = 0;
+    for (int q = 0; q < G; ++q) {
+        c[q] = 1;
+        t[q] = 0;
+    }
     uint64_t rows_cur = 1;
     for (uint32_t j = 0; j < nleaf; ++j) {
         const uint32_t L = L0 + j;
         const uint32_t gj = d.group_of[j];
-        uint32_t lj = 0;
-#pragma unroll
-        for (int q = 0; q < G; ++q)
-            if (q == (int)gj) {
-                lj = len[q];
-                jg[q] += 1;
-            }
-        by += rows_cur * (4ull * L + 8ull + 4ull * lj);
         uint64_t next = 1;
 #pragma unroll
-        for (int q = 0; q < G; ++q)
-            if (q < (int)ng) next *= binom_small(sv[q], jg[q]);
+        for (int q = 0; q < G; ++q) {
+            if (q >= (int)ng) continue;
+            if (q == (int)gj) {
+                by += rows_cur * (4ull * L + 8ull + 4ull * len[q]);
+                c[q] = sv[q] > t[q] ? div_exact(c[q] * (sv[q] - t[q]), t[q] + 1) : 0ull;
+                t[q] += 1;
+            }
+            next *= c[q];
+        }
         if (L + 1 < d.k) by += 4ull * (L + 1) * next;
         rows_cur = next;
     }
@@ -529,6 +541,47 @@ __device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
     return v;
+}
+
+// Spread accumulators.  Atomics from thousands of waves at one address are
+// applied one after another, and a query's embedding count and algorithmic
+// bytes used to take one such atomic per wave: a third of the plain path's
+// device time on the 1M-vertex Chung-Lu graph.  Each of these sums now lives
+// on kAccSlots cache lines (kAccStride words apart); a block adds its total to
+// line blockIdx % kAccSlots, and k_fold_acc folds the lines into one word
+// before the host reads it.
+constexpr uint32_t kAccSlots = 128;
+constexpr uint32_t kAccStride = 16;   // 128 B
+// The context's accumulator block: words 0..7 (count, bytes, output rows,
+// overflow / stop, ...), then the lines from word kAccLine0: word 0 of a line
+// sums the count, word 1 the algorithmic bytes.
+constexpr uint32_t kAccLine0 = 16;
+constexpr uint32_t kAccWords = kAccLine0 + kAccSlots * kAccStride;
+
+// Adds the block's sums of v0 and v1 (one value per thread) to line
+// blockIdx % kAccSlots of the spread accumulators p0 and p1 (nullable).  Every
+// thread of the block (kBlock threads) calls it.
+__device__ __forceinline__ void block_acc_add2(unsigned long long* p0, uint64_t v0,
+                                               unsigned long long* p1, uint64_t v1) {
+    __shared__ uint64_t part[kWavesPerBlock][2];
+    v0 = wave_sum_u64(v0);
+    v1 = wave_sum_u64(v1);
+    if (__lane_id() == 0) {
+        part[threadIdx.x >> 6][0] = v0;
+        part[threadIdx.x >> 6][1] = v1;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint64_t s0 = 0, s1 = 0;
+#pragma unroll
+        for (int i = 0; i < kWavesPerBlock; ++i) {
+            s0 += part[i][0];
+            s1 += part[i][1];
+        }
+        const uint64_t off = (uint64_t)(blockIdx.x % kAccSlots) * kAccStride;
+        if (p0 && s0) atomicAdd(p0 + off, (unsigned long long)s0);
+        if (p1 && s1) atomicAdd(p1 + off, (unsigned long long)s1);
+    }
 }
 
 

@@ -82,11 +82,11 @@ __global__ __launch_bounds__(kBlock) void k_size(SizeArgs a) {
             a.anc_i[ri[k]] = (uint8_t)ai[k];
         }
     }
-    local = wave_sum_u64(local);
-    if (lane_id() == 0 && local) {
-        atomicAdd(a.bytes, (unsigned long long)local);
-        if (kProf && a.units) atomicAdd(a.units + kKinds, (unsigned long long)local);
+    if (kProf && a.units) {
+        const uint64_t w = wave_sum_u64(local);
+        if (lane_id() == 0 && w) atomicAdd(a.units + kKinds, (unsigned long long)w);
     }
+    block_acc_add2(nullptr, 0, a.bytes, local);
     if (kProf && a.req) {
         rq = wave_sum_u64(rq);
         if (lane_id() == 0 && rq) atomicAdd(a.req, (unsigned long long)rq);
@@ -135,15 +135,11 @@ __global__ __launch_bounds__(kBlock) void k_count_rows(RowsArgs a) {
             }
         }
     }
-    cnt = wave_sum_u64(cnt);
-    by = wave_sum_u64(by);
-    if (lane_id() == 0) {
-        if (cnt) atomicAdd(a.count, (unsigned long long)cnt);
-        if (by) {
-            atomicAdd(a.bytes, (unsigned long long)by);
-            if (kProf && a.units) atomicAdd(a.units + kKinds, (unsigned long long)by);
-        }
+    if (kProf && a.units) {
+        const uint64_t w = wave_sum_u64(by);
+        if (lane_id() == 0 && w) atomicAdd(a.units + kKinds, (unsigned long long)w);
     }
+    block_acc_add2(a.count, cnt, a.bytes, by);
     if (kProf && a.req) {
         rq = wave_sum_u64(rq);
         if (lane_id() == 0 && rq) atomicAdd(a.req, (unsigned long long)rq);
@@ -189,15 +185,11 @@ __global__ __launch_bounds__(kBlock) void k_count_leaves(LeavesArgs a) {
             }
         }
     }
-    cnt = wave_sum_u64(cnt);
-    by = wave_sum_u64(by);
-    if (lane_id() == 0) {
-        if (cnt) atomicAdd(a.count, (unsigned long long)cnt);
-        if (by) {
-            atomicAdd(a.bytes, (unsigned long long)by);
-            if (kProf && a.units) atomicAdd(a.units + kKinds, (unsigned long long)by);
-        }
+    if (kProf && a.units) {
+        const uint64_t w = wave_sum_u64(by);
+        if (lane_id() == 0 && w) atomicAdd(a.units + kKinds, (unsigned long long)w);
     }
+    block_acc_add2(a.count, cnt, a.bytes, by);
     if (kProf && a.req) {
         rq = wave_sum_u64(rq);
         if (lane_id() == 0 && rq) atomicAdd(a.req, (unsigned long long)rq);
@@ -302,15 +294,11 @@ __global__ __launch_bounds__(kBlock) void k_count_chain(ChainCountArgs a) {
             }
         }
     }
-    cnt = wave_sum_u64(cnt);
-    by = wave_sum_u64(by);
-    if (lane_id() == 0) {
-        if (cnt) atomicAdd(a.count, (unsigned long long)cnt);
-        if (by) {
-            atomicAdd(a.bytes, (unsigned long long)by);
-            if (kProf && a.units) atomicAdd(a.units + kKinds, (unsigned long long)by);
-        }
+    if (kProf && a.units) {
+        const uint64_t w = wave_sum_u64(by);
+        if (lane_id() == 0 && w) atomicAdd(a.units + kKinds, (unsigned long long)w);
     }
+    block_acc_add2(a.count, cnt, a.bytes, by);
     if (kProf && a.req) {
         rq = wave_sum_u64(rq);
         if (lane_id() == 0 && rq) atomicAdd(a.req, (unsigned long long)rq);
@@ -526,15 +514,11 @@ __global__ __launch_bounds__(kBlock) void k_walk(WalkArgs a) {
         walk_tile<kCount, kIPT>(a.g, a.st, a.tail, a.has_tail != 0, in, t0, t1, o,
                                 kCount ? nullptr : stage[wib], lx[wib], win[wib], a.gbits, acc);
     }
-    if (kCount) {
-        acc.count = wave_sum_u64(acc.count);
-        if (lane == 0 && acc.count) atomicAdd(a.count, (unsigned long long)acc.count);
+    if (kProf && a.units) {
+        const uint64_t w = wave_sum_u64(acc.bytes);
+        if (lane == 0 && w) atomicAdd(a.units + kKinds, (unsigned long long)w);
     }
-    acc.bytes = wave_sum_u64(acc.bytes);
-    if (lane == 0 && acc.bytes) {
-        atomicAdd(a.bytes, (unsigned long long)acc.bytes);
-        if (kProf && a.units) atomicAdd(a.units + kKinds, (unsigned long long)acc.bytes);
-    }
+    block_acc_add2(kCount ? a.count : nullptr, acc.count, a.bytes, acc.bytes);
     if (kProf && a.req) {
         acc.rq = wave_sum_u64(acc.rq);
         if (lane == 0 && acc.rq) atomicAdd(a.req, (unsigned long long)acc.rq);
@@ -545,11 +529,26 @@ __global__ __launch_bounds__(kBlock) void k_walk(WalkArgs a) {
 __global__ void k_zero_query(unsigned long long* acc, uint32_t nacc, unsigned long long* lvl,
                              uint32_t nlvl, uint32_t* bits, uint32_t nbits) {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (blockIdx.x == 0) {
-        if (threadIdx.x < nacc) acc[threadIdx.x] = 0;
-        if (threadIdx.x < nlvl) lvl[threadIdx.x] = 0;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    if (blockIdx.x == 0 && threadIdx.x < nlvl) lvl[threadIdx.x] = 0;
+    for (uint32_t i = t; i < nacc; i += stride) acc[i] = 0;
+    for (uint32_t i = t; i < nbits; i += stride) bits[i] = 0;
+}
+
+// acc[0] and acc[1] = the sums over the spread lines of the count and the
+// algorithmic bytes (one wave)
+__global__ void k_fold_acc(unsigned long long* acc) {
+    uint64_t cnt = 0, by = 0;
+    for (uint32_t i = threadIdx.x; i < kAccSlots; i += 64) {
+        cnt += acc[kAccLine0 + i * kAccStride];
+        by += acc[kAccLine0 + i * kAccStride + 1];
     }
-    for (uint32_t i = t; i < nbits; i += gridDim.x * blockDim.x) bits[i] = 0;
+    cnt = wave_sum_u64(cnt);
+    by = wave_sum_u64(by);
+    if (threadIdx.x == 0) {
+        acc[0] = cnt;
+        acc[1] = by;
+    }
 }
 
 __global__ void k_copy_back(const uint32_t* src, uint32_t* dst, uint64_t n) {
@@ -664,11 +663,11 @@ int gv_ctx_create(int device, gv_ctx_t* out) {
         return GV_ERR_HIP;
     }
     for (auto& e : c->ev) HIPCHK(hipEventCreate(&e));
-    if (hipMalloc(&c->acc, 8 * sizeof(unsigned long long)) != hipSuccess) {
+    if (hipMalloc(&c->acc, kAccWords * sizeof(unsigned long long)) != hipSuccess) {
         delete c;
         return GV_ERR_NOMEM;
     }
-    HIPCHK(hipMemsetAsync(c->acc, 0, 8 * sizeof(unsigned long long), c->stream));
+    HIPCHK(hipMemsetAsync(c->acc, 0, kAccWords * sizeof(unsigned long long), c->stream));
     if (hipMalloc(&c->lvl, (GV_MAXK + 1) * sizeof(unsigned long long)) != hipSuccess) {
         hipFree(c->acc);
         delete c;
@@ -1015,7 +1014,7 @@ int gv_size_rows(gv_ctx_t c, gv_graph_t g, const gv_step_t* st, const uint32_t* 
     a.anc_beg = f->anc_beg;
     a.cnt = f->cnt;
     a.anc_i = f->anc_i;
-    a.bytes = c->acc + 1;
+    a.bytes = c->acc + kAccLine0 + 1;
     a.req = req_ptr(c, kKindSize);
     a.units = units_ptr(c, kKindSize);
     a.st = *st;
@@ -1071,8 +1070,8 @@ static void walk_common(WalkArgs& a, gv_ctx* c, gv_graph* g, const gv_step_t* st
     a.anc_i = f->anc_i;
     a.c0 = c0;
     a.c1 = c1;
-    a.count = c->acc + 0;
-    a.bytes = c->acc + 1;
+    a.count = c->acc + kAccLine0;
+    a.bytes = c->acc + kAccLine0 + 1;
     a.out_n = c->acc + 2;
     a.overflow = c->acc + 3;
     a.st = *st;
@@ -1130,8 +1129,8 @@ static void leaves_args(LeavesArgs& a, gv_ctx* c, gv_graph* g, const gv_leaves_t
     a.rows = rows;
     a.nrows = nrows;
     a.d = *d;
-    a.count = c->acc + 0;
-    a.bytes = c->acc + 1;
+    a.count = c->acc + kAccLine0;
+    a.bytes = c->acc + kAccLine0 + 1;
     a.req = req_ptr(c, kKindRows);
     a.units = units_ptr(c, kKindRows);
     a.gbits = (g->d_gof && c->gbits && c->gbits_words * 32ull >= g->ngraphs) ? c->gbits : nullptr;
@@ -1176,8 +1175,8 @@ int gv_count_rows(gv_ctx_t c, gv_graph_t g, const gv_step_t* st, const gv_step_t
     a.mode = mode;
     a.st = *st;
     if (tail) a.tail = *tail;
-    a.count = c->acc + 0;
-    a.bytes = c->acc + 1;
+    a.count = c->acc + kAccLine0;
+    a.bytes = c->acc + kAccLine0 + 1;
     a.req = req_ptr(c, kKindRows);
     a.units = units_ptr(c, kKindRows);
     a.gbits = (g->d_gof && c->gbits && c->gbits_words * 32ull >= g->ngraphs) ? c->gbits : nullptr;
@@ -1231,7 +1230,7 @@ int gv_mine_levels(gv_ctx_t c, gv_graph_t g, const gv_query_t* q, const uint32_t
     {
         const uint32_t words = g->ngraphs ? (g->ngraphs + 31) / 32 : 0;
         hipLaunchKernelGGL(k_zero_query, dim3(std::max<uint32_t>(1, std::min<uint32_t>((words + 255) / 256, 1024))),
-                           dim3(256), 0, c->stream, c->acc, 8u, c->lvl, (uint32_t)(GV_MAXK + 1),
+                           dim3(256), 0, c->stream, c->acc, kAccWords, c->lvl, (uint32_t)(GV_MAXK + 1),
                            gb, gb ? words : 0u);
     }
     const uint32_t* rows = start_rows;
@@ -1306,8 +1305,8 @@ int gv_mine_levels(gv_ctx_t c, gv_graph_t g, const gv_query_t* q, const uint32_t
             a.l1 = st.label;
             a.p1 = st.back_part[0];
             a.w = c->chainw;
-            a.count = c->acc + 0;
-            a.bytes = c->acc + 1;
+            a.count = c->acc + kAccLine0;
+            a.bytes = c->acc + kAccLine0 + 1;
             a.req = req_ptr(c, kKindRows);
             a.units = units_ptr(c, kKindRows);
             a.gbits = gb;
@@ -1339,8 +1338,8 @@ int gv_mine_levels(gv_ctx_t c, gv_graph_t g, const gv_query_t* q, const uint32_t
             a.mode = count_rows_mode(&st, tail, w);
             a.st = st;
             if (tail) a.tail = *tail;
-            a.count = c->acc + 0;
-            a.bytes = c->acc + 1;
+            a.count = c->acc + kAccLine0;
+            a.bytes = c->acc + kAccLine0 + 1;
             a.req = req_ptr(c, kKindRows);
             a.units = units_ptr(c, kKindRows);
             a.gbits = gb;
@@ -1367,7 +1366,7 @@ int gv_mine_levels(gv_ctx_t c, gv_graph_t g, const gv_query_t* q, const uint32_t
             a.anc_beg = f->anc_beg;
             a.cnt = f->cnt;
             a.anc_i = f->anc_i;
-            a.bytes = c->acc + 1;
+            a.bytes = c->acc + kAccLine0 + 1;
             a.req = req_ptr(c, kKindSize);
             a.units = units_ptr(c, kKindSize);
             a.st = st;
@@ -1398,8 +1397,8 @@ int gv_mine_levels(gv_ctx_t c, gv_graph_t g, const gv_query_t* q, const uint32_t
         a.cnt = f->cnt;
         a.csum = f->csum;
         a.anc_i = f->anc_i;
-        a.count = c->acc + 0;
-        a.bytes = c->acc + 1;
+        a.count = c->acc + kAccLine0;
+        a.bytes = c->acc + kAccLine0 + 1;
         a.overflow = stop;
         a.st = st;
         const bool counting = last || fused;
@@ -1433,6 +1432,8 @@ int gv_mine_levels(gv_ctx_t c, gv_graph_t g, const gv_query_t* q, const uint32_t
         rows = tables[w + 1];
     }
     unsigned long long h[4];
+    hipLaunchKernelGGL(k_fold_acc, dim3(1), dim3(64), 0, c->stream, c->acc);
+    HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(h, c->acc, sizeof(h), hipMemcpyDeviceToHost, c->stream));
     if (bits && nwords)
         HIPCHK(hipMemcpyAsync(bits, c->gbits, nwords * sizeof(uint32_t), hipMemcpyDeviceToHost,
@@ -1447,7 +1448,7 @@ int gv_mine_levels(gv_ctx_t c, gv_graph_t g, const gv_query_t* q, const uint32_t
 int gv_counters_reset(gv_ctx_t c, gv_graph_t g) {
     if (!c) return GV_ERR_ARG;
     HIPCHK(hipSetDevice(c->device));
-    HIPCHK(hipMemsetAsync(c->acc, 0, 8 * sizeof(unsigned long long), c->stream));
+    HIPCHK(hipMemsetAsync(c->acc, 0, kAccWords * sizeof(unsigned long long), c->stream));
     if (g && g->ngraphs) {
         int rc = ensure_gbits(c, g);
         if (rc != GV_OK) return rc;
@@ -1460,6 +1461,8 @@ int gv_counters_read(gv_ctx_t c, uint64_t* count, uint64_t* bytes, uint32_t* bit
                      uint32_t nwords) {
     if (!c) return GV_ERR_ARG;
     unsigned long long h[2];
+    hipLaunchKernelGGL(k_fold_acc, dim3(1), dim3(64), 0, c->stream, c->acc);
+    HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(h, c->acc, sizeof(h), hipMemcpyDeviceToHost, c->stream));
     if (bits && nwords) {
         if (nwords > c->gbits_words) return GV_ERR_ARG;
