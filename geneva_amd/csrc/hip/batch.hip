@@ -69,7 +69,17 @@ struct BArgs {
     uint64_t tile;              // candidates per walk tile (64 * IPT)
     uint32_t* gbits;            // nslots * gwords
     uint32_t gwords;
+    // spread count / bytes accumulators: kBLines lines (kAccStride words) per
+    // slot; blocks add to line blockIdx % kBLines of their slot and k_b_post
+    // folds the lines into the slot's state
+    unsigned long long* sacc;
 };
+
+constexpr uint32_t kBLines = 16;
+
+__device__ __forceinline__ unsigned long long* slot_line(const BArgs& a, uint32_t s) {
+    return a.sacc + ((uint64_t)s * kBLines + blockIdx.x % kBLines) * kAccStride;
+}
 
 __device__ __forceinline__ const uint32_t* slot_rows(const BArgs& a, uint32_t s, const BState& S,
                                                      const BDesc& D) {
@@ -184,8 +194,9 @@ __global__ __launch_bounds__(kBlock) void k_b_size(BArgs a) {
     auto flush = [&]() {
         if (cur == 0xffffffffu) return;
         const uint64_t w = wave_sum_u64(acc), c = wave_sum_u64(cacc);
-        if (lane == 0 && w) atomicAdd(&a.st[cur].bytes, (unsigned long long)w);
-        if (lane == 0 && c) atomicAdd(&a.st[cur].count, (unsigned long long)c);
+        unsigned long long* line = slot_line(a, cur);
+        if (lane == 0 && c) atomicAdd(line, (unsigned long long)c);
+        if (lane == 0 && w) atomicAdd(line + 1, (unsigned long long)w);
         acc = cacc = 0;
     };
     for (uint64_t RT = blockIdx.x; RT < ntiles; RT += gridDim.x) {
@@ -394,20 +405,29 @@ __global__ __launch_bounds__(kBlock) void k_b_walk(BArgs a) {
     __syncthreads();
     const uint64_t ntiles = a.glob[2];
     const unsigned lane = __lane_id(), wib = threadIdx.x >> 6;
-    const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
     uint32_t cur = 0xffffffffu;
     TileAcc acc;
+    __shared__ uint64_t resv[3 * kWavesPerBlock];
+    const BlockReserve reserve{resv};
     auto flush = [&]() {
         if (cur == 0xffffffffu) return;
         const uint64_t c = wave_sum_u64(acc.count), b = wave_sum_u64(acc.bytes);
+        unsigned long long* line = slot_line(a, cur);
         if (lane == 0) {
-            if (c) atomicAdd(&a.st[cur].count, (unsigned long long)c);
-            if (b) atomicAdd(&a.st[cur].bytes, (unsigned long long)b);
+            if (c) atomicAdd(line, (unsigned long long)c);
+            if (b) atomicAdd(line + 1, (unsigned long long)b);
         }
         acc = TileAcc();
     };
-    for (uint64_t T = wave; T < ntiles; T += nwaves) {
+    // block rounds: wave wib of block b takes tile (round * blocks + b) * 4 + wib,
+    // and every wave of the block runs every round (the reservation is shared)
+    for (uint64_t T0 = (uint64_t)blockIdx.x * kWavesPerBlock; T0 < ntiles;
+         T0 += (uint64_t)gridDim.x * kWavesPerBlock) {
+        const uint64_t T = T0 + wib;
+        if (T >= ntiles) {
+            reserve(nullptr, 0);
+            continue;
+        }
         // the tile's slot, made wave-uniform so its descriptor is read once
         const uint32_t s = __builtin_amdgcn_readfirstlane(slot_of(tile_end, a.nslots, T));
         if (s != cur) {
@@ -437,12 +457,13 @@ __global__ __launch_bounds__(kBlock) void k_b_walk(BArgs a) {
         if (S.mode == 1) {
             TileOut o{slot_out(a, s, S), a.slot_uints / (L + 1), &a.st[s].out_rows, &a.st[s].overflow};
             walk_tile<false, IPT>(a.g, st, st, false, in, t0, t1, o, stage[wib], lx[wib], win[wib], gb,
-                                  acc);
+                                  acc, reserve);
         } else {
             TileOut o{nullptr, 0, nullptr, nullptr};
             const gv_step_t& tl = D.q.step[L + 1 < GV_MAXK ? L + 1 : GV_MAXK - 1];
             walk_tile<true, IPT>(a.g, st, tl, S.mode == 3, in, t0, t1, o, nullptr, lx[wib], win[wib], gb,
-                                 acc);
+                                 acc, reserve);
+            reserve(nullptr, 0);
         }
     }
     flush();
@@ -452,6 +473,16 @@ __global__ __launch_bounds__(1024) void k_b_post(BArgs a) {
     const uint32_t s = threadIdx.x;
     if (s >= a.nslots) return;
     BState& S = a.st[s];
+    unsigned long long* lines = a.sacc + (uint64_t)s * kBLines * kAccStride;
+    uint64_t cnt = 0, by = 0;
+    for (uint32_t i = 0; i < kBLines; ++i) {
+        cnt += lines[i * kAccStride];
+        by += lines[i * kAccStride + 1];
+        lines[i * kAccStride] = 0;
+        lines[i * kAccStride + 1] = 0;
+    }
+    S.count += cnt;
+    S.bytes += by;
     if (S.mode == 0) return;
     if (S.mode >= 2) {   // counted (count, count + tail, leaf groups)
         S.state = GV_SLOT_DONE;
@@ -490,6 +521,7 @@ struct gv_batch {
     uint64_t* d_tiles = nullptr;
     unsigned long long* d_glob = nullptr;
     uint32_t* d_gbits = nullptr;
+    unsigned long long* d_sacc = nullptr;
     uint32_t gwords = 0;
     std::vector<uint8_t> used;          // host view: slot occupied
     // Admission stages a slot's descriptor and state in pinned memory and
@@ -528,6 +560,7 @@ BArgs bargs(gv_batch* b) {
     a.glob = b->d_glob;
     a.gbits = b->d_gbits;
     a.gwords = b->gwords;
+    a.sacc = b->d_sacc;
     a.tile = 64ull * b->tune.ipt;
     return a;
 }
@@ -644,6 +677,7 @@ int gv_batch_create(gv_ctx_t c, gv_graph_t g, uint32_t slots, uint64_t slot_byte
               hipMalloc(&b->d_tiles, 8 * (ntiles + 1)) == hipSuccess &&
               hipMalloc(&b->d_glob, 8 * 4) == hipSuccess &&
               hipMalloc(&b->d_gbits, 4ull * b->gwords * slots) == hipSuccess &&
+              hipMalloc(&b->d_sacc, 8ull * kBLines * kAccStride * slots) == hipSuccess &&
               hipHostMalloc((void**)&b->h_desc, sizeof(BDesc) * slots, hipHostMallocDefault) == hipSuccess &&
               hipHostMalloc((void**)&b->h_st, sizeof(BState) * slots, hipHostMallocDefault) == hipSuccess &&
               hipHostMalloc((void**)&b->h_zero, sizeof(BState), hipHostMallocDefault) == hipSuccess;
@@ -656,6 +690,7 @@ int gv_batch_create(gv_ctx_t c, gv_graph_t g, uint32_t slots, uint64_t slot_byte
     HIPCHK(hipMemsetAsync(b->d_st, 0, sizeof(BState) * slots, c->stream));
     HIPCHK(hipMemsetAsync(b->d_glob, 0, 8 * 4, c->stream));
     HIPCHK(hipMemsetAsync(b->d_gbits, 0, 4ull * b->gwords * slots, c->stream));
+    HIPCHK(hipMemsetAsync(b->d_sacc, 0, 8ull * kBLines * kAccStride * slots, c->stream));
     int rc = capture(b);
     if (rc != GV_OK) {
         gv_batch_destroy(b);
@@ -674,7 +709,7 @@ int gv_batch_destroy(gv_batch_t b) {
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
     void* ptrs[] = {b->d_desc, b->d_st, b->d_half[0], b->d_half[1], b->d_anc_beg, b->d_cnt,
-                    b->d_anc_i, b->d_csum, b->d_tiles, b->d_glob, b->d_gbits};
+                    b->d_anc_i, b->d_csum, b->d_tiles, b->d_glob, b->d_gbits, b->d_sacc};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (b->h_desc) (void)hipHostFree(b->h_desc);

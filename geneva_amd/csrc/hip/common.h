@@ -617,14 +617,56 @@ struct TileAcc {
     TailCache tc;
 };
 
+// Output reservation for the survivors of a wave's tile (returns the first
+// output row; any value when tot is 0).  Returning atomics on one output
+// counter are applied one after another, so a reservation per block round
+// instead of per wave tile matters: the waves of a block call
+// BlockReserve::operator() together, once per round each (tot 0 and out_n
+// null for a wave without an extension tile), and waves of the block that
+// write to the same counter share one atomic.
+struct BlockReserve {
+    uint64_t* sh;   // LDS, 3 * kWavesPerBlock words: tot, counter, base per wave
+    __device__ uint64_t operator()(unsigned long long* out_n, uint32_t tot) const {
+        const unsigned w = threadIdx.x >> 6;
+        if (__lane_id() == 0) {
+            sh[w] = tot;
+            sh[kWavesPerBlock + w] = tot ? (uint64_t)out_n : 0ull;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint32_t done = 0;
+            for (int i = 0; i < kWavesPerBlock; ++i) {
+                if (!sh[i] || ((done >> i) & 1u)) continue;
+                const uint64_t p = sh[kWavesPerBlock + i];
+                uint64_t sum = 0;
+                for (int k = i; k < kWavesPerBlock; ++k)
+                    if (sh[k] && sh[kWavesPerBlock + k] == p) sum += sh[k];
+                uint64_t base = atomicAdd(reinterpret_cast<unsigned long long*>(p),
+                                          (unsigned long long)sum);
+                for (int k = i; k < kWavesPerBlock; ++k)
+                    if (sh[k] && sh[kWavesPerBlock + k] == p) {
+                        sh[2 * kWavesPerBlock + k] = base;
+                        base += sh[k];
+                        done |= 1u << k;
+                    }
+            }
+        }
+        __syncthreads();
+        return sh[2 * kWavesPerBlock + w];
+    }
+};
+
 // lx: LDS of the wave, 64 * IPT + 1 entries: the candidate starts of the
 // tile's rows, so that a lane finds its row without global reads.
+// reserve: the BlockReserve of the block (extension only; every wave of the
+// block reaches it once per call of walk_tile).
 template <bool kCount, int IPT>
 __device__ __forceinline__ void walk_tile(const DevGraph& g, const gv_step_t& st,
                                           const gv_step_t& tail, bool has_tail, const TileIn& in,
                                           uint64_t t0, uint64_t t1, const TileOut& o,
                                           uint32_t* stage, uint64_t* lx, uint32_t* win,
-                                          uint32_t* gbits, TileAcc& acc) {
+                                          uint32_t* gbits, TileAcc& acc,
+                                          const BlockReserve& reserve) {
     const unsigned lane = __lane_id();
     const uint32_t L = in.L, W = L + 1;
     const uint64_t r_lo = wave_find_row(in.csum, in.row_lo, in.row_hi, t0);
@@ -828,10 +870,8 @@ __device__ __forceinline__ void walk_tile(const DevGraph& g, const gv_step_t& st
     uint32_t tot = 0;
 #pragma unroll
     for (int it = 0; it < IPT; ++it) tot += (uint32_t)__popcll(__ballot(okk[it]));
+    uint64_t base = reserve(o.out_n, tot);
     if (tot == 0) return;
-    uint64_t base = 0;
-    if (lane == 0) base = atomicAdd(o.out_n, (unsigned long long)tot);
-    base = __shfl(base, 0, 64);
     if (base + tot > o.cap) {
         if (lane == 0) atomicOr(o.overflow, 1ull);
         return;

@@ -487,8 +487,6 @@ __global__ __launch_bounds__(kBlock) void k_walk(WalkArgs a) {
     __shared__ uint32_t win[kWavesPerBlock][2 * kWin];
     const unsigned lane = lane_id();
     const unsigned wib = threadIdx.x >> 6;
-    const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
     uint64_t nrows = a.nrows, c0 = a.c0, c1 = a.c1;
     if (a.nrows_dev || a.stop) {
         nrows = dev_rows(a.nrows, a.nrows_dev, a.stop);
@@ -509,10 +507,22 @@ __global__ __launch_bounds__(kBlock) void k_walk(WalkArgs a) {
     in.row_hi = nrows;
     TileOut o{a.out, a.out_cap, a.out_n, a.overflow};
     TileAcc acc;
-    for (uint64_t t0 = c0 + wave * kTile; t0 < c1; t0 += nwaves * kTile) {
-        const uint64_t t1 = t0 + kTile < c1 ? t0 + kTile : c1;
-        walk_tile<kCount, kIPT>(a.g, a.st, a.tail, a.has_tail != 0, in, t0, t1, o,
-                                kCount ? nullptr : stage[wib], lx[wib], win[wib], a.gbits, acc);
+    __shared__ uint64_t resv[3 * kWavesPerBlock];
+    const BlockReserve reserve{resv};
+    // block rounds: wave wib of block b takes tile (round * blocks + b) * 4 + wib,
+    // and every wave of the block runs every round (the reservation is shared)
+    const uint64_t nb = gridDim.x;
+    for (uint64_t T0 = (uint64_t)blockIdx.x * kWavesPerBlock; c0 + T0 * kTile < c1;
+         T0 += nb * kWavesPerBlock) {
+        const uint64_t t0 = c0 + (T0 + wib) * kTile;
+        if (t0 < c1) {
+            const uint64_t t1 = t0 + kTile < c1 ? t0 + kTile : c1;
+            walk_tile<kCount, kIPT>(a.g, a.st, a.tail, a.has_tail != 0, in, t0, t1, o,
+                                    kCount ? nullptr : stage[wib], lx[wib], win[wib], a.gbits, acc,
+                                    reserve);
+        } else if (!kCount) {
+            reserve(nullptr, 0);
+        }
     }
     if (kProf && a.units) {
         const uint64_t w = wave_sum_u64(acc.bytes);
