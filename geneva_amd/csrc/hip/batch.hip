@@ -398,9 +398,7 @@ __global__ __launch_bounds__(1024) void k_b_tiles(BArgs a) {
 template <int IPT>
 __global__ __launch_bounds__(kBlock) void k_b_walk(BArgs a) {
     __shared__ uint64_t tile_end[kMaxSlots];
-    __shared__ uint32_t stage[kWavesPerBlock][64 * GV_MAXK];
-    __shared__ uint64_t lx[kWavesPerBlock][64 * IPT + 1];
-    __shared__ uint32_t win[kWavesPerBlock][2 * kWin];
+    __shared__ uint32_t wl[kWavesPerBlock][wave_lds_words<IPT>()];
     for (uint32_t i = threadIdx.x; i < a.nslots; i += blockDim.x) tile_end[i] = a.st[i].tile_end;
     __syncthreads();
     const uint64_t ntiles = a.glob[2];
@@ -456,12 +454,12 @@ __global__ __launch_bounds__(kBlock) void k_b_walk(BArgs a) {
         const gv_step_t& st = D.q.step[L];
         if (S.mode == 1) {
             TileOut o{slot_out(a, s, S), a.slot_uints / (L + 1), &a.st[s].out_rows, &a.st[s].overflow};
-            walk_tile<false, IPT>(a.g, st, st, false, in, t0, t1, o, stage[wib], lx[wib], win[wib], gb,
+            walk_tile<false, IPT>(a.g, st, st, false, in, t0, t1, o, wl[wib], wl[wib], wl[wib] + 64 * IPT + 1, gb,
                                   acc, reserve);
         } else {
             TileOut o{nullptr, 0, nullptr, nullptr};
             const gv_step_t& tl = D.q.step[L + 1 < GV_MAXK ? L + 1 : GV_MAXK - 1];
-            walk_tile<true, IPT>(a.g, st, tl, S.mode == 3, in, t0, t1, o, nullptr, lx[wib], win[wib], gb,
+            walk_tile<true, IPT>(a.g, st, tl, S.mode == 3, in, t0, t1, o, nullptr, wl[wib], wl[wib] + 64 * IPT + 1, gb,
                                  acc, reserve);
             reserve(nullptr, 0);
         }

@@ -626,7 +626,7 @@ struct TileAcc {
 // write to the same counter share one atomic.
 struct BlockReserve {
     uint64_t* sh;   // LDS, 3 * kWavesPerBlock words: tot, counter, base per wave
-    __device__ uint64_t operator()(unsigned long long* out_n, uint32_t tot) const {
+    __device__ __forceinline__ uint64_t operator()(unsigned long long* out_n, uint32_t tot) const {
         const unsigned w = threadIdx.x >> 6;
         if (__lane_id() == 0) {
             sh[w] = tot;
@@ -656,15 +656,25 @@ struct BlockReserve {
     }
 };
 
-// lx: LDS of the wave, 64 * IPT + 1 entries: the candidate starts of the
-// tile's rows, so that a lane finds its row without global reads.
+// LDS words per wave of a walk with IPT candidates per lane: stage placed over
+// lx and win (see walk_tile).
+template <int IPT>
+constexpr uint32_t wave_lds_words() {
+    return 64u * IPT + 1u + 2u * kWin > 64u * GV_MAXK ? 64u * IPT + 1u + 2u * kWin : 64u * GV_MAXK;
+}
+
+// LDS of a wave: lx (64 * IPT + 1 words), the candidate starts of the tile's
+// rows relative to t0, so that a lane finds its row without global reads; win
+// (2 * kWin words), windows of the other back lists; stage (64 * (L + 1)
+// words), survivor rows on their way out.  stage is used only after lx and win
+// are done with, so a kernel may place it over them (wave_lds_words).
 // reserve: the BlockReserve of the block (extension only; every wave of the
 // block reaches it once per call of walk_tile).
 template <bool kCount, int IPT>
 __device__ __forceinline__ void walk_tile(const DevGraph& g, const gv_step_t& st,
                                           const gv_step_t& tail, bool has_tail, const TileIn& in,
                                           uint64_t t0, uint64_t t1, const TileOut& o,
-                                          uint32_t* stage, uint64_t* lx, uint32_t* win,
+                                          uint32_t* stage, uint32_t* lx, uint32_t* win,
                                           uint32_t* gbits, TileAcc& acc,
                                           const BlockReserve& reserve) {
     const unsigned lane = __lane_id();
@@ -782,12 +792,19 @@ __device__ __forceinline__ void walk_tile(const DevGraph& g, const gv_step_t& st
         if (nob > 0) __builtin_amdgcn_wave_barrier();
     } else {
         // rows r_lo..r_hi hold the tile: when they fit, stage their candidate
-        // starts in LDS (lx[i] = first candidate of row r_lo + i, lx[nr] = end)
+        // starts in LDS relative to t0 (lx[i] = start of row r_lo + i, lx[nr]
+        // = end of row r_hi).  Rows after r_lo start inside the tile, so only
+        // lx[0] (clamped to 0) and lx[nr] (clamped to 2^32 - 1) are not exact,
+        // which the search over c - t0 < 64 * IPT does not see; row r_lo's
+        // own start is first0.
         const uint64_t nr = r_hi - r_lo + 1;
         const bool staged = nr <= 64ull * IPT;
+        const uint64_t first0 = in.csum[r_lo] - in.cnt[r_lo];
         if (staged) {
-            for (uint32_t i = lane; i <= nr; i += 64)
-                lx[i] = i < nr ? in.csum[r_lo + i] - in.cnt[r_lo + i] : in.csum[r_hi];
+            for (uint32_t i = lane; i <= nr; i += 64) {
+                const uint64_t x = i < nr ? in.csum[r_lo + i] - in.cnt[r_lo + i] : in.csum[r_hi];
+                lx[i] = x <= t0 ? 0u : (x - t0 > 0xffffffffull ? 0xffffffffu : (uint32_t)(x - t0));
+            }
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -801,11 +818,12 @@ __device__ __forceinline__ void walk_tile(const DevGraph& g, const gv_step_t& st
             if (c < t1) {
                 uint64_t first;
                 if (staged) {
-                    // first i with lx[i + 1] > c
+                    // first i with lx[i + 1] > c - t0
+                    const uint32_t cr = (uint32_t)(c - t0);
                     uint32_t lo = 0, n = (uint32_t)nr;
                     while (n > 0) {
                         const uint32_t h = n >> 1;
-                        if (lx[lo + h + 1] <= c) {
+                        if (lx[lo + h + 1] <= cr) {
                             lo += h + 1;
                             n -= h + 1;
                         } else {
@@ -813,7 +831,7 @@ __device__ __forceinline__ void walk_tile(const DevGraph& g, const gv_step_t& st
                         }
                     }
                     r = r_lo + lo;
-                    first = lx[lo];
+                    first = lo == 0 ? first0 : t0 + lx[lo];
                 } else {
                     r = find_row(in.csum, r_lo, r_hi + 1, c);
                     first = in.csum[r] - in.cnt[r];

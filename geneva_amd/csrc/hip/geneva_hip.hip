@@ -482,9 +482,7 @@ struct WalkArgs {
 
 template <bool kCount, bool kProf>
 __global__ __launch_bounds__(kBlock) void k_walk(WalkArgs a) {
-    __shared__ uint32_t stage[kWavesPerBlock][64 * GV_MAXK];
-    __shared__ uint64_t lx[kWavesPerBlock][kTile + 1];
-    __shared__ uint32_t win[kWavesPerBlock][2 * kWin];
+    __shared__ uint32_t wl[kWavesPerBlock][wave_lds_words<kIPT>()];
     const unsigned lane = lane_id();
     const unsigned wib = threadIdx.x >> 6;
     uint64_t nrows = a.nrows, c0 = a.c0, c1 = a.c1;
@@ -518,7 +516,7 @@ __global__ __launch_bounds__(kBlock) void k_walk(WalkArgs a) {
         if (t0 < c1) {
             const uint64_t t1 = t0 + kTile < c1 ? t0 + kTile : c1;
             walk_tile<kCount, kIPT>(a.g, a.st, a.tail, a.has_tail != 0, in, t0, t1, o,
-                                    kCount ? nullptr : stage[wib], lx[wib], win[wib], a.gbits, acc,
+                                    kCount ? nullptr : wl[wib], wl[wib], wl[wib] + kTile + 1, a.gbits, acc,
                                     reserve);
         } else if (!kCount) {
             reserve(nullptr, 0);
