@@ -32,7 +32,7 @@ namespace {
 
 constexpr int kBlock = 256;             // 4 wave64 per workgroup
 constexpr int kWavesPerBlock = kBlock / 64;
-constexpr int kIPT = 8;                 // candidates per lane per tile
+constexpr int kIPT = 4;                 // candidates per lane per tile
 constexpr uint64_t kTile = 64 * kIPT;   // candidates per wave tile
 constexpr int kScanItems = 16;          // per thread in the scan
 constexpr int kSizeRows = 4;            // rows per thread in a sizing pass

@@ -480,8 +480,11 @@ struct WalkArgs {
 
 
 
+// Eight waves per SIMD (at most 64 VGPRs): the walk waits on dependent loads,
+// and more waves in flight hide more of that latency than the few registers
+// the limit spills cost.
 template <bool kCount, bool kProf>
-__global__ __launch_bounds__(kBlock) void k_walk(WalkArgs a) {
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_walk(WalkArgs a) {
     __shared__ uint32_t wl[kWavesPerBlock][wave_lds_words<kIPT>()];
     const unsigned lane = lane_id();
     const unsigned wib = threadIdx.x >> 6;
