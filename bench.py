@@ -148,6 +148,95 @@ def batched_and_served(args):
     return out, out_s
 
 
+# kernel (lean variant, as rocprofv3 names it) -> the kind bench.py reports
+PMC_KINDS = [("k_walk<true, false>", "k_walk<count>"), ("k_walk<false, false>", "k_walk<extend>"),
+             ("k_size<false>", "k_size"), ("k_count_rows<false>", "k_count_rows"),
+             ("k_count_leaves<false>", "k_count_rows"), ("k_count_chain<false>", "k_count_rows"),
+             ("k_chain_mark", "k_count_rows"), ("k_chain_weights", "k_count_rows"),
+             ("k_scan_reduce", "k_scan"), ("k_scan_tiles", "k_scan"), ("k_scan_final", "k_scan")]
+
+
+def pmc_summary(paths, workload, cal_mib):
+    """HBM bytes per launch per kernel kind from two rocprofv3 --pmc runs of the
+    bench (FETCH_SIZE, then WRITE_SIZE; KiB per dispatch) and the same two
+    counters over k_copy_back moving a known number of bytes each way
+    (--copy-calibration cal_mib): each counter is scaled by bytes moved / KiB
+    read for the copy.  Writes profiles/traffic_workload<N>.json and prints the
+    table."""
+    import csv
+    import re
+    fetch_csv, write_csv, cal_fetch_csv, cal_write_csv = paths
+
+    def per_kernel(path, counter):
+        acc = {}
+        for r in csv.DictReader(open(path)):
+            if r["Counter_Name"] != counter:
+                continue
+            m = re.search(r"(k_\w+(?:<[^>]*>)?)\(", r["Kernel_Name"].replace("(anonymous namespace)::", ""))
+            if m:
+                d = acc.setdefault(m.group(1), {})
+                d[r["Dispatch_Id"]] = d.get(r["Dispatch_Id"], 0.0) + float(r["Counter_Value"])
+        return acc
+
+    def mean(d):
+        return sum(d.values()) / len(d)
+
+    moved = 4 * (((cal_mib << 20) // 4) // 2)   # bytes each way per k_copy_back launch
+    fetch_scale = moved / (mean(per_kernel(cal_fetch_csv, "FETCH_SIZE")["k_copy_back"]) * 1024)
+    write_scale = moved / (mean(per_kernel(cal_write_csv, "WRITE_SIZE")["k_copy_back"]) * 1024)
+    f, w = per_kernel(fetch_csv, "FETCH_SIZE"), per_kernel(write_csv, "WRITE_SIZE")
+    out = {"fetch_scale": round(fetch_scale, 4), "write_scale": round(write_scale, 4),
+           "calibration": f"k_copy_back: {moved} B read + {moved} B written per launch"}
+    kinds = {}
+    for kern, kind in PMC_KINDS:
+        if kern not in f or kern not in w:
+            continue
+        k = kinds.setdefault(kind, {"kernels": [], "fetch_kib": 0.0, "write_kib": 0.0, "launches": 0})
+        k["kernels"].append(f"{kern} x{len(f[kern])}")
+        k["fetch_kib"] += sum(f[kern].values())
+        k["write_kib"] += sum(w[kern].values())
+        # a launch of the scan kind is its three kernels; the row counters are
+        # counted by their counting kernel (the chain's marking and weights
+        # belong to the chain count)
+        if kern != "k_scan_tiles" and kern != "k_scan_final" and not kern.startswith("k_chain_"):
+            k["launches"] += len(f[kern])
+    print("| Kind | Kernels (dispatches) | FETCH_SIZE KiB / launch | WRITE_SIZE KiB / launch | HBM bytes / launch |")
+    print("|---|---|---|---|---|")
+    for kind, k in kinds.items():
+        n = max(k["launches"], 1)
+        fk, wk = k["fetch_kib"] / n, k["write_kib"] / n
+        hb = int(fk * 1024 * fetch_scale + wk * 1024 * write_scale)
+        out[kind] = {"kernels": k["kernels"], "launches": n, "fetch_kib_per_launch": round(fk, 1),
+                     "write_kib_per_launch": round(wk, 1), "hbm_bytes_per_launch": hb}
+        print(f"| `{kind}` | {', '.join(k['kernels'])} | {fk:.1f} | {wk:.1f} | {hb:,} |")
+    path = os.path.join(ROOT, "profiles", f"traffic_workload{workload}.json")
+    with open(path, "w") as fh:
+        json.dump(out, fh, indent=1)
+    print(f"scales: fetch {fetch_scale:.4f}, write {write_scale:.4f}; wrote {path}")
+
+
+def pmc_table(path):
+    """Per-kernel means of every counter in one rocprofv3 --pmc CSV (markdown)."""
+    import csv
+    import re
+    acc, dur = {}, {}
+    for r in csv.DictReader(open(path)):
+        name = r["Kernel_Name"].replace("(anonymous namespace)::", "")
+        m = re.search(r"(k_\w+(?:<[^>]*>)?)\(", name)
+        name = m.group(1) if m else name[:40]
+        d = acc.setdefault(name, {}).setdefault(r["Counter_Name"], {})
+        d[r["Dispatch_Id"]] = d.get(r["Dispatch_Id"], 0.0) + float(r["Counter_Value"])
+        dur.setdefault(name, {})[r["Dispatch_Id"]] = int(r["End_Timestamp"]) - int(r["Start_Timestamp"])
+    cols = sorted({c for k in acc.values() for c in k})
+    print("| Kernel | Dispatches | us / dispatch | " + " | ".join(cols) + " |")
+    print("|---|---|---|" + "---|" * len(cols))
+    for name in sorted(acc, key=lambda k: -sum(dur[k].values())):
+        n = len(dur[name])
+        vals = [sum(acc[name].get(c, {}).values()) / n for c in cols]
+        print(f"| `{name}` | {n} | {sum(dur[name].values()) / n / 1e3:.1f} | " +
+              " | ".join(f"{v:,.0f}" for v in vals) + " |")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -167,9 +256,18 @@ def main():
     ap.add_argument("--slot-mib", type=int, default=256, help="MiB per batch slot table")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run the copy kernel on MIB mebibytes (PMC scaling)")
+    ap.add_argument("--pmc-summary", nargs=4, metavar=("FETCH_CSV", "WRITE_CSV", "CAL_FETCH_CSV", "CAL_WRITE_CSV"),
+                    help="only summarise rocprofv3 FETCH_SIZE / WRITE_SIZE runs of the bench and of "
+                         "--copy-calibration into profiles/traffic_workload<N>.json (no GPU)")
+    ap.add_argument("--cal-mib", type=int, default=512, help="MIB of the calibration run (--pmc-summary)")
+    ap.add_argument("--pmc-table", metavar="CSV", help="only print per-kernel means of a rocprofv3 --pmc CSV")
     args = ap.parse_args()
     if args.copy_calibration:
         return copy_calibration(args.copy_calibration)
+    if args.pmc_summary:
+        return pmc_summary(args.pmc_summary, args.workload, args.cal_mib)
+    if args.pmc_table:
+        return pmc_table(args.pmc_table)
     if args.gpus != 1:
         log("bench.py measures one GPU: the embedding-extension path here is built and measured "
             "on a single MI355X (multi-GPU sharding is a design note, not part of this build).")

@@ -223,16 +223,20 @@ __global__ __launch_bounds__(kBlock) void k_chain_mark(const uint32_t* rows, uin
         need[rows[r * L + acol]] = epoch;
 }
 
+constexpr uint32_t kChainVertsPerWave = 4;
+
 __global__ __launch_bounds__(kBlock) void k_chain_weights(ChainArgs a) {
     const unsigned lane = lane_id();
     const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
-    // 64 vertices per wave: lanes test the marks, the wave sums each marked
-    // vertex's segment together
-    for (uint64_t base = wave * 64; base < a.nverts; base += nwaves * 64) {
+    // kChainVertsPerWave vertices per wave: lanes test their marks, then the
+    // wave sums each marked vertex's segment together (few vertices per wave,
+    // so that many waves hide the dependent loads)
+    for (uint64_t base = wave * kChainVertsPerWave; base < a.nverts; base += nwaves * kChainVertsPerWave) {
         const uint64_t i = base + lane;
-        const uint32_t yl = i < a.nverts ? a.verts[i] : 0u;
-        uint64_t m = __ballot(i < a.nverts && a.need[yl] == a.epoch);
+        const bool in = lane < kChainVertsPerWave && i < a.nverts;
+        const uint32_t yl = in ? a.verts[i] : 0u;
+        uint64_t m = __ballot(in && a.need[yl] == a.epoch);
         while (m) {
             const unsigned j = __builtin_ctzll(m);
             m &= m - 1;
@@ -1326,7 +1330,8 @@ int gv_mine_levels(gv_ctx_t c, gv_graph_t g, const gv_query_t* q, const uint32_t
                                c->stream, rows, cap, ndev, stop, w, acol, c->chainneed, c->chain_epoch);
             if (ca.nverts)
                 hipLaunchKernelGGL(k_chain_weights,
-                                   dim3(grid_for(ca.nverts, kBlock, c->num_cu * 16)),
+                                   dim3(grid_for(ca.nverts * (64 / kChainVertsPerWave), kBlock,
+                                                 c->num_cu * 16)),
                                    dim3(kBlock), 0, c->stream, ca);
             if (c->prof_bytes)
                 hipLaunchKernelGGL(k_count_chain<true>, dim3(grid_for(cap, kBlock, c->num_cu * 16)),
