@@ -153,7 +153,7 @@ PMC_KINDS = [("k_walk<true, false>", "k_walk<count>"), ("k_walk<false, false>", 
              ("k_size<false>", "k_size"), ("k_count_rows<false>", "k_count_rows"),
              ("k_count_leaves<false>", "k_count_rows"), ("k_count_chain<false>", "k_count_rows"),
              ("k_chain_mark", "k_count_rows"), ("k_chain_weights", "k_count_rows"),
-             ("k_scan_reduce", "k_scan"), ("k_scan_tiles", "k_scan"), ("k_scan_final", "k_scan")]
+             ("k_scan_tiles", "k_scan"), ("k_scan_final", "k_scan")]
 
 
 def pmc_summary(paths, workload, cal_mib):
@@ -195,10 +195,10 @@ def pmc_summary(paths, workload, cal_mib):
         k["kernels"].append(f"{kern} x{len(f[kern])}")
         k["fetch_kib"] += sum(f[kern].values())
         k["write_kib"] += sum(w[kern].values())
-        # a launch of the scan kind is its three kernels; the row counters are
+        # a launch of the scan kind is its two kernels; the row counters are
         # counted by their counting kernel (the chain's marking and weights
         # belong to the chain count)
-        if kern != "k_scan_tiles" and kern != "k_scan_final" and not kern.startswith("k_chain_"):
+        if kern != "k_scan_tiles" and not kern.startswith("k_chain_"):
             k["launches"] += len(f[kern])
     print("| Kind | Kernels (dispatches) | FETCH_SIZE KiB / launch | WRITE_SIZE KiB / launch | HBM bytes / launch |")
     print("|---|---|---|---|---|")

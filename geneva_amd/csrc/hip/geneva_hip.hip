@@ -44,6 +44,7 @@ struct SizeArgs {
     const unsigned long long* nrows_dev;
     const unsigned long long* stop;
     int64_t start_pos;         // >= 0: rows are the start table at this lverts position
+    uint64_t* tile_sums;       // count sum per sizing tile (the scan's first pass)
     gv_step_t st;
 };
 
@@ -74,13 +75,27 @@ __global__ __launch_bounds__(kBlock) void k_size(SizeArgs a) {
         }
         uint32_t beg[kSizeRows], len[kSizeRows], cnt[kSizeRows], ai[kSizeRows];
         size_rows<kSizeRows>(a.g, a.st, a.rows, a.L, ri, ok, beg, len, cnt, ai, local, rq, a.start_pos);
+        uint64_t ts = 0;
 #pragma unroll
         for (int k = 0; k < kSizeRows; ++k) {
             if (!ok[k]) continue;
             a.anc_beg[ri[k]] = beg[k];
             a.cnt[ri[k]] = cnt[k];
             a.anc_i[ri[k]] = (uint8_t)ai[k];
+            ts += cnt[k];
         }
+        // the tile's count sum: the scan's first pass, done here so that the
+        // counts are not read back for it
+        __shared__ uint64_t tsum[kWavesPerBlock];
+        ts = wave_sum_u64(ts);
+        if (lane_id() == 0) tsum[threadIdx.x >> 6] = ts;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint64_t x = 0;
+            for (int i = 0; i < kWavesPerBlock; ++i) x += tsum[i];
+            a.tile_sums[t] = x;
+        }
+        __syncthreads();
     }
     if (kProf && a.units) {
         const uint64_t w = wave_sum_u64(local);
@@ -349,35 +364,17 @@ __device__ __forceinline__ uint64_t block_exclusive_scan(uint64_t v, uint64_t* w
     return off + x - v;
 }
 
-// The scan kernels stride over tiles of kScanTile counts; n may come from
-// device memory (n_dev, clamped to n; nothing when *stop is set).
-__global__ __launch_bounds__(kBlock) void k_scan_reduce(const uint32_t* in, uint64_t n_max,
-                                                        const unsigned long long* n_dev,
-                                                        const unsigned long long* stop,
-                                                        uint64_t* tile_sums) {
-    __shared__ uint64_t wsum[kWavesPerBlock];
-    const uint64_t n = dev_rows(n_max, n_dev, stop);
-    const uint64_t ntiles = (n + kScanTile - 1) / kScanTile;
-    for (uint64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
-        const uint64_t base = t * kScanTile;
-        uint64_t s = 0;
-        for (int i = 0; i < kScanItems; ++i) {
-            uint64_t idx = base + (uint64_t)i * kBlock + threadIdx.x;
-            if (idx < n) s += in[idx];
-        }
-        s = wave_sum_u64(s);
-        if (lane_id() == 0) wsum[threadIdx.x >> 6] = s;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            uint64_t x = 0;
-            for (int i = 0; i < kWavesPerBlock; ++i) x += wsum[i];
-            tile_sums[t] = x;
-        }
-        __syncthreads();
-    }
-}
+// The scan of a sizing launch's counts into inclusive uint64 sums has its
+// first pass in k_size (one count sum per kSizeTile rows); n may come from
+// device memory (n_dev, clamped to n_max; nothing when *stop is set).
+constexpr int kFinalItems = kSizeRows;                    // counts per thread
+constexpr int kFinalTile = kBlock * kFinalItems;          // rows per tile
+constexpr int kFinalPadded = kFinalTile + kFinalTile / 16;
+constexpr int kTileItems = 16;                            // tile sums per thread per pass
+static_assert((uint64_t)kFinalTile == kSizeTile, "scan tiles are sizing tiles");
 
-// exclusive scan of tile sums in place, one workgroup
+// exclusive scan of the tile sums in place, one workgroup (kTileItems
+// consecutive sums per thread per pass)
 __global__ __launch_bounds__(kBlock) void k_scan_tiles(uint64_t* tile_sums, uint64_t n_max,
                                                        const unsigned long long* n_dev,
                                                        const unsigned long long* stop,
@@ -385,19 +382,31 @@ __global__ __launch_bounds__(kBlock) void k_scan_tiles(uint64_t* tile_sums, uint
                                                        unsigned long long* req) {
     __shared__ uint64_t wsum[kWavesPerBlock];
     const uint64_t n = dev_rows(n_max, n_dev, stop);
-    const uint64_t ntiles = (n + kScanTile - 1) / kScanTile;
+    const uint64_t ntiles = (n + kFinalTile - 1) / kFinalTile;
     if (threadIdx.x == 0 && n) {
         if (units) atomicAdd(units, (unsigned long long)n);
-        // counts read twice, sums written; tile sums read and written
-        if (req) atomicAdd(req, (unsigned long long)(16ull * n + 24ull * ntiles));
+        // counts read, sums written; a tile sum written (k_size), read and
+        // written here, read by k_scan_final
+        if (req) atomicAdd(req, (unsigned long long)(12ull * n + 32ull * ntiles));
     }
     uint64_t carry = 0;
-    for (uint64_t b = 0; b < ntiles; b += kBlock) {
-        uint64_t idx = b + threadIdx.x;
-        uint64_t v = idx < ntiles ? tile_sums[idx] : 0;
+    for (uint64_t b = 0; b < ntiles; b += (uint64_t)kBlock * kTileItems) {
+        const uint64_t first = b + (uint64_t)threadIdx.x * kTileItems;
+        uint64_t loc[kTileItems];
+        uint64_t s = 0;
+#pragma unroll
+        for (int i = 0; i < kTileItems; ++i) {
+            const uint64_t idx = first + i;
+            loc[i] = s;
+            s += idx < ntiles ? tile_sums[idx] : 0;
+        }
         uint64_t tot;
-        uint64_t ex = block_exclusive_scan(v, wsum, tot);
-        if (idx < ntiles) tile_sums[idx] = carry + ex;
+        const uint64_t ex = carry + block_exclusive_scan(s, wsum, tot);
+#pragma unroll
+        for (int i = 0; i < kTileItems; ++i) {
+            const uint64_t idx = first + i;
+            if (idx < ntiles) tile_sums[idx] = ex + loc[i];
+        }
         carry += tot;
         __syncthreads();
     }
@@ -409,43 +418,43 @@ __global__ __launch_bounds__(kBlock) void k_scan_final(const uint32_t* in, uint6
                                                        const uint64_t* tile_off, uint64_t* out) {
     // the counts and the sums share one padded tile: the counts are all read
     // (and the block scan has synchronised) before the first sum is written
-    __shared__ uint64_t outv[kScanTilePadded / 2];
+    __shared__ uint64_t outv[kFinalPadded / 2];
     uint32_t* vals = reinterpret_cast<uint32_t*>(outv);
     __shared__ uint64_t wsum[kWavesPerBlock];
     const uint64_t n = dev_rows(n_max, n_dev, stop);
-    const uint64_t ntiles = (n + kScanTile - 1) / kScanTile;
+    const uint64_t ntiles = (n + kFinalTile - 1) / kFinalTile;
     for (uint64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
-    const uint64_t base = t * kScanTile;
-    for (int i = 0; i < kScanItems; ++i) {
-        uint64_t idx = base + (uint64_t)i * kBlock + threadIdx.x;
-        vals[lds_pad16(i * kBlock + threadIdx.x)] = idx < n ? in[idx] : 0u;
-    }
-    __syncthreads();
-    uint64_t loc[kScanItems];
-    uint64_t s = 0;
-#pragma unroll
-    for (int i = 0; i < kScanItems; ++i) {
-        s += vals[lds_pad16(threadIdx.x * kScanItems + i)];
-        loc[i] = s;
-    }
-    uint64_t tot;
-    uint64_t ex = block_exclusive_scan(s, wsum, tot) + tile_off[t];
-    // write back through LDS in two halves to keep coalesced stores
-    for (int half = 0; half < 2; ++half) {
-        __syncthreads();
-        const int lo = half * (kScanTile / 2);
-#pragma unroll
-        for (int i = 0; i < kScanItems; ++i) {
-            int p = threadIdx.x * kScanItems + i - lo;
-            if (p >= 0 && p < kScanTile / 2) outv[lds_pad16(p)] = ex + loc[i];
+        const uint64_t base = t * kFinalTile;
+        for (int i = 0; i < kFinalItems; ++i) {
+            const uint64_t idx = base + (uint64_t)i * kBlock + threadIdx.x;
+            vals[lds_pad16(i * kBlock + threadIdx.x)] = idx < n ? in[idx] : 0u;
         }
         __syncthreads();
-        for (int i = threadIdx.x; i < kScanTile / 2; i += kBlock) {
-            uint64_t idx = base + lo + i;
-            if (idx < n) out[idx] = outv[lds_pad16(i)];
+        uint64_t loc[kFinalItems];
+        uint64_t s = 0;
+#pragma unroll
+        for (int i = 0; i < kFinalItems; ++i) {
+            s += vals[lds_pad16(threadIdx.x * kFinalItems + i)];
+            loc[i] = s;
         }
-    }
-    __syncthreads();
+        uint64_t tot;
+        const uint64_t ex = block_exclusive_scan(s, wsum, tot) + tile_off[t];
+        // write back through LDS in two halves to keep coalesced stores
+        for (int half = 0; half < 2; ++half) {
+            __syncthreads();
+            const int lo = half * (kFinalTile / 2);
+#pragma unroll
+            for (int i = 0; i < kFinalItems; ++i) {
+                const int p = threadIdx.x * kFinalItems + i - lo;
+                if (p >= 0 && p < kFinalTile / 2) outv[lds_pad16(p)] = ex + loc[i];
+            }
+            __syncthreads();
+            for (int i = threadIdx.x; i < kFinalTile / 2; i += kBlock) {
+                const uint64_t idx = base + lo + i;
+                if (idx < n) out[idx] = outv[lds_pad16(i)];
+            }
+        }
+        __syncthreads();
     }
 }
 
@@ -603,8 +612,9 @@ int ensure_gbits(gv_ctx* c, const gv_graph* g) {
     return GV_OK;
 }
 
+// tile sums of a sizing launch over n rows
 int ensure_tiles(gv_ctx* c, uint64_t n) {
-    const uint64_t ntiles = (n + kScanTile - 1) / kScanTile;
+    const uint64_t ntiles = (n + kSizeTile - 1) / kSizeTile;
     if (ntiles > c->tiles_cap) {
         if (c->tiles) HIPCHK(hipFree(c->tiles));
         c->tiles = nullptr;
@@ -619,16 +629,13 @@ int ensure_tiles(gv_ctx* c, uint64_t n) {
 }
 
 // Inclusive scan of n counts (n_max when n_dev is null, else min(*n_dev,
-// n_max) read on the device; nothing when *stop is set).
+// n_max) read on the device; nothing when *stop is set) that the sizing launch
+// just before wrote, with their tile sums in c->tiles.
 int scan_counts(gv_ctx* c, const uint32_t* cnt, uint64_t n_max, const unsigned long long* n_dev,
                 const unsigned long long* stop, uint64_t* csum) {
-    int rc = ensure_tiles(c, n_max);
-    if (rc != GV_OK) return rc;
-    const uint64_t ntiles = (n_max + kScanTile - 1) / kScanTile;
-    const unsigned grid = (unsigned)std::min<uint64_t>(std::max<uint64_t>(ntiles, 1), 8ull * c->num_cu);
+    const uint64_t ntiles = (n_max + kSizeTile - 1) / kSizeTile;
+    const unsigned grid = (unsigned)std::min<uint64_t>(std::max<uint64_t>(ntiles, 1), 16ull * c->num_cu);
     ProfScope ps(c, kKindScan);
-    hipLaunchKernelGGL(k_scan_reduce, dim3(grid), dim3(kBlock), 0, c->stream, cnt, n_max, n_dev, stop,
-                       c->tiles);
     hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(kBlock), 0, c->stream, c->tiles, n_max, n_dev,
                        stop, units_ptr(c, kKindScan), req_ptr(c, kKindScan));
     hipLaunchKernelGGL(k_scan_final, dim3(grid), dim3(kBlock), 0, c->stream, cnt, n_max, n_dev, stop,
@@ -1035,6 +1042,9 @@ int gv_size_rows(gv_ctx_t c, gv_graph_t g, const gv_step_t* st, const uint32_t* 
     a.st = *st;
     a.start_pos = start_pos_of(g, rows, L);
     {
+        int rc0 = ensure_tiles(c, nrows);
+        if (rc0 != GV_OK) return rc0;
+        a.tile_sums = c->tiles;
         ProfScope ps(c, kKindSize);
         if (c->prof_bytes)
             hipLaunchKernelGGL(k_size<true>, dim3(grid_for(nrows, kSizeTile, c->num_cu * 16)),
@@ -1387,6 +1397,9 @@ int gv_mine_levels(gv_ctx_t c, gv_graph_t g, const gv_query_t* q, const uint32_t
             a.units = units_ptr(c, kKindSize);
             a.st = st;
             a.start_pos = start_pos_of(g, rows, w);
+            int rc0 = ensure_tiles(c, cap);
+            if (rc0 != GV_OK) return rc0;
+            a.tile_sums = c->tiles;
             ProfScope ps(c, kKindSize);
             if (c->prof_bytes)
                 hipLaunchKernelGGL(k_size<true>, dim3(grid_for(cap, kSizeTile, c->num_cu * 16)),
