@@ -155,18 +155,20 @@ __device__ __forceinline__ uint32_t wave_lower_bound(const uint32_t* a, uint32_t
     return lo;
 }
 
+// x in the sorted w[0, n0): branchless lower bound (Khuong & Morin, "Array
+// layouts for comparison-based searching", 2017), whose halving depends on n0
+// only, so all lanes of a wave searching one window run the same rounds; at
+// the end the answer is base or base + 1.
 __device__ __forceinline__ bool lds_contains(const uint32_t* w, uint32_t n0, uint32_t x) {
-    uint32_t lo = 0, n = n0;
-    while (n > 0) {
-        const uint32_t h = n >> 1;
-        if (w[lo + h] < x) {
-            lo += h + 1;
-            n -= h + 1;
-        } else {
-            n = h;
-        }
+    if (n0 == 0) return false;
+    uint32_t base = 0;
+    for (uint32_t len = n0; len > 1;) {
+        const uint32_t half = len >> 1;
+        base = w[base + half] < x ? base + half : base;
+        len -= half;
     }
-    return lo < n0 && w[lo] == x;
+    const uint32_t b1 = base + 1 < n0 ? base + 1 : base;
+    return w[base] == x || w[b1] == x;
 }
 
 // Count of the last vertex when it has a single back edge (step `t`, rows of
@@ -757,13 +759,17 @@ __device__ __forceinline__ void walk_tile(const DevGraph& g, const gv_step_t& st
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         }
+        // 32-bit offsets inside the tile (t1 - t0 <= 64 IPT) and the row
+        const uint32_t tn = (uint32_t)(t1 - t0);
+        const uint32_t o0 = (uint32_t)(t0 - first);
+        const uint32_t* anc = g.nbr + abeg + o0;
 #pragma unroll
         for (int it = 0; it < IPT; ++it) {
-            const uint64_t c = t0 + (uint64_t)it * 64 + lane;
+            const uint32_t k = (uint32_t)it * 64u + lane;
             uint32_t v = 0;
             bool ok = false;
-            if (c < t1) {
-                v = g.nbr[abeg + (uint32_t)(c - first)];
+            if (k < tn) {
+                v = anc[k];
                 acc.rq += 4;
                 ok = v != cv0 && v != cv1 && v != cv2;
                 if (ok && nob > 0) {
@@ -786,7 +792,7 @@ __device__ __forceinline__ void walk_tile(const DevGraph& g, const gv_step_t& st
             okk[it] = ok;
             vv[it] = v;
             rr[it] = r;
-            six[it] = shared ? (uint32_t)(c - first) : ~0u;
+            six[it] = shared ? o0 + (uint32_t)it * 64u + lane : ~0u;
         }
         // the windows are rewritten by the next tile
         if (nob > 0) __builtin_amdgcn_wave_barrier();
