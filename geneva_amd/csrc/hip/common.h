@@ -155,20 +155,49 @@ __device__ __forceinline__ uint32_t wave_lower_bound(const uint32_t* a, uint32_t
     return lo;
 }
 
-// x in the sorted w[0, n0): branchless lower bound (Khuong & Morin, "Array
-// layouts for comparison-based searching", 2017), whose halving depends on n0
-// only, so all lanes of a wave searching one window run the same rounds; at
-// the end the answer is base or base + 1.
-__device__ __forceinline__ bool lds_contains(const uint32_t* w, uint32_t n0, uint32_t x) {
-    if (n0 == 0) return false;
-    uint32_t base = 0;
-    for (uint32_t len = n0; len > 1;) {
+
+// Multi-row tiles of at most kRowStage rows stage kRowWords words per row in
+// the window space of the walk (2 * kWin words).
+constexpr uint32_t kRowWords = 9;
+constexpr uint32_t kRowStage = 2 * kWin / kRowWords;
+
+// First index in [lo, n) of the sorted w (LDS) with w[i] >= x (n if none).
+__device__ __forceinline__ uint32_t lds_lower_bound(const uint32_t* w, uint32_t lo, uint32_t n,
+                                                    uint32_t x) {
+    uint32_t len = n - lo;
+    while (len > 0) {
         const uint32_t half = len >> 1;
-        base = w[base + half] < x ? base + half : base;
-        len -= half;
+        if (w[lo + half] < x) {
+            lo += half + 1;
+            len -= half + 1;
+        } else {
+            len = half;
+        }
     }
-    const uint32_t b1 = base + 1 < n0 ? base + 1 : base;
-    return w[base] == x || w[b1] == x;
+    return lo;
+}
+
+// ok[it] &= (v[it] is in the sorted window w[0, n)), for a lane's IPT candidates
+// in ascending order (those with it < nv): the first is found by a binary
+// search, each later one by advancing a cursor from the previous position (a
+// few steps, then a search of the rest), a merge of the lane's run of
+// candidates with the window.
+template <int IPT>
+__device__ __forceinline__ void window_member(const uint32_t* w, uint32_t n, const uint32_t (&v)[IPT],
+                                              uint32_t nv, bool (&ok)[IPT]) {
+    uint32_t p = 0;
+#pragma unroll
+    for (int it = 0; it < IPT; ++it) {
+        if ((uint32_t)it >= nv) break;
+        const uint32_t x = v[it];
+        if (it == 0) {
+            p = lds_lower_bound(w, 0, n, x);
+        } else {
+            for (int s = 0; s < 4 && p < n && w[p] < x; ++s) ++p;
+            if (p < n && w[p] < x) p = lds_lower_bound(w, p, n, x);
+        }
+        ok[it] = ok[it] && p < n && w[p] == x;
+    }
 }
 
 // Count of the last vertex when it has a single back edge (step `t`, rows of
@@ -759,40 +788,52 @@ __device__ __forceinline__ void walk_tile(const DevGraph& g, const gv_step_t& st
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         }
-        // 32-bit offsets inside the tile (t1 - t0 <= 64 IPT) and the row
+        // 32-bit offsets inside the tile (t1 - t0 <= 64 IPT) and the row; a
+        // lane takes IPT consecutive candidates, so that its run can be
+        // merged with the windows
         const uint32_t tn = (uint32_t)(t1 - t0);
         const uint32_t o0 = (uint32_t)(t0 - first);
         const uint32_t* anc = g.nbr + abeg + o0;
+        const uint32_t k0 = lane * (uint32_t)IPT;
+        const uint32_t nv = tn > k0 ? (tn - k0 < (uint32_t)IPT ? tn - k0 : (uint32_t)IPT) : 0u;
 #pragma unroll
         for (int it = 0; it < IPT; ++it) {
-            const uint32_t k = (uint32_t)it * 64u + lane;
+            const uint32_t k = k0 + (uint32_t)it;
             uint32_t v = 0;
             bool ok = false;
             if (k < tn) {
                 v = anc[k];
                 acc.rq += 4;
                 ok = v != cv0 && v != cv1 && v != cv2;
-                if (ok && nob > 0) {
-                    if (w0) {
-                        ok = lds_contains(win, n0, v);
-                    } else {
-                        acc.rq += 4ull * probes(oe0 - ob0);
-                        ok = contains_u32(g.nbr, ob0, oe0, v);
-                    }
-                }
-                if (ok && nob > 1) {
-                    if (w1) {
-                        ok = lds_contains(win + kWin, n1, v);
-                    } else {
-                        acc.rq += 4ull * probes(oe1 - ob1);
-                        ok = contains_u32(g.nbr, ob1, oe1, v);
-                    }
-                }
             }
             okk[it] = ok;
             vv[it] = v;
             rr[it] = r;
-            six[it] = shared ? o0 + (uint32_t)it * 64u + lane : ~0u;
+            six[it] = shared ? o0 + k : ~0u;
+        }
+        if (nob > 0) {
+            if (w0) {
+                window_member<IPT>(win, n0, vv, nv, okk);
+            } else {
+#pragma unroll
+                for (int it = 0; it < IPT; ++it)
+                    if (okk[it]) {
+                        acc.rq += 4ull * probes(oe0 - ob0);
+                        okk[it] = contains_u32(g.nbr, ob0, oe0, vv[it]);
+                    }
+            }
+        }
+        if (nob > 1) {
+            if (w1) {
+                window_member<IPT>(win + kWin, n1, vv, nv, okk);
+            } else {
+#pragma unroll
+                for (int it = 0; it < IPT; ++it)
+                    if (okk[it]) {
+                        acc.rq += 4ull * probes(oe1 - ob1);
+                        okk[it] = contains_u32(g.nbr, ob1, oe1, vv[it]);
+                    }
+            }
         }
         // the windows are rewritten by the next tile
         if (nob > 0) __builtin_amdgcn_wave_barrier();
@@ -806,10 +847,38 @@ __device__ __forceinline__ void walk_tile(const DevGraph& g, const gv_step_t& st
         const uint64_t nr = r_hi - r_lo + 1;
         const bool staged = nr <= 64ull * IPT;
         const uint64_t first0 = in.csum[r_lo] - in.cnt[r_lo];
+        // up to kRowStage rows: each row's data is also staged once (in the
+        // window space, kRowWords words a row): anchor start, the other
+        // back lists' segments, the same-label values and whether the row's
+        // anchor segment is the fused last vertex's
+        const bool rstaged = staged && nr <= kRowStage && st.nback <= 3 && st.ncoll <= 3;
+        const uint32_t nob = st.nback - 1;
         if (staged) {
             for (uint32_t i = lane; i <= nr; i += 64) {
                 const uint64_t x = i < nr ? in.csum[r_lo + i] - in.cnt[r_lo + i] : in.csum[r_hi];
                 lx[i] = x <= t0 ? 0u : (x - t0 > 0xffffffffull ? 0xffffffffu : (uint32_t)(x - t0));
+            }
+            if (rstaged && lane < nr) {
+                const uint64_t r = r_lo + lane;
+                const uint32_t* row = rowp(r);
+                const uint32_t skip = in.anc_i[r];
+                uint32_t* q = win + lane * kRowWords;
+                q[0] = in.anc_beg[r];
+                q[1] = q[2] = q[3] = q[4] = 0;
+                uint32_t k = 0;
+                for (uint32_t i = 0; i < st.nback; ++i) {
+                    if (i == skip) continue;
+                    uint32_t b, e;
+                    seg(g, st.back_part[i], row[st.back_col[i]], st.label, b, e);
+                    q[1 + 2 * k] = b;
+                    q[2 + 2 * k] = e;
+                    ++k;
+                }
+                q[5] = st.ncoll > 0 ? row[st.coll_col[0]] : 0xffffffffu;
+                q[6] = st.ncoll > 1 ? row[st.coll_col[1]] : 0xffffffffu;
+                q[7] = st.ncoll > 2 ? row[st.coll_col[2]] : 0xffffffffu;
+                q[8] = (smask >> skip) & 1u;
+                acc.rq += 4ull * L + 17ull + 8ull * nob;
             }
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
@@ -817,6 +886,44 @@ __device__ __forceinline__ void walk_tile(const DevGraph& g, const gv_step_t& st
         }
 #pragma unroll
         for (int it = 0; it < IPT; ++it) {
+            if (rstaged) {
+                const uint64_t c = t0 + (uint64_t)it * 64 + lane;
+                bool ok = false;
+                uint32_t v = 0, sx = ~0u, i = 0;
+                if (c < t1) {
+                    // first i with lx[i + 1] > c - t0
+                    const uint32_t cr = (uint32_t)(c - t0);
+                    uint32_t n = (uint32_t)nr;
+                    while (n > 0) {
+                        const uint32_t h = n >> 1;
+                        if (lx[i + h + 1] <= cr) {
+                            i += h + 1;
+                            n -= h + 1;
+                        } else {
+                            n = h;
+                        }
+                    }
+                    const uint32_t* q = win + i * kRowWords;
+                    const uint32_t off = i == 0 ? (uint32_t)(c - first0) : cr - lx[i];
+                    v = g.nbr[q[0] + off];
+                    acc.rq += 4;
+                    ok = v != q[5] && v != q[6] && v != q[7];
+                    if (ok && nob > 0) {
+                        acc.rq += 4ull * probes(q[2] - q[1]);
+                        ok = contains_u32(g.nbr, q[1], q[2], v);
+                    }
+                    if (ok && nob > 1) {
+                        acc.rq += 4ull * probes(q[4] - q[3]);
+                        ok = contains_u32(g.nbr, q[3], q[4], v);
+                    }
+                    if (q[8]) sx = off;
+                }
+                okk[it] = ok;
+                vv[it] = v;
+                rr[it] = r_lo + i;
+                six[it] = sx;
+                continue;
+            }
             const uint64_t c = t0 + (uint64_t)it * 64 + lane;
             bool ok = false;
             uint32_t v = 0, sx = ~0u;
