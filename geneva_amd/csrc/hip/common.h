@@ -1041,6 +1041,11 @@ struct gv_ctx {
     hipEvent_t ev[8] = {};
     // accumulators: [0] count, [1] bytes, [2] rows written, [3] overflow
     unsigned long long* acc = nullptr;
+    // a query's folded acc[0..3], written by k_fold_acc straight into pinned
+    // coherent host memory (host and device addresses), so reading a result
+    // takes no copy
+    unsigned long long* hres = nullptr;
+    unsigned long long* dres = nullptr;
     uint32_t* gbits = nullptr;
     uint32_t gbits_words = 0;
     // scan scratch

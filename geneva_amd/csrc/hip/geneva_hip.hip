@@ -560,8 +560,8 @@ __global__ void k_zero_query(unsigned long long* acc, uint32_t nacc, unsigned lo
 }
 
 // acc[0] and acc[1] = the sums over the spread lines of the count and the
-// algorithmic bytes (one wave)
-__global__ void k_fold_acc(unsigned long long* acc) {
+// algorithmic bytes (one wave); with out, acc[0..3] are also copied there
+__global__ void k_fold_acc(unsigned long long* acc, unsigned long long* out) {
     uint64_t cnt = 0, by = 0;
     for (uint32_t i = threadIdx.x; i < kAccSlots; i += 64) {
         cnt += acc[kAccLine0 + i * kAccStride];
@@ -572,6 +572,12 @@ __global__ void k_fold_acc(unsigned long long* acc) {
     if (threadIdx.x == 0) {
         acc[0] = cnt;
         acc[1] = by;
+        if (out) {
+            out[0] = cnt;
+            out[1] = by;
+            out[2] = acc[2];
+            out[3] = acc[3];
+        }
     }
 }
 
@@ -696,6 +702,12 @@ int gv_ctx_create(int device, gv_ctx_t* out) {
         return GV_ERR_NOMEM;
     }
     HIPCHK(hipMemsetAsync(c->lvl, 0, (GV_MAXK + 1) * sizeof(unsigned long long), c->stream));
+    if (hipHostMalloc((void**)&c->hres, 4 * sizeof(unsigned long long), hipHostMallocCoherent) !=
+            hipSuccess ||
+        hipHostGetDevicePointer((void**)&c->dres, c->hres, 0) != hipSuccess) {
+        gv_ctx_destroy(c);
+        return GV_ERR_NOMEM;
+    }
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->num_cu = prop.multiProcessorCount;
     *out = c;
@@ -707,6 +719,7 @@ int gv_ctx_destroy(gv_ctx_t c) {
     hipSetDevice(c->device);
     hipStreamSynchronize(c->stream);
     if (c->acc) hipFree(c->acc);
+    if (c->hres) hipHostFree(c->hres);
     if (c->gbits) hipFree(c->gbits);
     if (c->tiles) hipFree(c->tiles);
     if (c->preq) hipFree(c->preq);
@@ -1460,14 +1473,13 @@ int gv_mine_levels(gv_ctx_t c, gv_graph_t g, const gv_query_t* q, const uint32_t
         if (counting) break;
         rows = tables[w + 1];
     }
-    unsigned long long h[4];
-    hipLaunchKernelGGL(k_fold_acc, dim3(1), dim3(64), 0, c->stream, c->acc);
+    hipLaunchKernelGGL(k_fold_acc, dim3(1), dim3(64), 0, c->stream, c->acc, c->dres);
     HIPCHK(hipGetLastError());
-    HIPCHK(hipMemcpyAsync(h, c->acc, sizeof(h), hipMemcpyDeviceToHost, c->stream));
     if (bits && nwords)
         HIPCHK(hipMemcpyAsync(bits, c->gbits, nwords * sizeof(uint32_t), hipMemcpyDeviceToHost,
                               c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
+    const volatile unsigned long long* h = c->hres;
     if (h[3]) return GV_ERR_CAPACITY;
     if (count) *count = h[0];
     if (alg_bytes) *alg_bytes = h[1];
@@ -1490,7 +1502,7 @@ int gv_counters_read(gv_ctx_t c, uint64_t* count, uint64_t* bytes, uint32_t* bit
                      uint32_t nwords) {
     if (!c) return GV_ERR_ARG;
     unsigned long long h[2];
-    hipLaunchKernelGGL(k_fold_acc, dim3(1), dim3(64), 0, c->stream, c->acc);
+    hipLaunchKernelGGL(k_fold_acc, dim3(1), dim3(64), 0, c->stream, c->acc, nullptr);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(h, c->acc, sizeof(h), hipMemcpyDeviceToHost, c->stream));
     if (bits && nwords) {
