@@ -1046,6 +1046,9 @@ struct gv_ctx {
     // takes no copy
     unsigned long long* hres = nullptr;
     unsigned long long* dres = nullptr;
+    // acc and lvl are all zero (gv_mine_levels leaves them so; the other
+    // entry points that write them clear the flag)
+    bool clean = true;
     uint32_t* gbits = nullptr;
     uint32_t gbits_words = 0;
     // scan scratch

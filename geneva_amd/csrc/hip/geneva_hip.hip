@@ -560,8 +560,10 @@ __global__ void k_zero_query(unsigned long long* acc, uint32_t nacc, unsigned lo
 }
 
 // acc[0] and acc[1] = the sums over the spread lines of the count and the
-// algorithmic bytes (one wave); with out, acc[0..3] are also copied there
-__global__ void k_fold_acc(unsigned long long* acc, unsigned long long* out) {
+// algorithmic bytes (one wave of 64); with out, acc[0..3] go there instead
+// and acc (nacc words) and lvl (nlvl words) are cleared for the next query
+__global__ void k_fold_acc(unsigned long long* acc, unsigned long long* out, uint32_t nacc,
+                           unsigned long long* lvl, uint32_t nlvl) {
     uint64_t cnt = 0, by = 0;
     for (uint32_t i = threadIdx.x; i < kAccSlots; i += 64) {
         cnt += acc[kAccLine0 + i * kAccStride];
@@ -569,16 +571,23 @@ __global__ void k_fold_acc(unsigned long long* acc, unsigned long long* out) {
     }
     cnt = wave_sum_u64(cnt);
     by = wave_sum_u64(by);
-    if (threadIdx.x == 0) {
-        acc[0] = cnt;
-        acc[1] = by;
-        if (out) {
-            out[0] = cnt;
-            out[1] = by;
-            out[2] = acc[2];
-            out[3] = acc[3];
+    if (!out) {
+        if (threadIdx.x == 0) {
+            acc[0] = cnt;
+            acc[1] = by;
         }
+        return;
     }
+    const uint64_t rows = acc[2], over = acc[3];
+    __syncthreads();   // every read of acc is done before it is cleared
+    if (threadIdx.x == 0) {
+        out[0] = cnt;
+        out[1] = by;
+        out[2] = rows;
+        out[3] = over;
+    }
+    for (uint32_t i = threadIdx.x; i < nacc; i += 64) acc[i] = 0;
+    for (uint32_t i = threadIdx.x; i < nlvl; i += 64) lvl[i] = 0;
 }
 
 __global__ void k_copy_back(const uint32_t* src, uint32_t* dst, uint64_t n) {
@@ -1035,6 +1044,7 @@ int gv_frontier_destroy(gv_frontier_t f) {
 int gv_size_rows(gv_ctx_t c, gv_graph_t g, const gv_step_t* st, const uint32_t* rows,
                  uint32_t L, uint64_t nrows, gv_frontier_t f, uint64_t* total) {
     if (!c || !g || !f || !total || !rows || L == 0 || !valid_step(st, L, g)) return GV_ERR_ARG;
+    c->clean = false;
     if (nrows > f->cap) return GV_ERR_CAPACITY;
     *total = 0;
     f->nrows = nrows;
@@ -1119,6 +1129,7 @@ int gv_ext_emb(gv_ctx_t c, gv_graph_t g, const gv_step_t* st, const uint32_t* ro
                uint64_t nrows, gv_frontier_t f, uint64_t c0, uint64_t c1, uint32_t* out,
                uint64_t max_rows, uint64_t* written) {
     if (!c || !g || !f || !written || !valid_step(st, L, g) || c0 > c1) return GV_ERR_ARG;
+    c->clean = false;
     if (nrows != f->nrows || c1 > f->total) return GV_ERR_ARG;
     *written = 0;
     if (c0 == c1 || nrows == 0) return GV_OK;
@@ -1142,6 +1153,7 @@ int gv_count_emb(gv_ctx_t c, gv_graph_t g, const gv_step_t* st, const gv_step_t*
                  const uint32_t* rows, uint32_t L, uint64_t nrows, gv_frontier_t f, uint64_t c0,
                  uint64_t c1) {
     if (!c || !g || !f || !valid_step(st, L, g) || c0 > c1) return GV_ERR_ARG;
+    c->clean = false;
     if (tail && !valid_tail(tail, L, g)) return GV_ERR_ARG;
     if (nrows != f->nrows || c1 > f->total) return GV_ERR_ARG;
     if (c0 == c1 || nrows == 0) return GV_OK;
@@ -1177,6 +1189,7 @@ static void leaves_args(LeavesArgs& a, gv_ctx* c, gv_graph* g, const gv_leaves_t
 int gv_count_leaves(gv_ctx_t c, gv_graph_t g, const gv_leaves_t* d, const uint32_t* rows,
                     uint64_t nrows) {
     if (!c || !g || !valid_leaves(d, g)) return GV_ERR_ARG;
+    c->clean = false;
     if (nrows == 0) return GV_OK;
     if (!rows) return GV_ERR_ARG;
     HIPCHK(hipSetDevice(c->device));
@@ -1198,6 +1211,7 @@ int gv_count_leaves(gv_ctx_t c, gv_graph_t g, const gv_leaves_t* d, const uint32
 int gv_count_rows(gv_ctx_t c, gv_graph_t g, const gv_step_t* st, const gv_step_t* tail,
                   const uint32_t* rows, uint32_t L, uint64_t nrows) {
     if (!c || !g || !valid_step(st, L, g)) return GV_ERR_ARG;
+    c->clean = false;
     if (tail && !valid_tail(tail, L, g)) return GV_ERR_ARG;
     const int mode = count_rows_mode(st, tail, L);
     if (mode < 0) return GV_ERR_ARG;
@@ -1265,12 +1279,13 @@ int gv_mine_levels(gv_ctx_t c, gv_graph_t g, const gv_query_t* q, const uint32_t
     const DevGraph dg = dev_view(g);
     unsigned long long* stop = c->acc + 3;
     uint32_t* gb = (g->d_gof && c->gbits && c->gbits_words * 32ull >= g->ngraphs) ? c->gbits : nullptr;
-    {
+    if (!c->clean || gb) {
         const uint32_t words = g->ngraphs ? (g->ngraphs + 31) / 32 : 0;
         hipLaunchKernelGGL(k_zero_query, dim3(std::max<uint32_t>(1, std::min<uint32_t>((words + 255) / 256, 1024))),
                            dim3(256), 0, c->stream, c->acc, kAccWords, c->lvl, (uint32_t)(GV_MAXK + 1),
                            gb, gb ? words : 0u);
     }
+    c->clean = false;
     const uint32_t* rows = start_rows;
     for (uint32_t w = 1; w <= wlast && nstart; ++w) {
         const gv_step_t& st = q->step[w];
@@ -1473,8 +1488,10 @@ int gv_mine_levels(gv_ctx_t c, gv_graph_t g, const gv_query_t* q, const uint32_t
         if (counting) break;
         rows = tables[w + 1];
     }
-    hipLaunchKernelGGL(k_fold_acc, dim3(1), dim3(64), 0, c->stream, c->acc, c->dres);
+    hipLaunchKernelGGL(k_fold_acc, dim3(1), dim3(64), 0, c->stream, c->acc, c->dres, kAccWords,
+                       c->lvl, (uint32_t)(GV_MAXK + 1));
     HIPCHK(hipGetLastError());
+    c->clean = true;
     if (bits && nwords)
         HIPCHK(hipMemcpyAsync(bits, c->gbits, nwords * sizeof(uint32_t), hipMemcpyDeviceToHost,
                               c->stream));
@@ -1488,6 +1505,7 @@ int gv_mine_levels(gv_ctx_t c, gv_graph_t g, const gv_query_t* q, const uint32_t
 
 int gv_counters_reset(gv_ctx_t c, gv_graph_t g) {
     if (!c) return GV_ERR_ARG;
+    c->clean = false;
     HIPCHK(hipSetDevice(c->device));
     HIPCHK(hipMemsetAsync(c->acc, 0, kAccWords * sizeof(unsigned long long), c->stream));
     if (g && g->ngraphs) {
@@ -1501,8 +1519,9 @@ int gv_counters_reset(gv_ctx_t c, gv_graph_t g) {
 int gv_counters_read(gv_ctx_t c, uint64_t* count, uint64_t* bytes, uint32_t* bits,
                      uint32_t nwords) {
     if (!c) return GV_ERR_ARG;
+    c->clean = false;
     unsigned long long h[2];
-    hipLaunchKernelGGL(k_fold_acc, dim3(1), dim3(64), 0, c->stream, c->acc, nullptr);
+    hipLaunchKernelGGL(k_fold_acc, dim3(1), dim3(64), 0, c->stream, c->acc, nullptr, 0u, nullptr, 0u);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(h, c->acc, sizeof(h), hipMemcpyDeviceToHost, c->stream));
     if (bits && nwords) {
