@@ -442,6 +442,7 @@ __global__ __launch_bounds__(kBlock) void k_b_walk(BArgs a) {
         const uint64_t t1 = t0 + a.tile < S.cand_end ? t0 + a.tile : S.cand_end;
         TileIn in;
         in.rows = slot_rows(a, s, S, D);
+        in.anc = a.g.nbr;
         in.row_base = S.row_end - rows;
         in.L = L;
         in.csum = a.csum;

@@ -626,6 +626,9 @@ __device__ __forceinline__ void block_acc_add2(unsigned long long* p0, uint64_t 
 
 struct TileIn {
     const uint32_t* rows;    // table of the step's input rows (row r at rows + (r - row_base) * L)
+    // anchor entries: a row's candidates are anc[anc_beg[r] ..]; the graph's
+    // neighbour array, or the parents' intersections of a prefix-shared step
+    const uint32_t* anc;
     uint64_t row_base;
     uint32_t L;
     const uint64_t* csum;    // inclusive candidate prefix per row
@@ -640,6 +643,7 @@ struct TileOut {             // extension output
     uint64_t cap;            // rows
     unsigned long long* out_n;
     unsigned long long* overflow;
+    uint32_t* par = nullptr;  // nullable: input row of every output row
 };
 
 struct TileAcc {
@@ -717,7 +721,7 @@ __device__ __forceinline__ void walk_tile(const DevGraph& g, const gv_step_t& st
     // anchors (back-edge indices) whose segment is also the fused last
     // vertex's segment
     uint32_t smask = 0;
-    if (kCount && has_tail && tail.label == st.label && tail.back_col[0] != L) {
+    if (kCount && has_tail && tail.label == st.label && tail.back_col[0] != L && in.anc == g.nbr) {
         for (uint32_t i = 0; i < st.nback; ++i)
             if (st.back_col[i] == tail.back_col[0] && st.back_part[i] == tail.back_part[0])
                 smask |= 1u << i;
@@ -762,8 +766,8 @@ __device__ __forceinline__ void walk_tile(const DevGraph& g, const gv_step_t& st
         uint32_t n0 = 0, n1 = 0;
         bool w0 = false, w1 = false;
         if (nob > 0) {
-            const uint32_t vmin = g.nbr[abeg + (uint32_t)(t0 - first)];
-            const uint32_t vmax = g.nbr[abeg + (uint32_t)(t1 - 1 - first)];
+            const uint32_t vmin = in.anc[abeg + (uint32_t)(t0 - first)];
+            const uint32_t vmax = in.anc[abeg + (uint32_t)(t1 - 1 - first)];
             const uint32_t l0 = wave_lower_bound(g.nbr, ob0, oe0, vmin);
             const uint32_t h0 = wave_lower_bound(g.nbr, l0, oe0, vmax + 1);
             n0 = h0 - l0;
@@ -793,7 +797,7 @@ __device__ __forceinline__ void walk_tile(const DevGraph& g, const gv_step_t& st
         // merged with the windows
         const uint32_t tn = (uint32_t)(t1 - t0);
         const uint32_t o0 = (uint32_t)(t0 - first);
-        const uint32_t* anc = g.nbr + abeg + o0;
+        const uint32_t* anc = in.anc + abeg + o0;
         const uint32_t k0 = lane * (uint32_t)IPT;
         const uint32_t nv = tn > k0 ? (tn - k0 < (uint32_t)IPT ? tn - k0 : (uint32_t)IPT) : 0u;
 #pragma unroll
@@ -905,7 +909,7 @@ __device__ __forceinline__ void walk_tile(const DevGraph& g, const gv_step_t& st
                     }
                     const uint32_t* q = win + i * kRowWords;
                     const uint32_t off = i == 0 ? (uint32_t)(c - first0) : cr - lx[i];
-                    v = g.nbr[q[0] + off];
+                    v = in.anc[q[0] + off];
                     acc.rq += 4;
                     ok = v != q[5] && v != q[6] && v != q[7];
                     if (ok && nob > 0) {
@@ -949,7 +953,7 @@ __device__ __forceinline__ void walk_tile(const DevGraph& g, const gv_step_t& st
                     r = find_row(in.csum, r_lo, r_hi + 1, c);
                     first = in.csum[r] - in.cnt[r];
                 }
-                v = g.nbr[in.anc_beg[r] + (uint32_t)(c - first)];
+                v = in.anc[in.anc_beg[r] + (uint32_t)(c - first)];
                 acc.rq += 4;
                 const uint32_t* row = rowp(r);
                 ok = true;
@@ -1019,6 +1023,7 @@ __device__ __forceinline__ void walk_tile(const DevGraph& g, const gv_step_t& st
             const uint32_t p = lanes_below(mask) * W;
             for (uint32_t j = 0; j < L; ++j) stage[p + j] = row[j];
             stage[p + L] = vv[it];
+            if (o.par) o.par[base + lanes_below(mask)] = (uint32_t)(rr[it] - in.row_base);
         }
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -1073,6 +1078,14 @@ struct gv_ctx {
     unsigned long long* chainw = nullptr;
     uint32_t* chainneed = nullptr;        // chainw[y] is needed when == chain_epoch
     uint64_t chainw_cap = 0;
+    // prefix-shared counted step (gv_mine_levels): parent row of every row of
+    // the counted table, the parents' intersections and their lengths
+    uint32_t* par = nullptr;
+    uint64_t par_cap = 0;
+    uint32_t* isect = nullptr;
+    uint64_t isect_cap = 0;
+    uint32_t* ilen = nullptr;
+    uint64_t ilen_cap = 0;
     uint32_t chain_epoch = 0;
 };
 

@@ -324,6 +324,147 @@ __global__ __launch_bounds__(kBlock) void k_count_chain(ChainCountArgs a) {
     }
 }
 
+// ---------------------------------------------------------------------------
+// prefix-shared counted step.  The counted vertex u has back edges into the
+// parent prefix (columns < L - 1, the same for every row extended from one
+// parent row) and possibly to column L - 1.  The prefix lists are
+// intersected once per parent (k_isect, after a sizing of the parent table by
+// the prefix step), and each row then only meets its parent's intersection
+// with the list of column L - 1 (k_child_size sizes it, the counting walk
+// reads its candidates from the intersections).  The algorithmic bytes stay
+// those of u's step over the full lists, added by k_child_size.
+struct IsectArgs {
+    DevGraph g;
+    const uint32_t* rows;       // parent table, width L
+    uint64_t nrows;
+    const unsigned long long* nrows_dev;
+    const unsigned long long* stop;
+    uint32_t L;
+    const uint32_t* anc_beg;    // sizing of the parents by the prefix step
+    const uint32_t* cnt;
+    const uint8_t* anc_i;
+    const uint64_t* csum;
+    uint32_t* out;              // intersections, parent p's at csum[p] - cnt[p]
+    uint64_t out_cap;
+    uint32_t* len;              // entries of each parent's intersection
+    unsigned long long* overflow;
+    gv_step_t st;               // the prefix step: prefix lists, same-label prefix columns
+};
+
+// one wave per parent: the anchor's (cut) entries in order, those in every
+// other prefix list and not equal to a same-label prefix value written
+// contiguously, so each intersection is sorted
+__global__ __launch_bounds__(kBlock) void k_isect(IsectArgs a) {
+    const unsigned lane = lane_id();
+    const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+    const uint64_t n = dev_rows(a.nrows, a.nrows_dev, a.stop);
+    if (n == 0) return;
+    if (a.csum[n - 1] > a.out_cap) {
+        if (wave == 0 && lane == 0) atomicOr(a.overflow, 1ull);
+        return;
+    }
+    for (uint64_t p = wave; p < n; p += nwaves) {
+        const uint32_t* row = a.rows + p * a.L;
+        const uint32_t c = a.cnt[p], ab = a.anc_beg[p], skip = a.anc_i[p];
+        const uint64_t o = a.csum[p] - c;
+        uint32_t ob[3] = {0, 0, 0}, oe[3] = {0, 0, 0}, no = 0;
+        for (uint32_t i = 0; i < a.st.nback; ++i) {
+            if (i == skip) continue;
+            uint32_t b, e;
+            seg(a.g, a.st.back_part[i], row[a.st.back_col[i]], a.st.label, b, e);
+            if (no == 0) {
+                ob[0] = b;
+                oe[0] = e;
+            } else if (no == 1) {
+                ob[1] = b;
+                oe[1] = e;
+            } else {
+                ob[2] = b;
+                oe[2] = e;
+            }
+            ++no;
+        }
+        const uint32_t cv0 = a.st.ncoll > 0 ? row[a.st.coll_col[0]] : 0xffffffffu;
+        const uint32_t cv1 = a.st.ncoll > 1 ? row[a.st.coll_col[1]] : 0xffffffffu;
+        const uint32_t cv2 = a.st.ncoll > 2 ? row[a.st.coll_col[2]] : 0xffffffffu;
+        uint32_t written = 0;
+        for (uint32_t k0 = 0; k0 < c; k0 += 64) {
+            const uint32_t k = k0 + lane;
+            bool ok = k < c;
+            const uint32_t v = ok ? a.g.nbr[ab + k] : 0u;
+            ok = ok && v != cv0 && v != cv1 && v != cv2;
+            if (ok && no > 0) ok = contains_u32(a.g.nbr, ob[0], oe[0], v);
+            if (ok && no > 1) ok = contains_u32(a.g.nbr, ob[1], oe[1], v);
+            if (ok && no > 2) ok = contains_u32(a.g.nbr, ob[2], oe[2], v);
+            const uint64_t m = __ballot(ok);
+            if (ok) a.out[o + written + lanes_below(m)] = v;
+            written += (uint32_t)__popcll(m);
+        }
+        if (lane == 0) a.len[p] = written;
+    }
+}
+
+struct ChildSizeArgs {
+    DevGraph g;
+    const uint32_t* rows;       // counted table, width L
+    uint64_t nrows;
+    const unsigned long long* nrows_dev;
+    const unsigned long long* stop;
+    uint32_t L;
+    const uint32_t* par;        // parent of each row
+    const uint32_t* pcnt;       // parents' sizing (offsets of their intersections)
+    const uint64_t* pcsum;
+    const uint32_t* ilen;
+    const uint32_t* isect;
+    uint32_t* anc_beg;          // the counted step's sizing, anchor = the intersection
+    uint32_t* cnt;
+    uint8_t* anc_i;
+    uint64_t* tile_sums;        // as in k_size
+    unsigned long long* bytes;
+    gv_step_t st;               // u's step (its full lists give the algorithmic bytes)
+    uint32_t cut_last;          // u is restricted below column L - 1
+};
+
+__global__ __launch_bounds__(kBlock) void k_child_size(ChildSizeArgs a) {
+    uint64_t by = 0;
+    const uint64_t nrows = dev_rows(a.nrows, a.nrows_dev, a.stop);
+    const uint64_t ntiles = (nrows + kSizeTile - 1) / kSizeTile;
+    for (uint64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        uint64_t ts = 0;
+        for (int k = 0; k < kSizeRows; ++k) {
+            const uint64_t r = t * kSizeTile + (uint64_t)k * kBlock + threadIdx.x;
+            if (r >= nrows) continue;
+            const uint32_t* row = a.rows + r * a.L;
+            const uint32_t p = a.par[r];
+            const uint32_t o = (uint32_t)(a.pcsum[p] - a.pcnt[p]);
+            uint32_t c = a.ilen[p];
+            if (a.cut_last) c = lower_bound_u32(a.isect, o, o + c, row[a.L - 1]) - o;
+            a.anc_beg[r] = o;
+            a.cnt[r] = c;
+            a.anc_i[r] = 0;
+            ts += c;
+            by += 4ull * a.L;
+            for (uint32_t i = 0; i < a.st.nback; ++i) {
+                uint32_t b, e;
+                seg(a.g, a.st.back_part[i], row[a.st.back_col[i]], a.st.label, b, e);
+                by += 8ull + 4ull * (e - b);
+            }
+        }
+        __shared__ uint64_t tsum[kWavesPerBlock];
+        ts = wave_sum_u64(ts);
+        if (lane_id() == 0) tsum[threadIdx.x >> 6] = ts;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint64_t x = 0;
+            for (int i = 0; i < kWavesPerBlock; ++i) x += tsum[i];
+            a.tile_sums[t] = x;
+        }
+        __syncthreads();
+    }
+    block_acc_add2(nullptr, 0, a.bytes, by);
+}
+
 constexpr uint32_t kChainMinDegree = 256;
 
 // The pendant chain of the last two steps of q (counted step w = k - 2),
@@ -337,6 +478,61 @@ bool chain_tail(const gv_query_t* q, uint32_t w) {
     // shared one)
     return u1.nback == 1 && u1.back_col[0] < w && u1.ncoll == 0 && u1.restrict_col == GV_NONE &&
            u2.nback == 1 && u2.back_col[0] == w && u2.ncoll == 0 && u2.restrict_col == GV_NONE;
+}
+
+// Counted step w of q split into its prefix part (back edges and same-label
+// columns below w - 1, the restriction if it lies there) and the part left to
+// each row (the edge to column w - 1, its same-label check and restriction),
+// when at least two back edges lie in the prefix (k_isect).  child's list 0
+// stands for the parent's intersection (its column is never read).
+bool prefix_split(const gv_query_t* q, uint32_t w, gv_step_t& pre, gv_step_t& child,
+                  uint32_t& cut_last) {
+    if (w < 3) return false;
+    const gv_step_t& st = q->step[w];
+    pre = st;
+    pre.nback = 0;
+    bool has_last = false;
+    uint8_t part_last = 0;
+    for (uint32_t i = 0; i < st.nback; ++i) {
+        if (st.back_col[i] < w - 1) {
+            pre.back_col[pre.nback] = st.back_col[i];
+            pre.back_part[pre.nback] = st.back_part[i];
+            ++pre.nback;
+        } else {
+            has_last = true;
+            part_last = st.back_part[i];
+        }
+    }
+    if (pre.nback < 2 || pre.nback > 4) return false;
+    pre.restrict_col = st.restrict_col != GV_NONE && st.restrict_col < w - 1 ? st.restrict_col : GV_NONE;
+    pre.ncoll = 0;
+    child = st;
+    child.ncoll = 0;
+    for (uint32_t j = 0; j < st.ncoll; ++j) {
+        if (st.coll_col[j] < w - 1) pre.coll_col[pre.ncoll++] = st.coll_col[j];
+        else child.coll_col[child.ncoll++] = st.coll_col[j];
+    }
+    if (pre.ncoll > 3) return false;
+    child.nback = has_last ? 2 : 1;
+    child.back_col[0] = 0;
+    child.back_part[0] = pre.back_part[0];
+    child.back_col[1] = (uint8_t)(w - 1);
+    child.back_part[1] = part_last;
+    child.restrict_col = GV_NONE;
+    cut_last = st.restrict_col == w - 1 ? 1u : 0u;
+    return true;
+}
+
+// grow-only device buffer of n uint32
+int ensure_u32(gv_ctx* c, uint32_t*& p, uint64_t& cap, uint64_t n) {
+    if (n <= cap) return GV_OK;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (p) HIPCHK(hipFree(p));
+    p = nullptr;
+    cap = 0;
+    if (hipMalloc(&p, std::max<uint64_t>(n, 1) * sizeof(uint32_t)) != hipSuccess) return GV_ERR_NOMEM;
+    cap = n;
+    return GV_OK;
 }
 
 // ---------------------------------------------------------------------------
@@ -486,6 +682,11 @@ struct WalkArgs {
     // the candidates are all of theirs, [0, csum[rows - 1]); none when *stop
     const unsigned long long* nrows_dev;
     const unsigned long long* stop;
+    // prefix-shared step (nullable): candidates come from anc_base instead of
+    // the graph's neighbour array; extension: the input row of every output
+    // row is written to par
+    const uint32_t* anc_base;
+    uint32_t* par;
     gv_step_t st;
     gv_step_t tail;
     int has_tail;
@@ -511,6 +712,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) 
         atomicAdd(a.units, (unsigned long long)(c1 - c0));
     TileIn in;
     in.rows = a.rows;
+    in.anc = a.anc_base ? a.anc_base : a.g.nbr;
     in.row_base = 0;
     in.L = a.L;
     in.csum = a.csum;
@@ -519,7 +721,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) 
     in.anc_i = a.anc_i;
     in.row_lo = 0;
     in.row_hi = nrows;
-    TileOut o{a.out, a.out_cap, a.out_n, a.overflow};
+    TileOut o{a.out, a.out_cap, a.out_n, a.overflow, a.par};
     TileAcc acc;
     __shared__ uint64_t resv[3 * kWavesPerBlock];
     const BlockReserve reserve{resv};
@@ -736,6 +938,9 @@ int gv_ctx_destroy(gv_ctx_t c) {
     if (c->lvl) hipFree(c->lvl);
     if (c->chainw) hipFree(c->chainw);
     if (c->chainneed) hipFree(c->chainneed);
+    if (c->par) hipFree(c->par);
+    if (c->isect) hipFree(c->isect);
+    if (c->ilen) hipFree(c->ilen);
     for (auto& r : c->recs) {
         hipEventDestroy(r.a);
         hipEventDestroy(r.b);
@@ -1274,6 +1479,28 @@ int gv_mine_levels(gv_ctx_t c, gv_graph_t g, const gv_query_t* q, const uint32_t
         if (!tables[w + 1]) return GV_ERR_ARG;
     }
     HIPCHK(hipSetDevice(c->device));
+    // the counted step in prefix-shared form (with the counting modes only,
+    // as the leaf, chain and row counts)
+    gv_step_t pre, child;
+    uint32_t cut_last = 0;
+    bool split = false;
+    if (count_by_rows && !(leaves && wlast == leaves->width) && wlast >= 3) {
+        const bool lastw = wlast + 1 == k;
+        const bool fusedw = wlast + 2 == k && q->step[wlast + 1].nback == 1;
+        const gv_step_t* tl = fusedw ? &q->step[wlast + 1] : nullptr;
+        const bool chain = fusedw && g->max_deg > kChainMinDegree && chain_tail(q, wlast);
+        const bool rowsm = count_rows_mode(&q->step[wlast], tl, wlast) >= 0;
+        if ((lastw || fusedw) && !chain && !rowsm && frontiers[wlast - 1] && table_rows[wlast - 1] &&
+            table_rows[wlast])
+            split = prefix_split(q, wlast, pre, child, cut_last);
+    }
+    if (split) {
+        const uint64_t icap = std::min<uint64_t>(table_rows[wlast] * wlast, 0xffffffffull);
+        int rc = ensure_u32(c, c->par, c->par_cap, table_rows[wlast]);
+        if (rc == GV_OK) rc = ensure_u32(c, c->ilen, c->ilen_cap, table_rows[wlast - 1]);
+        if (rc == GV_OK) rc = ensure_u32(c, c->isect, c->isect_cap, icap);
+        if (rc != GV_OK) return rc;
+    }
     int rc0 = ensure_gbits(c, g);
     if (rc0 != GV_OK) return rc0;
     const DevGraph dg = dev_view(g);
@@ -1408,7 +1635,78 @@ int gv_mine_levels(gv_ctx_t c, gv_graph_t g, const gv_query_t* q, const uint32_t
             break;
         }
         gv_frontier* f = frontiers[w];
-        {
+        const bool split_here = split && w == wlast;
+        if (split_here) {
+            // parents: sized by the prefix step (no algorithmic bytes), then
+            // intersected; the rows then sized by their parents' intersections
+            gv_frontier* fp = frontiers[w - 1];
+            const uint64_t pcap = table_rows[w - 1];
+            const unsigned long long* pndev = c->lvl + (w - 1);
+            const uint32_t* prows = tables[w - 1];
+            SizeArgs sa;
+            memset(&sa, 0, sizeof(sa));
+            sa.g = dg;
+            sa.rows = prows;
+            sa.nrows = pcap;
+            sa.nrows_dev = pndev;
+            sa.stop = stop;
+            sa.L = w - 1;
+            sa.anc_beg = fp->anc_beg;
+            sa.cnt = fp->cnt;
+            sa.anc_i = fp->anc_i;
+            sa.st = pre;
+            sa.start_pos = -1;
+            int rc1 = ensure_tiles(c, std::max<uint64_t>(pcap, cap));
+            if (rc1 != GV_OK) return rc1;
+            sa.tile_sums = c->tiles;
+            ProfScope ps(c, kKindSize);
+            hipLaunchKernelGGL(k_size<false>, dim3(grid_for(pcap, kSizeTile, c->num_cu * 16)), dim3(kBlock),
+                               0, c->stream, sa);
+            HIPCHK(hipGetLastError());
+            rc1 = scan_counts(c, fp->cnt, pcap, pndev, stop, fp->csum);
+            if (rc1 != GV_OK) return rc1;
+            IsectArgs ia;
+            memset(&ia, 0, sizeof(ia));
+            ia.g = dg;
+            ia.rows = prows;
+            ia.nrows = pcap;
+            ia.nrows_dev = pndev;
+            ia.stop = stop;
+            ia.L = w - 1;
+            ia.anc_beg = fp->anc_beg;
+            ia.cnt = fp->cnt;
+            ia.anc_i = fp->anc_i;
+            ia.csum = fp->csum;
+            ia.out = c->isect;
+            ia.out_cap = c->isect_cap;
+            ia.len = c->ilen;
+            ia.overflow = stop;
+            ia.st = pre;
+            hipLaunchKernelGGL(k_isect, dim3(c->num_cu * 16), dim3(kBlock), 0, c->stream, ia);
+            ChildSizeArgs ca;
+            memset(&ca, 0, sizeof(ca));
+            ca.g = dg;
+            ca.rows = rows;
+            ca.nrows = cap;
+            ca.nrows_dev = ndev;
+            ca.stop = stop;
+            ca.L = w;
+            ca.par = c->par;
+            ca.pcnt = fp->cnt;
+            ca.pcsum = fp->csum;
+            ca.ilen = c->ilen;
+            ca.isect = c->isect;
+            ca.anc_beg = f->anc_beg;
+            ca.cnt = f->cnt;
+            ca.anc_i = f->anc_i;
+            ca.tile_sums = c->tiles;
+            ca.bytes = c->acc + kAccLine0 + 1;
+            ca.st = st;
+            ca.cut_last = cut_last;
+            hipLaunchKernelGGL(k_child_size, dim3(grid_for(cap, kSizeTile, c->num_cu * 16)), dim3(kBlock),
+                               0, c->stream, ca);
+            HIPCHK(hipGetLastError());
+        } else {
             SizeArgs a;
             memset(&a, 0, sizeof(a));
             a.g = dg;
@@ -1457,7 +1755,9 @@ int gv_mine_levels(gv_ctx_t c, gv_graph_t g, const gv_query_t* q, const uint32_t
         a.count = c->acc + kAccLine0;
         a.bytes = c->acc + kAccLine0 + 1;
         a.overflow = stop;
-        a.st = st;
+        a.st = split_here ? child : st;
+        a.anc_base = split_here ? c->isect : nullptr;
+        if (split && w + 1 == wlast) a.par = c->par;
         const bool counting = last || fused;
         if (counting) {
             if (tail) {
