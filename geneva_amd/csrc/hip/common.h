@@ -1086,6 +1086,9 @@ struct gv_ctx {
     uint64_t isect_cap = 0;
     uint32_t* ilen = nullptr;
     uint64_t ilen_cap = 0;
+    uint32_t* pmark = nullptr;            // parent p has rows when pmark[p] == pmark_epoch
+    uint64_t pmark_cap = 0;
+    uint32_t pmark_epoch = 0;
     uint32_t chain_epoch = 0;
 };
 

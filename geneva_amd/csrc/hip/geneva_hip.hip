@@ -333,6 +333,9 @@ __global__ __launch_bounds__(kBlock) void k_count_chain(ChainCountArgs a) {
 // with the list of column L - 1 (k_child_size sizes it, the counting walk
 // reads its candidates from the intersections).  The algorithmic bytes stay
 // those of u's step over the full lists, added by k_child_size.
+constexpr uint32_t kIsectWave = 256;    // larger parents: k_isect_big
+constexpr uint32_t kIsectLds = 8192;    // LDS entries of k_isect_big's lists
+
 struct IsectArgs {
     DevGraph g;
     const uint32_t* rows;       // parent table, width L
@@ -348,6 +351,8 @@ struct IsectArgs {
     uint64_t out_cap;
     uint32_t* len;              // entries of each parent's intersection
     unsigned long long* overflow;
+    const uint32_t* mark;       // only parents with mark[p] == epoch have rows
+    uint32_t epoch;
     gv_step_t st;               // the prefix step: prefix lists, same-label prefix columns
 };
 
@@ -367,6 +372,7 @@ __global__ __launch_bounds__(kBlock) void k_isect(IsectArgs a) {
     for (uint64_t p = wave; p < n; p += nwaves) {
         const uint32_t* row = a.rows + p * a.L;
         const uint32_t c = a.cnt[p], ab = a.anc_beg[p], skip = a.anc_i[p];
+        if (c > kIsectWave || a.mark[p] != a.epoch) continue;   // k_isect_big; no rows
         const uint64_t o = a.csum[p] - c;
         uint32_t ob[3] = {0, 0, 0}, oe[3] = {0, 0, 0}, no = 0;
         for (uint32_t i = 0; i < a.st.nback; ++i) {
@@ -402,6 +408,92 @@ __global__ __launch_bounds__(kBlock) void k_isect(IsectArgs a) {
             written += (uint32_t)__popcll(m);
         }
         if (lane == 0) a.len[p] = written;
+    }
+}
+
+// Parents with more than kIsectWave anchor entries: one workgroup each, the
+// other prefix lists staged in LDS when they fit (kIsectLds entries in all),
+// the 4 waves taking 64-entry chunks in rounds and writing their survivors in
+// chunk order (a prefix of the waves' counts per round), so the intersection
+// stays sorted.
+__global__ __launch_bounds__(kBlock) void k_isect_big(IsectArgs a) {
+    __shared__ uint32_t lst[kIsectLds];
+    __shared__ uint32_t wcnt[kWavesPerBlock];
+    const unsigned lane = lane_id(), wib = threadIdx.x >> 6;
+    const uint64_t n = dev_rows(a.nrows, a.nrows_dev, a.stop);
+    if (n == 0 || a.csum[n - 1] > a.out_cap) return;   // k_isect flags the overflow
+    for (uint64_t p = blockIdx.x; p < n; p += gridDim.x) {
+        const uint32_t c = a.cnt[p];
+        if (c <= kIsectWave || a.mark[p] != a.epoch) continue;
+        const uint32_t* row = a.rows + p * a.L;
+        const uint32_t ab = a.anc_beg[p], skip = a.anc_i[p];
+        const uint64_t o = a.csum[p] - c;
+        uint32_t ob[3] = {0, 0, 0}, oe[3] = {0, 0, 0}, no = 0;
+        for (uint32_t i = 0; i < a.st.nback; ++i) {
+            if (i == skip) continue;
+            uint32_t b, e;
+            seg(a.g, a.st.back_part[i], row[a.st.back_col[i]], a.st.label, b, e);
+            if (no == 0) {
+                ob[0] = b;
+                oe[0] = e;
+            } else if (no == 1) {
+                ob[1] = b;
+                oe[1] = e;
+            } else {
+                ob[2] = b;
+                oe[2] = e;
+            }
+            ++no;
+        }
+        // stage the other lists in LDS (one after another) when they fit
+        const uint32_t l0 = oe[0] - ob[0], l1 = oe[1] - ob[1], l2 = oe[2] - ob[2];
+        const bool in_lds = (uint64_t)l0 + l1 + l2 <= kIsectLds;
+        if (in_lds) {
+            for (uint32_t i = threadIdx.x; i < l0; i += kBlock) lst[i] = a.g.nbr[ob[0] + i];
+            for (uint32_t i = threadIdx.x; i < l1; i += kBlock) lst[l0 + i] = a.g.nbr[ob[1] + i];
+            for (uint32_t i = threadIdx.x; i < l2; i += kBlock) lst[l0 + l1 + i] = a.g.nbr[ob[2] + i];
+        }
+        __syncthreads();
+        const uint32_t cv0 = a.st.ncoll > 0 ? row[a.st.coll_col[0]] : 0xffffffffu;
+        const uint32_t cv1 = a.st.ncoll > 1 ? row[a.st.coll_col[1]] : 0xffffffffu;
+        const uint32_t cv2 = a.st.ncoll > 2 ? row[a.st.coll_col[2]] : 0xffffffffu;
+        uint32_t written = 0;
+        for (uint32_t r0 = 0; r0 < c; r0 += 64 * kWavesPerBlock) {
+            const uint32_t k = r0 + wib * 64 + lane;
+            bool ok = k < c;
+            const uint32_t v = ok ? a.g.nbr[ab + k] : 0u;
+            ok = ok && v != cv0 && v != cv1 && v != cv2;
+            if (in_lds) {
+                if (ok && no > 0) {
+                    const uint32_t q = lds_lower_bound(lst, 0, l0, v);
+                    ok = q < l0 && lst[q] == v;
+                }
+                if (ok && no > 1) {
+                    const uint32_t q = lds_lower_bound(lst, l0, l0 + l1, v);
+                    ok = q < l0 + l1 && lst[q] == v;
+                }
+                if (ok && no > 2) {
+                    const uint32_t q = lds_lower_bound(lst, l0 + l1, l0 + l1 + l2, v);
+                    ok = q < l0 + l1 + l2 && lst[q] == v;
+                }
+            } else {
+                if (ok && no > 0) ok = contains_u32(a.g.nbr, ob[0], oe[0], v);
+                if (ok && no > 1) ok = contains_u32(a.g.nbr, ob[1], oe[1], v);
+                if (ok && no > 2) ok = contains_u32(a.g.nbr, ob[2], oe[2], v);
+            }
+            const uint64_t m = __ballot(ok);
+            if (lane == 0) wcnt[wib] = (uint32_t)__popcll(m);
+            __syncthreads();
+            uint32_t before = 0, all = 0;
+            for (int i = 0; i < kWavesPerBlock; ++i) {
+                before += i < (int)wib ? wcnt[i] : 0u;
+                all += wcnt[i];
+            }
+            if (ok) a.out[o + written + before + lanes_below(m)] = v;
+            written += all;
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) a.len[p] = written;
     }
 }
 
@@ -941,6 +1033,7 @@ int gv_ctx_destroy(gv_ctx_t c) {
     if (c->par) hipFree(c->par);
     if (c->isect) hipFree(c->isect);
     if (c->ilen) hipFree(c->ilen);
+    if (c->pmark) hipFree(c->pmark);
     for (auto& r : c->recs) {
         hipEventDestroy(r.a);
         hipEventDestroy(r.b);
@@ -1499,7 +1592,18 @@ int gv_mine_levels(gv_ctx_t c, gv_graph_t g, const gv_query_t* q, const uint32_t
         int rc = ensure_u32(c, c->par, c->par_cap, table_rows[wlast]);
         if (rc == GV_OK) rc = ensure_u32(c, c->ilen, c->ilen_cap, table_rows[wlast - 1]);
         if (rc == GV_OK) rc = ensure_u32(c, c->isect, c->isect_cap, icap);
+        if (rc == GV_OK && c->pmark_cap < table_rows[wlast - 1]) {
+            rc = ensure_u32(c, c->pmark, c->pmark_cap, table_rows[wlast - 1]);
+            if (rc == GV_OK) {
+                HIPCHK(hipMemsetAsync(c->pmark, 0, c->pmark_cap * sizeof(uint32_t), c->stream));
+                c->pmark_epoch = 0;
+            }
+        }
         if (rc != GV_OK) return rc;
+        if (++c->pmark_epoch == 0) {   // marks of 2^32 queries ago could match
+            HIPCHK(hipMemsetAsync(c->pmark, 0, c->pmark_cap * sizeof(uint32_t), c->stream));
+            c->pmark_epoch = 1;
+        }
     }
     int rc0 = ensure_gbits(c, g);
     if (rc0 != GV_OK) return rc0;
@@ -1681,8 +1785,14 @@ int gv_mine_levels(gv_ctx_t c, gv_graph_t g, const gv_query_t* q, const uint32_t
             ia.out_cap = c->isect_cap;
             ia.len = c->ilen;
             ia.overflow = stop;
+            ia.mark = c->pmark;
+            ia.epoch = c->pmark_epoch;
             ia.st = pre;
+            // only the parents of some row are intersected
+            hipLaunchKernelGGL(k_chain_mark, dim3(grid_for(cap, kBlock, c->num_cu * 16)), dim3(kBlock), 0,
+                               c->stream, c->par, cap, ndev, stop, 1u, 0u, c->pmark, c->pmark_epoch);
             hipLaunchKernelGGL(k_isect, dim3(c->num_cu * 16), dim3(kBlock), 0, c->stream, ia);
+            hipLaunchKernelGGL(k_isect_big, dim3(c->num_cu * 4), dim3(kBlock), 0, c->stream, ia);
             ChildSizeArgs ca;
             memset(&ca, 0, sizeof(ca));
             ca.g = dg;
