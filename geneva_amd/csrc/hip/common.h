@@ -587,6 +587,9 @@ constexpr uint32_t kAccStride = 16;   // 128 B
 // overflow / stop, ...), then the lines from word kAccLine0: word 0 of a line
 // sums the count, word 1 the algorithmic bytes.
 constexpr uint32_t kAccLine0 = 16;
+// words 5 and 6: parents with rows listed for the prefix-shared step (small,
+// large)
+constexpr uint32_t kAccParSmall = 5, kAccParBig = 6;
 constexpr uint32_t kAccWords = kAccLine0 + kAccSlots * kAccStride;
 
 // Adds the block's sums of v0 and v1 (one value per thread) to line
@@ -1089,6 +1092,8 @@ struct gv_ctx {
     uint32_t* pmark = nullptr;            // parent p has rows when pmark[p] == pmark_epoch
     uint64_t pmark_cap = 0;
     uint32_t pmark_epoch = 0;
+    uint32_t* plist = nullptr;            // parents with rows: small ones, then large ones from plist_cap / 2
+    uint64_t plist_cap = 0;
     uint32_t chain_epoch = 0;
 };
 

@@ -351,10 +351,40 @@ struct IsectArgs {
     uint64_t out_cap;
     uint32_t* len;              // entries of each parent's intersection
     unsigned long long* overflow;
-    const uint32_t* mark;       // only parents with mark[p] == epoch have rows
-    uint32_t epoch;
+    const uint32_t* list;       // the parents to intersect (k_mark_parents)
+    const unsigned long long* nlist;
     gv_step_t st;               // the prefix step: prefix lists, same-label prefix columns
 };
+
+// The parents of the rows, each listed once (the row that marks it first
+// appends it): the small ones (at most kIsectWave anchor entries) at list,
+// the others at list + half; one atomic per wave and list.
+__global__ __launch_bounds__(kBlock) void k_mark_parents(const uint32_t* par, uint64_t nrows_max,
+                                                         const unsigned long long* nrows_dev,
+                                                         const unsigned long long* stop,
+                                                         const uint32_t* pcnt, uint32_t* mark,
+                                                         uint32_t epoch, uint32_t* list, uint64_t half,
+                                                         unsigned long long* nsmall,
+                                                         unsigned long long* nbig) {
+    const unsigned lane = lane_id();
+    const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+    const uint64_t n = dev_rows(nrows_max, nrows_dev, stop);
+    for (uint64_t b = wave * 64; b < n; b += nwaves * 64) {
+        const uint64_t r = b + lane;
+        const uint32_t p = r < n ? par[r] : 0u;
+        const bool fresh = r < n && atomicExch(&mark[p], epoch) != epoch;
+        const bool big = fresh && pcnt[p] > kIsectWave;
+        const uint64_t ms = __ballot(fresh && !big), mb = __ballot(big);
+        uint64_t bs = 0, bb = 0;
+        if (lane == 0 && ms) bs = atomicAdd(nsmall, (unsigned long long)__popcll(ms));
+        if (lane == 0 && mb) bb = atomicAdd(nbig, (unsigned long long)__popcll(mb));
+        bs = __shfl(bs, 0, 64);
+        bb = __shfl(bb, 0, 64);
+        if (fresh && !big) list[bs + lanes_below(ms)] = p;
+        if (big) list[half + bb + lanes_below(mb)] = p;
+    }
+}
 
 // one wave per parent: the anchor's (cut) entries in order, those in every
 // other prefix list and not equal to a same-label prefix value written
@@ -369,10 +399,11 @@ __global__ __launch_bounds__(kBlock) void k_isect(IsectArgs a) {
         if (wave == 0 && lane == 0) atomicOr(a.overflow, 1ull);
         return;
     }
-    for (uint64_t p = wave; p < n; p += nwaves) {
+    const uint64_t nl = *a.nlist;
+    for (uint64_t li = wave; li < nl; li += nwaves) {
+        const uint64_t p = a.list[li];
         const uint32_t* row = a.rows + p * a.L;
         const uint32_t c = a.cnt[p], ab = a.anc_beg[p], skip = a.anc_i[p];
-        if (c > kIsectWave || a.mark[p] != a.epoch) continue;   // k_isect_big; no rows
         const uint64_t o = a.csum[p] - c;
         uint32_t ob[3] = {0, 0, 0}, oe[3] = {0, 0, 0}, no = 0;
         for (uint32_t i = 0; i < a.st.nback; ++i) {
@@ -422,9 +453,10 @@ __global__ __launch_bounds__(kBlock) void k_isect_big(IsectArgs a) {
     const unsigned lane = lane_id(), wib = threadIdx.x >> 6;
     const uint64_t n = dev_rows(a.nrows, a.nrows_dev, a.stop);
     if (n == 0 || a.csum[n - 1] > a.out_cap) return;   // k_isect flags the overflow
-    for (uint64_t p = blockIdx.x; p < n; p += gridDim.x) {
+    const uint64_t nl = *a.nlist;
+    for (uint64_t li = blockIdx.x; li < nl; li += gridDim.x) {
+        const uint64_t p = a.list[li];
         const uint32_t c = a.cnt[p];
-        if (c <= kIsectWave || a.mark[p] != a.epoch) continue;
         const uint32_t* row = a.rows + p * a.L;
         const uint32_t ab = a.anc_beg[p], skip = a.anc_i[p];
         const uint64_t o = a.csum[p] - c;
@@ -1034,6 +1066,7 @@ int gv_ctx_destroy(gv_ctx_t c) {
     if (c->isect) hipFree(c->isect);
     if (c->ilen) hipFree(c->ilen);
     if (c->pmark) hipFree(c->pmark);
+    if (c->plist) hipFree(c->plist);
     for (auto& r : c->recs) {
         hipEventDestroy(r.a);
         hipEventDestroy(r.b);
@@ -1592,6 +1625,7 @@ int gv_mine_levels(gv_ctx_t c, gv_graph_t g, const gv_query_t* q, const uint32_t
         int rc = ensure_u32(c, c->par, c->par_cap, table_rows[wlast]);
         if (rc == GV_OK) rc = ensure_u32(c, c->ilen, c->ilen_cap, table_rows[wlast - 1]);
         if (rc == GV_OK) rc = ensure_u32(c, c->isect, c->isect_cap, icap);
+        if (rc == GV_OK) rc = ensure_u32(c, c->plist, c->plist_cap, 2 * table_rows[wlast - 1]);
         if (rc == GV_OK && c->pmark_cap < table_rows[wlast - 1]) {
             rc = ensure_u32(c, c->pmark, c->pmark_cap, table_rows[wlast - 1]);
             if (rc == GV_OK) {
@@ -1785,13 +1819,17 @@ int gv_mine_levels(gv_ctx_t c, gv_graph_t g, const gv_query_t* q, const uint32_t
             ia.out_cap = c->isect_cap;
             ia.len = c->ilen;
             ia.overflow = stop;
-            ia.mark = c->pmark;
-            ia.epoch = c->pmark_epoch;
             ia.st = pre;
             // only the parents of some row are intersected
-            hipLaunchKernelGGL(k_chain_mark, dim3(grid_for(cap, kBlock, c->num_cu * 16)), dim3(kBlock), 0,
-                               c->stream, c->par, cap, ndev, stop, 1u, 0u, c->pmark, c->pmark_epoch);
+            const uint64_t half = c->plist_cap / 2;
+            hipLaunchKernelGGL(k_mark_parents, dim3(grid_for(cap, kBlock, c->num_cu * 16)), dim3(kBlock), 0,
+                               c->stream, c->par, cap, ndev, stop, fp->cnt, c->pmark, c->pmark_epoch,
+                               c->plist, half, c->acc + kAccParSmall, c->acc + kAccParBig);
+            ia.list = c->plist;
+            ia.nlist = c->acc + kAccParSmall;
             hipLaunchKernelGGL(k_isect, dim3(c->num_cu * 16), dim3(kBlock), 0, c->stream, ia);
+            ia.list = c->plist + half;
+            ia.nlist = c->acc + kAccParBig;
             hipLaunchKernelGGL(k_isect_big, dim3(c->num_cu * 4), dim3(kBlock), 0, c->stream, ia);
             ChildSizeArgs ca;
             memset(&ca, 0, sizeof(ca));
