@@ -356,9 +356,12 @@ struct IsectArgs {
     gv_step_t st;               // the prefix step: prefix lists, same-label prefix columns
 };
 
-// The parents of the rows, each listed once (the row that marks it first
-// appends it): the small ones (at most kIsectWave anchor entries) at list,
-// the others at list + half; one atomic per wave and list.
+// The parents of the rows, listed for intersection: the small ones (at most
+// kIsectWave anchor entries) at list, the others at list + half; one atomic
+// per wave and list.  A row lists its parent unless the row before has the
+// same parent or the parent is already marked; rows of one parent are mostly
+// adjacent, and a parent that two rows list at once is intersected twice into
+// the same place, which is harmless.
 __global__ __launch_bounds__(kBlock) void k_mark_parents(const uint32_t* par, uint64_t nrows_max,
                                                          const unsigned long long* nrows_dev,
                                                          const unsigned long long* stop,
@@ -373,7 +376,8 @@ __global__ __launch_bounds__(kBlock) void k_mark_parents(const uint32_t* par, ui
     for (uint64_t b = wave * 64; b < n; b += nwaves * 64) {
         const uint64_t r = b + lane;
         const uint32_t p = r < n ? par[r] : 0u;
-        const bool fresh = r < n && atomicExch(&mark[p], epoch) != epoch;
+        bool fresh = r < n && (r == 0 || par[r - 1] != p) && mark[p] != epoch;
+        if (fresh) mark[p] = epoch;
         const bool big = fresh && pcnt[p] > kIsectWave;
         const uint64_t ms = __ballot(fresh && !big), mb = __ballot(big);
         uint64_t bs = 0, bb = 0;
@@ -1830,7 +1834,7 @@ int gv_mine_levels(gv_ctx_t c, gv_graph_t g, const gv_query_t* q, const uint32_t
             hipLaunchKernelGGL(k_isect, dim3(c->num_cu * 16), dim3(kBlock), 0, c->stream, ia);
             ia.list = c->plist + half;
             ia.nlist = c->acc + kAccParBig;
-            hipLaunchKernelGGL(k_isect_big, dim3(c->num_cu * 4), dim3(kBlock), 0, c->stream, ia);
+            hipLaunchKernelGGL(k_isect_big, dim3(c->num_cu * 8), dim3(kBlock), 0, c->stream, ia);
             ChildSizeArgs ca;
             memset(&ca, 0, sizeof(ca));
             ca.g = dg;
