@@ -153,7 +153,9 @@ PMC_KINDS = [("k_walk<true, false>", "k_walk<count>"), ("k_walk<false, false>", 
              ("k_size<false>", "k_size"), ("k_count_rows<false>", "k_count_rows"),
              ("k_count_leaves<false>", "k_count_rows"), ("k_count_chain<false>", "k_count_rows"),
              ("k_chain_mark", "k_count_rows"), ("k_chain_weights", "k_count_rows"),
-             ("k_scan_tiles", "k_scan"), ("k_scan_final", "k_scan")]
+             ("k_scan_tiles", "k_scan"), ("k_scan_final", "k_scan"),
+             ("k_child_size", "k_isect"), ("k_mark_parents", "k_isect"), ("k_isect", "k_isect"),
+             ("k_isect_big", "k_isect")]
 
 
 def pmc_summary(paths, workload, cal_mib):
@@ -198,7 +200,8 @@ def pmc_summary(paths, workload, cal_mib):
         # a launch of the scan kind is its two kernels; the row counters are
         # counted by their counting kernel (the chain's marking and weights
         # belong to the chain count)
-        if kern != "k_scan_tiles" and not kern.startswith("k_chain_"):
+        if kern not in ("k_scan_tiles", "k_mark_parents", "k_isect", "k_isect_big") \
+                and not kern.startswith("k_chain_"):
             k["launches"] += len(f[kern])
     print("| Kind | Kernels (dispatches) | FETCH_SIZE KiB / launch | WRITE_SIZE KiB / launch | HBM bytes / launch |")
     print("|---|---|---|---|---|")

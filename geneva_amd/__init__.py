@@ -251,7 +251,7 @@ class Engine:
         return lib().gh_engine_graph_bytes(self._h)
 
     KERNELS = {0: "k_size", 1: "k_scan", 2: "k_walk<extend>", 3: "k_walk<count>",
-               4: "batch_iteration", 5: "k_count_rows"}
+               4: "batch_iteration", 5: "k_count_rows", 6: "k_isect"}
 
     def batch_config(self, slots=0, slot_bytes=0, row_cap=0, ipt=0, blocks_per_cu=0):
         """Configure the batch engine (0 keeps a setting); rebuilt on next use."""

@@ -1145,7 +1145,7 @@ DevGraph dev_view(const gv_graph* g) {
 }
 
 enum { kKindSize = 0, kKindScan = 1, kKindExt = 2, kKindCount = 3, kKindBatch = 4, kKindRows = 5,
-       kKinds = 6 };
+       kKindIsect = 6, kKinds = 7 };
 
 hipEvent_t pool_get(gv_ctx* c) {
     if (!c->pool.empty()) {

@@ -1801,10 +1801,12 @@ int gv_mine_levels(gv_ctx_t c, gv_graph_t g, const gv_query_t* q, const uint32_t
             int rc1 = ensure_tiles(c, std::max<uint64_t>(pcap, cap));
             if (rc1 != GV_OK) return rc1;
             sa.tile_sums = c->tiles;
-            ProfScope ps(c, kKindSize);
-            hipLaunchKernelGGL(k_size<false>, dim3(grid_for(pcap, kSizeTile, c->num_cu * 16)), dim3(kBlock),
-                               0, c->stream, sa);
-            HIPCHK(hipGetLastError());
+            {
+                ProfScope ps(c, kKindIsect);
+                hipLaunchKernelGGL(k_size<false>, dim3(grid_for(pcap, kSizeTile, c->num_cu * 16)),
+                                   dim3(kBlock), 0, c->stream, sa);
+                HIPCHK(hipGetLastError());
+            }
             rc1 = scan_counts(c, fp->cnt, pcap, pndev, stop, fp->csum);
             if (rc1 != GV_OK) return rc1;
             IsectArgs ia;
@@ -1825,6 +1827,7 @@ int gv_mine_levels(gv_ctx_t c, gv_graph_t g, const gv_query_t* q, const uint32_t
             ia.overflow = stop;
             ia.st = pre;
             // only the parents of some row are intersected
+            ProfScope ps(c, kKindIsect);
             const uint64_t half = c->plist_cap / 2;
             hipLaunchKernelGGL(k_mark_parents, dim3(grid_for(cap, kBlock, c->num_cu * 16)), dim3(kBlock), 0,
                                c->stream, c->par, cap, ndev, stop, fp->cnt, c->pmark, c->pmark_epoch,

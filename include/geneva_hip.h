@@ -307,7 +307,9 @@ int gv_batch_info(gv_batch_t b, uint32_t* slots, uint64_t* slot_bytes, uint32_t*
  * (launch variants that carry that arithmetic); 0: off.  Kinds: 0 sizing
  * (k_size), 1 scan (k_scan_*), 2 extension (k_walk, writing rows), 3 counting
  * (k_walk, last step), 4 batch iteration (one graph launch), 5 per-row
- * counting (k_count_rows, k_count_leaves).  read synchronises and returns the
+ * counting (k_count_rows, k_count_leaves), 6 parent intersections of a
+ * prefix-shared counted step (their sizing, k_isect, k_child_size; no work
+ * units or requested bytes).  read synchronises and returns the
  * launch count, the summed device milliseconds and the requested bytes since
  * the last reset. */
 #define GV_KIND_SIZE 0
@@ -316,6 +318,7 @@ int gv_batch_info(gv_batch_t b, uint32_t* slots, uint64_t* slot_bytes, uint32_t*
 #define GV_KIND_COUNT 3
 #define GV_KIND_BATCH 4
 #define GV_KIND_ROWS 5
+#define GV_KIND_ISECT 6
 int gv_profile_enable(gv_ctx_t ctx, int on);
 int gv_profile_reset(gv_ctx_t ctx);
 int gv_profile_read(gv_ctx_t ctx, int kind, uint64_t* launches, double* total_ms,
