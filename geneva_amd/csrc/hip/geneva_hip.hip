@@ -369,24 +369,37 @@ __global__ __launch_bounds__(kBlock) void k_mark_parents(const uint32_t* par, ui
                                                          uint32_t epoch, uint32_t* list, uint64_t half,
                                                          unsigned long long* nsmall,
                                                          unsigned long long* nbig) {
-    const unsigned lane = lane_id();
-    const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+    __shared__ uint32_t wn[2][kWavesPerBlock];
+    __shared__ uint64_t wb[2][kWavesPerBlock];
+    const unsigned lane = lane_id(), wib = threadIdx.x >> 6;
     const uint64_t n = dev_rows(nrows_max, nrows_dev, stop);
-    for (uint64_t b = wave * 64; b < n; b += nwaves * 64) {
-        const uint64_t r = b + lane;
+    // block rounds of kBlock rows: the block's waves share one atomic per list
+    for (uint64_t b0 = (uint64_t)blockIdx.x * kBlock; b0 < n; b0 += (uint64_t)gridDim.x * kBlock) {
+        const uint64_t r = b0 + threadIdx.x;
         const uint32_t p = r < n ? par[r] : 0u;
         bool fresh = r < n && (r == 0 || par[r - 1] != p) && mark[p] != epoch;
         if (fresh) mark[p] = epoch;
         const bool big = fresh && pcnt[p] > kIsectWave;
         const uint64_t ms = __ballot(fresh && !big), mb = __ballot(big);
-        uint64_t bs = 0, bb = 0;
-        if (lane == 0 && ms) bs = atomicAdd(nsmall, (unsigned long long)__popcll(ms));
-        if (lane == 0 && mb) bb = atomicAdd(nbig, (unsigned long long)__popcll(mb));
-        bs = __shfl(bs, 0, 64);
-        bb = __shfl(bb, 0, 64);
-        if (fresh && !big) list[bs + lanes_below(ms)] = p;
-        if (big) list[half + bb + lanes_below(mb)] = p;
+        if (lane == 0) {
+            wn[0][wib] = (uint32_t)__popcll(ms);
+            wn[1][wib] = (uint32_t)__popcll(mb);
+        }
+        __syncthreads();
+        if (threadIdx.x < 2) {
+            const int l = threadIdx.x;
+            uint32_t tot = 0;
+            for (int i = 0; i < kWavesPerBlock; ++i) tot += wn[l][i];
+            uint64_t base = tot ? atomicAdd(l == 0 ? nsmall : nbig, (unsigned long long)tot) : 0ull;
+            for (int i = 0; i < kWavesPerBlock; ++i) {
+                wb[l][i] = base;
+                base += wn[l][i];
+            }
+        }
+        __syncthreads();
+        if (fresh && !big) list[wb[0][wib] + lanes_below(ms)] = p;
+        if (big) list[half + wb[1][wib] + lanes_below(mb)] = p;
+        __syncthreads();
     }
 }
 
@@ -1829,7 +1842,7 @@ int gv_mine_levels(gv_ctx_t c, gv_graph_t g, const gv_query_t* q, const uint32_t
             // only the parents of some row are intersected
             ProfScope ps(c, kKindIsect);
             const uint64_t half = c->plist_cap / 2;
-            hipLaunchKernelGGL(k_mark_parents, dim3(grid_for(cap, kBlock, c->num_cu * 16)), dim3(kBlock), 0,
+            hipLaunchKernelGGL(k_mark_parents, dim3(grid_for(cap, kBlock, c->num_cu * 4)), dim3(kBlock), 0,
                                c->stream, c->par, cap, ndev, stop, fp->cnt, c->pmark, c->pmark_epoch,
                                c->plist, half, c->acc + kAccParSmall, c->acc + kAccParBig);
             ia.list = c->plist;
