@@ -647,6 +647,10 @@ struct TileOut {             // extension output
     unsigned long long* out_n;
     unsigned long long* overflow;
     uint32_t* par = nullptr;  // nullable: input row of every output row
+    // ~0: survivors are compacted through reservations; otherwise every
+    // candidate survives (one back list, no same-label column) and candidate
+    // c is output row c - origin, so no reservation is made
+    uint64_t origin = ~0ull;
 };
 
 struct TileAcc {
@@ -1008,7 +1012,7 @@ __device__ __forceinline__ void walk_tile(const DevGraph& g, const gv_step_t& st
     uint32_t tot = 0;
 #pragma unroll
     for (int it = 0; it < IPT; ++it) tot += (uint32_t)__popcll(__ballot(okk[it]));
-    uint64_t base = reserve(o.out_n, tot);
+    uint64_t base = o.origin != ~0ull ? t0 - o.origin : reserve(o.out_n, tot);
     if (tot == 0) return;
     if (base + tot > o.cap) {
         if (lane == 0) atomicOr(o.overflow, 1ull);

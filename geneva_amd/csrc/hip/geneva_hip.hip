@@ -863,6 +863,13 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) 
     in.row_lo = 0;
     in.row_hi = nrows;
     TileOut o{a.out, a.out_cap, a.out_n, a.overflow, a.par};
+    // a step with one back list and no same-label column keeps every
+    // candidate: rows go to their candidate index, the count is the total
+    const bool direct = !kCount && a.st.nback == 1 && a.st.ncoll == 0;
+    if (direct) {
+        o.origin = c0;
+        if (blockIdx.x == 0 && threadIdx.x == 0 && c1 > c0) atomicAdd(a.out_n, (unsigned long long)(c1 - c0));
+    }
     TileAcc acc;
     __shared__ uint64_t resv[3 * kWavesPerBlock];
     const BlockReserve reserve{resv};
@@ -877,7 +884,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) 
             walk_tile<kCount, kIPT>(a.g, a.st, a.tail, a.has_tail != 0, in, t0, t1, o,
                                     kCount ? nullptr : wl[wib], wl[wib], wl[wib] + kTile + 1, a.gbits, acc,
                                     reserve);
-        } else if (!kCount) {
+        } else if (!kCount && !direct) {
             reserve(nullptr, 0);
         }
     }
