@@ -34,6 +34,10 @@ constexpr int kBlock = 256;             // 4 wave64 per workgroup
 constexpr int kWavesPerBlock = kBlock / 64;
 constexpr int kIPT = 4;                 // candidates per lane per tile
 constexpr uint64_t kTile = 64 * kIPT;   // candidates per wave tile
+// candidates per lane of a whole query's extension walk over a step with one
+// back list: with no membership test a candidate costs little, and smaller
+// tiles spread a launch of a few million candidates over more waves
+constexpr int kSingleIPT = 2;
 constexpr int kScanItems = 16;          // per thread in the scan
 constexpr int kSizeRows = 4;            // rows per thread in a sizing pass
 constexpr uint32_t kWin = 256;          // LDS window of a back-neighbour list per wave

@@ -838,9 +838,10 @@ struct WalkArgs {
 // Eight waves per SIMD (at most 64 VGPRs): the walk waits on dependent loads,
 // and more waves in flight hide more of that latency than the few registers
 // the limit spills cost.
-template <bool kCount, bool kProf>
+template <bool kCount, bool kProf, int IPT = kIPT>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_walk(WalkArgs a) {
-    __shared__ uint32_t wl[kWavesPerBlock][wave_lds_words<kIPT>()];
+    __shared__ uint32_t wl[kWavesPerBlock][wave_lds_words<IPT>()];
+    constexpr uint64_t kT = 64 * IPT;
     const unsigned lane = lane_id();
     const unsigned wib = threadIdx.x >> 6;
     uint64_t nrows = a.nrows, c0 = a.c0, c1 = a.c1;
@@ -876,13 +877,13 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) 
     // block rounds: wave wib of block b takes tile (round * blocks + b) * 4 + wib,
     // and every wave of the block runs every round (the reservation is shared)
     const uint64_t nb = gridDim.x;
-    for (uint64_t T0 = (uint64_t)blockIdx.x * kWavesPerBlock; c0 + T0 * kTile < c1;
+    for (uint64_t T0 = (uint64_t)blockIdx.x * kWavesPerBlock; c0 + T0 * kT < c1;
          T0 += nb * kWavesPerBlock) {
-        const uint64_t t0 = c0 + (T0 + wib) * kTile;
+        const uint64_t t0 = c0 + (T0 + wib) * kT;
         if (t0 < c1) {
-            const uint64_t t1 = t0 + kTile < c1 ? t0 + kTile : c1;
-            walk_tile<kCount, kIPT>(a.g, a.st, a.tail, a.has_tail != 0, in, t0, t1, o,
-                                    kCount ? nullptr : wl[wib], wl[wib], wl[wib] + kTile + 1, a.gbits, acc,
+            const uint64_t t1 = t0 + kT < c1 ? t0 + kT : c1;
+            walk_tile<kCount, IPT>(a.g, a.st, a.tail, a.has_tail != 0, in, t0, t1, o,
+                                    kCount ? nullptr : wl[wib], wl[wib], wl[wib] + kT + 1, a.gbits, acc,
                                     reserve);
         } else if (!kCount && !direct) {
             reserve(nullptr, 0);
@@ -1954,8 +1955,12 @@ int gv_mine_levels(gv_ctx_t c, gv_graph_t g, const gv_query_t* q, const uint32_t
                 hipLaunchKernelGGL((k_walk<true, true>), grid, dim3(kBlock), 0, c->stream, a);
             else if (counting)
                 hipLaunchKernelGGL((k_walk<true, false>), grid, dim3(kBlock), 0, c->stream, a);
+            else if (c->prof_bytes && a.st.nback == 1)
+                hipLaunchKernelGGL((k_walk<false, true, kSingleIPT>), grid, dim3(kBlock), 0, c->stream, a);
             else if (c->prof_bytes)
                 hipLaunchKernelGGL((k_walk<false, true>), grid, dim3(kBlock), 0, c->stream, a);
+            else if (a.st.nback == 1)
+                hipLaunchKernelGGL((k_walk<false, false, kSingleIPT>), grid, dim3(kBlock), 0, c->stream, a);
             else
                 hipLaunchKernelGGL((k_walk<false, false>), grid, dim3(kBlock), 0, c->stream, a);
             HIPCHK(hipGetLastError());
