@@ -149,7 +149,7 @@ def batched_and_served(args):
 
 
 # kernel (lean variant, as rocprofv3 names it) -> the kind bench.py reports
-PMC_KINDS = [("k_walk<true, false, 4>", "k_walk<count>"), ("k_walk<false, false, 4>", "k_walk<extend>"),
+PMC_KINDS = [("k_walk<true, false, 2>", "k_walk<count>"), ("k_walk<true, false, 4>", "k_walk<count>"), ("k_walk<false, false, 4>", "k_walk<extend>"),
              ("k_walk<false, false, 2>", "k_walk<extend>"),
              ("k_size<false>", "k_size"), ("k_count_rows<false>", "k_count_rows"),
              ("k_count_leaves<false>", "k_count_rows"), ("k_count_chain<false>", "k_count_rows"),

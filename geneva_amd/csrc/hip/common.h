@@ -38,6 +38,9 @@ constexpr uint64_t kTile = 64 * kIPT;   // candidates per wave tile
 // back list: with no membership test a candidate costs little, and smaller
 // tiles spread a launch of a few million candidates over more waves
 constexpr int kSingleIPT = 2;
+// candidates per lane of a whole query's counting walk (measured: 2 beats 4
+// and 8 on the 5-cycle and matches 4 elsewhere)
+constexpr int kCountIPT = 2;
 constexpr int kScanItems = 16;          // per thread in the scan
 constexpr int kSizeRows = 4;            // rows per thread in a sizing pass
 constexpr uint32_t kWin = 256;          // LDS window of a back-neighbour list per wave

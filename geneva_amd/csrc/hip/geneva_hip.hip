@@ -1952,9 +1952,9 @@ int gv_mine_levels(gv_ctx_t c, gv_graph_t g, const gv_query_t* q, const uint32_t
             ProfScope ps(c, counting ? kKindCount : kKindExt);
             const dim3 grid(c->num_cu * 32);
             if (counting && c->prof_bytes)
-                hipLaunchKernelGGL((k_walk<true, true>), grid, dim3(kBlock), 0, c->stream, a);
+                hipLaunchKernelGGL((k_walk<true, true, kCountIPT>), grid, dim3(kBlock), 0, c->stream, a);
             else if (counting)
-                hipLaunchKernelGGL((k_walk<true, false>), grid, dim3(kBlock), 0, c->stream, a);
+                hipLaunchKernelGGL((k_walk<true, false, kCountIPT>), grid, dim3(kBlock), 0, c->stream, a);
             else if (c->prof_bytes && a.st.nback == 1)
                 hipLaunchKernelGGL((k_walk<false, true, kSingleIPT>), grid, dim3(kBlock), 0, c->stream, a);
             else if (c->prof_bytes)
