@@ -42,7 +42,7 @@ constexpr int kSingleIPT = 2;
 // and 8 on the 5-cycle and matches 4 elsewhere)
 constexpr int kCountIPT = 2;
 constexpr int kScanItems = 16;          // per thread in the scan
-constexpr int kSizeRows = 4;            // rows per thread in a sizing pass
+constexpr int kSizeRows = 2;            // rows per thread in a sizing pass
 constexpr uint32_t kWin = 256;          // LDS window of a back-neighbour list per wave
 constexpr uint64_t kSizeTile = (uint64_t)kBlock * kSizeRows;   // rows per sizing tile
 constexpr int kScanTile = kBlock * kScanItems;
