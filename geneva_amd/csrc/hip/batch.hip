@@ -19,6 +19,11 @@
 
 namespace {
 
+// Rows per thread of the engine's sizing passes.  Its launches size many
+// slots at once and are large, so four rows per thread with their loads
+// interleaved beat two here (the plain path's single-query levels use two).
+constexpr int kBSizeRows = 4;
+constexpr uint64_t kBSizeTile = (uint64_t)kBlock * kBSizeRows;
 constexpr uint32_t kMaxSlots = 1024;
 
 struct BDesc {               // written by the host at admission
@@ -47,7 +52,7 @@ struct BState {              // dynamic, polled by the host (mirrors gv_slot_inf
     uint64_t cand_base;      // first candidate of this slot in the iteration's space
     uint64_t cand_end;
     uint64_t tile_end;       // end of this slot's tiles (tiles never straddle slots)
-    uint64_t rtile_end;      // end of this slot's sizing tiles (kSizeTile rows each)
+    uint64_t rtile_end;      // end of this slot's sizing tiles (kBSizeTile rows each)
 };
 
 struct BArgs {
@@ -167,8 +172,8 @@ __global__ __launch_bounds__(1024) void k_b_prep(BArgs a) {
     }
     ex = block_scan_1024(rows, wsum, tot);
     const uint64_t total_rows = tot;
-    // sizing tiles: kSizeTile rows of one slot each
-    const uint64_t nrt = (rows + kSizeTile - 1) / kSizeTile;
+    // sizing tiles: kBSizeTile rows of one slot each
+    const uint64_t nrt = (rows + kBSizeTile - 1) / kBSizeTile;
     uint64_t rtot;
     const uint64_t rex = block_scan_1024(nrt, wsum, rtot);
     if (s < a.nslots) {
@@ -211,15 +216,15 @@ __global__ __launch_bounds__(kBlock) void k_b_size(BArgs a) {
         const BDesc& D = a.desc[s];
         const uint32_t L = S.width;
         const uint64_t rows = L == 1 ? D.start_n : S.rows;
-        const uint64_t nrt = (rows + kSizeTile - 1) / kSizeTile;
-        const uint64_t tile0 = (RT - (rtile_end[s] - nrt)) * kSizeTile;
+        const uint64_t nrt = (rows + kBSizeTile - 1) / kBSizeTile;
+        const uint64_t tile0 = (RT - (rtile_end[s] - nrt)) * kBSizeTile;
         const uint64_t rbase = S.row_end - rows;   // slot's first row in the iteration space
         const gv_step_t& st = D.q.step[L];
         const uint32_t* trows = slot_rows(a, s, S, D);
-        uint64_t li[kSizeRows];
-        bool ok[kSizeRows];
+        uint64_t li[kBSizeRows];
+        bool ok[kBSizeRows];
 #pragma unroll
-        for (int k = 0; k < kSizeRows; ++k) {
+        for (int k = 0; k < kBSizeRows; ++k) {
             li[k] = tile0 + (uint64_t)k * kBlock + threadIdx.x;
             ok[k] = li[k] < rows;
         }
@@ -227,7 +232,7 @@ __global__ __launch_bounds__(kBlock) void k_b_size(BArgs a) {
             // leaf groups on these core rows: counted here, no candidates (a
             // row's own read is part of the leaf bytes)
 #pragma unroll
-            for (int k = 0; k < kSizeRows; ++k) {
+            for (int k = 0; k < kBSizeRows; ++k) {
                 if (!ok[k]) continue;
                 const uint64_t r = rbase + li[k];
                 const uint32_t* row = trows + li[k] * L;
@@ -248,13 +253,13 @@ __global__ __launch_bounds__(kBlock) void k_b_size(BArgs a) {
             }
             continue;
         }
-        uint32_t beg[kSizeRows], len[kSizeRows], cnt[kSizeRows], ai[kSizeRows];
+        uint32_t beg[kBSizeRows], len[kBSizeRows], cnt[kBSizeRows], ai[kBSizeRows];
         uint64_t by = 0;
-        size_rows<kSizeRows>(a.g, st, trows, L, li, ok, beg, len, cnt, ai, by, rq,
+        size_rows<kBSizeRows>(a.g, st, trows, L, li, ok, beg, len, cnt, ai, by, rq,
                              L == 1 ? (int64_t)D.start_off : (int64_t)-1);
         acc += by;
 #pragma unroll
-        for (int k = 0; k < kSizeRows; ++k) {
+        for (int k = 0; k < kBSizeRows; ++k) {
             if (!ok[k]) continue;
             const uint64_t r = rbase + li[k];
             a.anc_beg[r] = beg[k];

@@ -334,7 +334,7 @@ __global__ __launch_bounds__(kBlock) void k_count_chain(ChainCountArgs a) {
 // reads its candidates from the intersections).  The algorithmic bytes stay
 // those of u's step over the full lists, added by k_child_size.
 constexpr uint32_t kIsectWave = 256;    // larger parents: k_isect_big
-constexpr uint32_t kIsectLds = 8192;    // LDS entries of k_isect_big's lists
+constexpr uint32_t kIsectLds = 4096;    // LDS entries of k_isect_big's lists (16 KB: 8 blocks a CU)
 
 struct IsectArgs {
     DevGraph g;
