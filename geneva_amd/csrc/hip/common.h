@@ -729,7 +729,15 @@ __device__ __forceinline__ void walk_tile(const DevGraph& g, const gv_step_t& st
     const unsigned lane = __lane_id();
     const uint32_t L = in.L, W = L + 1;
     const uint64_t r_lo = wave_find_row(in.csum, in.row_lo, in.row_hi, t0);
-    const uint64_t r_hi = wave_find_row(in.csum, r_lo, in.row_hi, t1 - 1);
+    // the tile's last row is mostly within 64 rows of its first: one round of
+    // 64 consecutive rows, then the wide search past them (when none of them
+    // ends after t1 - 1, r_lo + 64 < row_hi, since csum[row_hi - 1] does)
+    uint64_t r_hi;
+    {
+        const uint64_t x = r_lo + lane;
+        const uint64_t m = __ballot(x < in.row_hi && in.csum[x] > t1 - 1);
+        r_hi = m ? r_lo + (uint64_t)__builtin_ctzll(m) : wave_find_row(in.csum, r_lo + 64, in.row_hi, t1 - 1);
+    }
     auto rowp = [&](uint64_t r) { return in.rows + (r - in.row_base) * L; };
 
     // anchors (back-edge indices) whose segment is also the fused last
