@@ -75,7 +75,7 @@ def lib():
         "gh_engine_autotune": (i32, [vp, C.POINTER(cs), u32, C.POINTER(u32), C.POINTER(u32),
                                      C.POINTER(dbl)]),
         "gh_engine_serve": (i32, [vp, C.POINTER(cs), C.POINTER(dbl), u32, i32, u64, C.POINTER(dbl),
-                                  C.POINTER(u64), C.POINTER(dbl)]),
+                                  C.POINTER(u64), _u32p(), u32, C.POINTER(dbl)]),
         "gh_engine_profile": (i32, [vp, i32]),
         "gh_engine_profile_reset": (i32, [vp]),
         "gh_engine_profile_read": (i32, [vp, i32, C.POINTER(u64), C.POINTER(dbl), C.POINTER(u64)]),
@@ -251,7 +251,7 @@ class Engine:
         return lib().gh_engine_graph_bytes(self._h)
 
     KERNELS = {0: "k_size", 1: "k_scan", 2: "k_walk<extend>", 3: "k_walk<count>",
-               4: "batch_iteration", 5: "k_count_rows", 6: "k_isect"}
+               4: "batch_iteration", 5: "k_count_rows", 6: "k_isect", 7: "k_count_chain"}
 
     def batch_config(self, slots=0, slot_bytes=0, row_cap=0, ipt=0, blocks_per_cu=0):
         """Configure the batch engine (0 keeps a setting); rebuilt on next use."""
@@ -292,19 +292,29 @@ class Engine:
             raise _err("autotune", rc)
         return {"ipt": ipt.value, "blocks_per_cu": bpc.value, "device_ms": ms.value}
 
-    def serve(self, queries, arrival_s, batched=True, mem_budget=16 << 30):
-        """Serve requests arriving at arrival_s (seconds).  Returns (latency_ms, counts, stats)."""
+    def serve(self, queries, arrival_s, batched=True, mem_budget=16 << 30, with_graphs=False):
+        """Serve requests arriving at arrival_s (seconds).  Returns (latency_ms,
+        counts, stats), and with with_graphs the matched graph ids of every
+        request as a fourth item."""
         n = len(queries)
         arr = (C.c_double * max(n, 1))(*arrival_s)
         lat = (C.c_double * max(n, 1))()
         cnt = (C.c_uint64 * max(n, 1))()
-        st = (C.c_double * 5)()
+        st = (C.c_double * 6)()
+        nwords = (self.ngraphs + 31) // 32 if (with_graphs and self.ngraphs > 1) else 0
+        bits = (C.c_uint32 * max(n * nwords, 1))()
         rc = lib().gh_engine_serve(self._h, self._cstrs(queries), arr, n, 0 if batched else 1,
-                                   mem_budget, lat, cnt, st)
+                                   mem_budget, lat, cnt, bits if nwords else None, nwords, st)
         if rc:
             raise _err("serve", rc)
-        stats = dict(zip(("wall_s", "iterations", "max_concurrent", "overflowed", "direct"), list(st)))
-        return list(lat)[:n], list(cnt)[:n], stats
+        stats = dict(zip(("wall_s", "iterations", "max_concurrent", "overflowed", "direct",
+                          "max_batched"), list(st)))
+        if not with_graphs:
+            return list(lat)[:n], list(cnt)[:n], stats
+        words = np.ctypeslib.as_array(bits)
+        ids = [_bits_to_ids(words[i * nwords:(i + 1) * nwords], self.ngraphs) if nwords else frozenset()
+               for i in range(n)]
+        return list(lat)[:n], list(cnt)[:n], stats, ids
 
     def count_by_rows(self, on=True):
         """Count single-back-edge last steps per row (default) or by a candidate walk."""

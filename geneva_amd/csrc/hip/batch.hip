@@ -53,7 +53,14 @@ struct BState {              // dynamic, polled by the host (mirrors gv_slot_inf
     uint64_t cand_end;
     uint64_t tile_end;       // end of this slot's tiles (tiles never straddle slots)
     uint64_t rtile_end;      // end of this slot's sizing tiles (kBSizeTile rows each)
+    uint32_t defer;          // iterations in a row this slot waited for row space
+    uint32_t pad_;
 };
+
+// A slot that waited this many iterations in a row for row space is placed
+// before the others, so a large request in a high slot is not passed over
+// forever while lower slots keep refilling.
+constexpr uint32_t kAgeLimit = 4;
 
 struct BArgs {
     DevGraph g;
@@ -163,14 +170,25 @@ __global__ __launch_bounds__(1024) void k_b_prep(BArgs a) {
             }
         }
     }
-    // first pass: running prefix; slots whose rows end past row_cap wait
-    uint64_t tot;
-    uint64_t ex = block_scan_1024(rows, wsum, tot);
-    if (ex + rows > a.row_cap) {
+    // which slots run: those that waited kAgeLimit iterations first, then the
+    // rest, each group in slot order while the rows fit row_cap; the others
+    // wait (and age)
+    const bool aged = s < a.nslots && a.st[s].defer >= kAgeLimit;
+    uint64_t tot, tot_a;
+    const uint64_t ra = aged ? rows : 0;
+    const uint64_t ex_a = block_scan_1024(ra, wsum, tot);
+    const bool run_a = aged && ex_a + ra <= a.row_cap;
+    block_scan_1024(run_a ? rows : 0, wsum, tot_a);
+    const uint64_t rb = aged ? 0 : rows;
+    const uint64_t ex_b = block_scan_1024(rb, wsum, tot);
+    const bool run = run_a || (!aged && tot_a + ex_b + rb <= a.row_cap);
+    if (s < a.nslots && mode != 0) a.st[s].defer = run ? 0 : a.st[s].defer + 1;
+    if (!run) {
         rows = 0;
         mode = 0;
     }
-    ex = block_scan_1024(rows, wsum, tot);
+    // the running slots' row ranges, in slot order
+    uint64_t ex = block_scan_1024(rows, wsum, tot);
     const uint64_t total_rows = tot;
     // sizing tiles: kBSizeTile rows of one slot each
     const uint64_t nrt = (rows + kBSizeTile - 1) / kBSizeTile;
@@ -838,6 +856,33 @@ int gv_batch_release(gv_batch_t b, uint32_t s, uint32_t* bits, uint32_t nwords) 
         if (rc != GV_OK) return rc;
     }
     b->used[s] = 0;
+    return GV_OK;
+}
+
+int gv_batch_release_many(gv_batch_t b, uint32_t n, const uint32_t* slots, uint32_t* bits,
+                          uint32_t nwords) {
+    if (!b || (n && !slots) || (bits && nwords != b->gwords)) return GV_ERR_ARG;
+    for (uint32_t i = 0; i < n; ++i)
+        if (slots[i] >= b->nslots || !b->used[slots[i]]) return GV_ERR_ARG;
+    bool copied = false;
+    for (uint32_t i = 0; i < n; ++i) {
+        const uint32_t s = slots[i];
+        if (b->dirty[s]) {   // never reached the device: no bits, nothing to undo
+            if (bits) memset(bits + (uint64_t)i * nwords, 0, 4ull * nwords);
+            b->dirty[s] = 0;
+            b->used[s] = 0;
+            continue;
+        }
+        if (bits) {
+            HIPCHK(hipMemcpyAsync(bits + (uint64_t)i * nwords, b->d_gbits + (uint64_t)s * b->gwords,
+                                  4ull * nwords, hipMemcpyDeviceToHost, b->ctx->stream));
+            copied = true;
+        }
+        HIPCHK(hipMemcpyAsync(b->d_st + s, b->h_zero, sizeof(BState), hipMemcpyHostToDevice,
+                              b->ctx->stream));
+        b->used[s] = 0;
+    }
+    if (copied) return sync(b);
     return GV_OK;
 }
 

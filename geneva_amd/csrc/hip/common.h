@@ -1164,7 +1164,7 @@ DevGraph dev_view(const gv_graph* g) {
 }
 
 enum { kKindSize = 0, kKindScan = 1, kKindExt = 2, kKindCount = 3, kKindBatch = 4, kKindRows = 5,
-       kKindIsect = 6, kKinds = 7 };
+       kKindIsect = 6, kKindChain = 7, kKinds = 8 };
 
 hipEvent_t pool_get(gv_ctx* c) {
     if (!c->pool.empty()) {

@@ -358,10 +358,12 @@ struct IsectArgs {
 
 // The parents of the rows, listed for intersection: the small ones (at most
 // kIsectWave anchor entries) at list, the others at list + half; one atomic
-// per wave and list.  A row lists its parent unless the row before has the
-// same parent or the parent is already marked; rows of one parent are mostly
-// adjacent, and a parent that two rows list at once is intersected twice into
-// the same place, which is harmless.
+// per block round and list.  A row whose row before has another parent claims
+// its parent with an exchange on mark[p] (rows of one parent are mostly
+// adjacent, so few rows try); only the row that finds another epoch there
+// lists it.  Each parent is therefore listed at most once, whichever waves or
+// chiplets see its rows, and each list holds at most the parent count
+// (half >= the parent table's rows).
 __global__ __launch_bounds__(kBlock) void k_mark_parents(const uint32_t* par, uint64_t nrows_max,
                                                          const unsigned long long* nrows_dev,
                                                          const unsigned long long* stop,
@@ -377,8 +379,8 @@ __global__ __launch_bounds__(kBlock) void k_mark_parents(const uint32_t* par, ui
     for (uint64_t b0 = (uint64_t)blockIdx.x * kBlock; b0 < n; b0 += (uint64_t)gridDim.x * kBlock) {
         const uint64_t r = b0 + threadIdx.x;
         const uint32_t p = r < n ? par[r] : 0u;
-        bool fresh = r < n && (r == 0 || par[r - 1] != p) && mark[p] != epoch;
-        if (fresh) mark[p] = epoch;
+        const bool fresh = r < n && (r == 0 || par[r - 1] != p) && mark[p] != epoch &&
+                           atomicExch(&mark[p], epoch) != epoch;
         const bool big = fresh && pcnt[p] > kIsectWave;
         const uint64_t ms = __ballot(fresh && !big), mb = __ballot(big);
         if (lane == 0) {
@@ -1750,10 +1752,10 @@ int gv_mine_levels(gv_ctx_t c, gv_graph_t g, const gv_query_t* q, const uint32_t
             a.w = c->chainw;
             a.count = c->acc + kAccLine0;
             a.bytes = c->acc + kAccLine0 + 1;
-            a.req = req_ptr(c, kKindRows);
-            a.units = units_ptr(c, kKindRows);
+            a.req = req_ptr(c, kKindChain);
+            a.units = units_ptr(c, kKindChain);
             a.gbits = gb;
-            ProfScope ps(c, kKindRows);
+            ProfScope ps(c, kKindChain);
             hipLaunchKernelGGL(k_chain_mark, dim3(grid_for(cap, kBlock, c->num_cu * 16)), dim3(kBlock), 0,
                                c->stream, rows, cap, ndev, stop, w, acol, c->chainneed, c->chain_epoch);
             if (ca.nverts)

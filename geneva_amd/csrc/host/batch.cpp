@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <chrono>
 #include <deque>
+#include <mutex>
 
 namespace geneva {
 
@@ -86,7 +87,7 @@ int BatchMiner::run(const std::vector<CompiledQuery>& qs, std::vector<MineResult
     std::vector<size_t> slotQuery(opt_.slots, SIZE_MAX);
     std::vector<size_t> overflow;
     std::vector<gv_slot_info_t> info(opt_.slots);
-    std::vector<uint32_t> bits((size_t)opt_.slots * nwords_);
+    std::vector<uint32_t> bits;
     BatchStats local;
     uint32_t running = 0;
     gv_mark(ctx_, 2);
@@ -109,30 +110,35 @@ int BatchMiner::run(const std::vector<CompiledQuery>& qs, std::vector<MineResult
         rc = gv_batch_run(b_, iters);
         if (rc != GV_OK) return rc;
         local.graphLaunches += iters;
-        // states and every slot's graph bits in one synchronised copy
-        rc = nwords_ ? gv_batch_poll_bits(b_, info.data(), bits.data(), nwords_)
-                     : gv_batch_poll(b_, info.data());
+        rc = gv_batch_poll(b_, info.data());
         if (rc != GV_OK) return rc;
+        // finished slots: their graph bits in one synchronised copy, then free
+        std::vector<uint32_t> done, over;
         for (uint32_t s = 0; s < opt_.slots; ++s) {
             if (slotQuery[s] == SIZE_MAX) continue;
-            const size_t qi = slotQuery[s];
-            if (info[s].state == GV_SLOT_DONE) {
-                MineResult& r = out[qi];
-                r.count = info[s].count;
-                r.algBytes = info[s].alg_bytes;
-                r.graphBits.assign(bits.begin() + (size_t)s * nwords_, bits.begin() + (size_t)(s + 1) * nwords_);
-                rc = gv_batch_release(b_, s, nullptr, 0);
-                if (rc != GV_OK) return rc;
-                slotQuery[s] = SIZE_MAX;
-                --running;
-                local.batched++;
-            } else if (info[s].state == GV_SLOT_OVERFLOW) {
-                rc = gv_batch_release(b_, s, nullptr, 0);
-                if (rc != GV_OK) return rc;
-                slotQuery[s] = SIZE_MAX;
-                --running;
-                overflow.push_back(qi);
-            }
+            if (info[s].state == GV_SLOT_DONE) done.push_back(s);
+            else if (info[s].state == GV_SLOT_OVERFLOW) over.push_back(s);
+        }
+        bits.assign(done.size() * (size_t)nwords_, 0);
+        rc = gv_batch_release_many(b_, (uint32_t)done.size(), done.data(),
+                                   nwords_ ? bits.data() : nullptr, nwords_);
+        if (rc != GV_OK) return rc;
+        for (size_t i = 0; i < done.size(); ++i) {
+            const uint32_t s = done[i];
+            MineResult& r = out[slotQuery[s]];
+            r.count = info[s].count;
+            r.algBytes = info[s].alg_bytes;
+            r.graphBits.assign(bits.begin() + i * nwords_, bits.begin() + (i + 1) * nwords_);
+            slotQuery[s] = SIZE_MAX;
+            --running;
+            local.batched++;
+        }
+        rc = gv_batch_release_many(b_, (uint32_t)over.size(), over.data(), nullptr, 0);
+        if (rc != GV_OK) return rc;
+        for (uint32_t s : over) {
+            overflow.push_back(slotQuery[s]);
+            slotQuery[s] = SIZE_MAX;
+            --running;
         }
     }
     gv_mark(ctx_, 3);
@@ -154,9 +160,24 @@ int BatchMiner::run(const std::vector<CompiledQuery>& qs, std::vector<MineResult
 }
 
 namespace {
+// Tunings measured in this process, keyed by graph, engine options and the
+// sample's shape; shared by every BatchMiner, so guarded by a mutex.
+std::mutex& tuneMutex() {
+    static std::mutex m;
+    return m;
+}
 std::map<std::string, gv_tune_t>& tuneCache() {
     static std::map<std::string, gv_tune_t> m;
     return m;
+}
+std::string graphKey(gv_graph_t g, const BatchOptions& o) {
+    uint64_t nv1 = 0, nnbr = 0;
+    uint32_t nl = 0, np = 0, md = 0;
+    gv_graph_info(g, &nv1, &nl, &np, &nnbr, nullptr);
+    gv_graph_max_degree(g, &md);
+    return "g" + std::to_string(nv1) + "/" + std::to_string(nnbr) + "/" + std::to_string(nl) + "/" +
+           std::to_string(np) + "/" + std::to_string(md) + " s" + std::to_string(o.slots) + "/" +
+           std::to_string(o.slotBytes) + "/" + std::to_string(o.rowCap) + " ";
 }
 std::string shapeKey(const std::vector<CompiledQuery>& qs) {
     // histogram of query sizes and of multi-edge steps
@@ -175,11 +196,14 @@ std::string shapeKey(const std::vector<CompiledQuery>& qs) {
 
 int autotune(BatchMiner& bm, const std::vector<CompiledQuery>& sample, TuneResult& out) {
     out = TuneResult();
-    const std::string key = shapeKey(sample);
-    auto it = tuneCache().find(key);
-    if (it != tuneCache().end()) {
-        out.tune = it->second;
-        return bm.setTune(out.tune);
+    const std::string key = graphKey(bm.graph(), bm.options()) + shapeKey(sample);
+    {
+        std::lock_guard<std::mutex> l(tuneMutex());
+        auto it = tuneCache().find(key);
+        if (it != tuneCache().end()) {
+            out.tune = it->second;
+            return bm.setTune(out.tune);
+        }
     }
     const uint32_t ipts[] = {1, 2, 4, 8};
     const uint32_t bpcs[] = {4, 8, 16, 32};
@@ -204,7 +228,10 @@ int autotune(BatchMiner& bm, const std::vector<CompiledQuery>& sample, TuneResul
         }
     }
     out.bestMs = best;
-    tuneCache()[key] = out.tune;
+    {
+        std::lock_guard<std::mutex> l(tuneMutex());
+        tuneCache()[key] = out.tune;
+    }
     return bm.setTune(out.tune);
 }
 

@@ -52,7 +52,10 @@ public:
     int setTune(const gv_tune_t& t);
     const BatchOptions& options() const { return opt_; }
     gv_batch_t handle() const { return b_; }
+    gv_graph_t graph() const { return g_; }
     PlainMiner* fallback() const { return fallback_; }
+    // words of a matched-graph bitset (0 without an owner table); valid after init()
+    uint32_t graphWords() const { return nwords_; }
 
     // Mine every query; results in input order.  Queries with a plan error
     // get status GV_ERR_ARG; overflowing ones are re-run on the plain path.
@@ -82,20 +85,28 @@ int autotune(BatchMiner& bm, const std::vector<CompiledQuery>& sample, TuneResul
 // arrived, while slots are free and the summed peak-table estimate of the
 // running requests stays under memBudget, then runs one iteration, in which
 // every running request advances by one vertex; requests leave as they
-// finish; a request that arrives alone while nothing runs is mined on the plain
-// path directly.  Plain: requests are mined one at a time in arrival order.
+// finish.  Requests that do not go through the batch engine (one that arrives
+// alone while nothing runs, one whose estimate exceeds a slot table, one
+// whose table overflowed its slot) are mined on the plain path: by `side`,
+// a PlainMiner on its own context (own HIP stream) driven by a worker thread,
+// so admission and iterations go on meanwhile; without `side`, inline by the
+// engine's fallback.  Plain: requests are mined one at a time in arrival
+// order.
 struct ServeResult {
     std::vector<double> latencyMs;   // arrival to result, per request
     std::vector<uint64_t> counts;
+    std::vector<uint32_t> graphBits; // nwords per request (matched graph ids)
+    uint32_t nwords = 0;
     double wallS = 0;                // first arrival to last result
     uint64_t iterations = 0;
-    uint64_t maxConcurrent = 0;
+    uint64_t maxConcurrent = 0;      // requests in flight (batch engine + plain worker)
+    uint64_t maxBatched = 0;         // requests on the batch engine at once
     uint64_t overflowed = 0;     // left the batch engine for the plain path
-    uint64_t direct = 0;         // arrived alone at an idle engine: plain path
+    uint64_t direct = 0;         // went to the plain path without entering the batch engine
 };
 int serveBatched(BatchMiner& bm, const std::vector<CompiledQuery>& reqs,
                  const std::vector<double>& arrivalS, uint64_t memBudget, const GraphStats& gs,
-                 ServeResult& out);
+                 ServeResult& out, PlainMiner* side = nullptr);
 int servePlain(PlainMiner& pm, const std::vector<CompiledQuery>& reqs,
                const std::vector<double>& arrivalS, ServeResult& out);
 

@@ -31,6 +31,11 @@ struct gh_engine {
     geneva::BatchOptions batchOpt;
     std::unique_ptr<geneva::GraphStats> stats;
     uint32_t ngraphs = 0;
+    int device = 0;
+    uint64_t levelBytes = 0;
+    // the serving loop's plain-path worker: its own context (own stream)
+    gv_ctx_t sideCtx = nullptr;
+    std::unique_ptr<geneva::PlainMiner> side;
 };
 
 namespace {
@@ -316,7 +321,9 @@ gh_engine_t gh_engine_create(gh_dataset_t d, int device, uint64_t level_bytes, i
         }
         e->ngraphs = d->ds.ngraphs;
     }
-    e->plain.reset(new geneva::PlainMiner(e->ctx, e->graph, level_bytes ? level_bytes : (1ull << 30)));
+    e->device = device;
+    e->levelBytes = level_bytes ? level_bytes : (1ull << 30);
+    e->plain.reset(new geneva::PlainMiner(e->ctx, e->graph, e->levelBytes));
     if (status) *status = GV_OK;
     return e.release();
 }
@@ -324,6 +331,8 @@ gh_engine_t gh_engine_create(gh_dataset_t d, int device, uint64_t level_bytes, i
 void gh_engine_free(gh_engine_t e) {
     if (!e) return;
     e->batch.reset();
+    e->side.reset();
+    if (e->sideCtx) gv_ctx_destroy(e->sideCtx);
     e->plain.reset();
     if (e->graph) gv_graph_destroy(e->graph);
     if (e->ctx) gv_ctx_destroy(e->ctx);
@@ -449,7 +458,7 @@ int gh_engine_autotune(gh_engine_t e, const char* const* queries, uint32_t n, ui
 
 int gh_engine_serve(gh_engine_t e, const char* const* queries, const double* arrival_s, uint32_t n,
                     int mode, uint64_t mem_budget, double* latency_ms, uint64_t* counts,
-                    double* stats) {
+                    uint32_t* graph_bits, uint32_t nwords, double* stats) {
     if (!e || !arrival_s || !latency_ms || (n && !queries)) return GV_ERR_ARG;
     for (uint32_t i = 1; i < n; ++i)
         if (arrival_s[i] < arrival_s[i - 1]) return GV_ERR_ARG;
@@ -462,7 +471,12 @@ int gh_engine_serve(gh_engine_t e, const char* const* queries, const double* arr
         rc = ensureBatch(e);
         if (rc != GV_OK) return rc;
         if (!e->stats) e->stats.reset(new geneva::GraphStats(geneva::GraphStats::of(e->graph)));
-        rc = geneva::serveBatched(*e->batch, qs, arr, mem_budget, *e->stats, sr);
+        if (!e->side) {
+            rc = gv_ctx_create(e->device, &e->sideCtx);
+            if (rc != GV_OK) return rc;
+            e->side.reset(new geneva::PlainMiner(e->sideCtx, e->graph, e->levelBytes));
+        }
+        rc = geneva::serveBatched(*e->batch, qs, arr, mem_budget, *e->stats, sr, e->side.get());
     } else {
         rc = geneva::servePlain(*e->plain, qs, arr, sr);
     }
@@ -473,6 +487,11 @@ int gh_engine_serve(gh_engine_t e, const char* const* queries, const double* arr
     for (uint32_t i = 0; i < n; ++i) {
         latency_ms[i] = sr.latencyMs[i];
         if (counts) counts[i] = sr.counts[i];
+        if (graph_bits && nwords) {
+            uint32_t* dst = graph_bits + (size_t)i * nwords;
+            for (uint32_t w = 0; w < nwords; ++w)
+                dst[w] = w < sr.nwords ? sr.graphBits[(size_t)i * sr.nwords + w] : 0;
+        }
     }
     if (stats) {
         stats[0] = sr.wallS;
@@ -480,6 +499,7 @@ int gh_engine_serve(gh_engine_t e, const char* const* queries, const double* arr
         stats[2] = (double)sr.maxConcurrent;
         stats[3] = (double)sr.overflowed;
         stats[4] = (double)sr.direct;
+        stats[5] = (double)sr.maxBatched;
     }
     return GV_OK;
 }

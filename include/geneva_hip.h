@@ -298,6 +298,11 @@ int gv_batch_poll_bits(gv_batch_t b, gv_slot_info_t* info, uint32_t* bits, uint3
  * receives its matched graph ids, which synchronises; without it the call
  * only enqueues the reset. */
 int gv_batch_release(gv_batch_t b, uint32_t slot, uint32_t* graph_bits, uint32_t nwords);
+/* Free n slots at once; bits (nullable) receives slot slots[i]'s matched graph
+ * ids at bits[i * nwords ..] (nwords = ceil(graphs / 32)), with one
+ * synchronisation for all of them. */
+int gv_batch_release_many(gv_batch_t b, uint32_t n, const uint32_t* slots, uint32_t* bits,
+                          uint32_t nwords);
 /* Slots, rows a slot table holds at a given width, free slots. */
 int gv_batch_info(gv_batch_t b, uint32_t* slots, uint64_t* slot_bytes, uint32_t* free_slots);
 
@@ -309,7 +314,8 @@ int gv_batch_info(gv_batch_t b, uint32_t* slots, uint64_t* slot_bytes, uint32_t*
  * (k_walk, last step), 4 batch iteration (one graph launch), 5 per-row
  * counting (k_count_rows, k_count_leaves), 6 parent intersections of a
  * prefix-shared counted step (their sizing, k_isect, k_child_size; no work
- * units or requested bytes).  read synchronises and returns the
+ * units or requested bytes), 7 pendant chains (k_chain_mark,
+ * k_chain_weights, k_count_chain).  read synchronises and returns the
  * launch count, the summed device milliseconds and the requested bytes since
  * the last reset. */
 #define GV_KIND_SIZE 0
@@ -319,6 +325,7 @@ int gv_batch_info(gv_batch_t b, uint32_t* slots, uint64_t* slot_bytes, uint32_t*
 #define GV_KIND_BATCH 4
 #define GV_KIND_ROWS 5
 #define GV_KIND_ISECT 6
+#define GV_KIND_CHAIN 7
 int gv_profile_enable(gv_ctx_t ctx, int on);
 int gv_profile_reset(gv_ctx_t ctx);
 int gv_profile_read(gv_ctx_t ctx, int kind, uint64_t* launches, double* total_ms,

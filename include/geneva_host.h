@@ -105,14 +105,18 @@ int gh_engine_autotune(gh_engine_t e, const char* const* queries, uint32_t n, ui
                        uint32_t* blocks_per_cu, double* best_ms);
 /* Serve n requests arriving at arrival_s[i] seconds (non-decreasing).
  * mode 0: continuous batching with admission under mem_budget bytes of
- * estimated peak tables; mode 1: plain path, one request at a time.
- * latency_ms and counts receive n entries; stats (nullable, 5 doubles): wall
- * seconds, iterations, most requests in flight, requests re-run on the plain
- * path, requests that arrived alone at an idle engine and went to the plain
- * path directly. */
+ * estimated peak tables (requests that leave the batch engine are mined on
+ * the plain path by a worker thread on a second context); mode 1: plain
+ * path, one request at a time.  latency_ms and counts receive n entries;
+ * graph_bits (nullable) receives each request's matched graph ids,
+ * nwords = ceil(ngraphs/32) words per request; stats (nullable, 6 doubles):
+ * wall seconds, iterations, most requests in flight, requests re-run on the
+ * plain path after overflowing a slot, requests sent to the plain path
+ * without entering the batch engine, most requests on the batch engine at
+ * once. */
 int gh_engine_serve(gh_engine_t e, const char* const* queries, const double* arrival_s, uint32_t n,
                     int mode, uint64_t mem_budget, double* latency_ms, uint64_t* counts,
-                    double* stats);
+                    uint32_t* graph_bits, uint32_t nwords, double* stats);
 
 /* Per-kernel profiling of the engine's context (see gv_profile_* in
  * geneva_hip.h): enable/disable, reset, and read one kernel kind. */

@@ -4,6 +4,8 @@
     python tests/golden/make_golden.py --full2    # + full-size workload 2 counts (long)
     python tests/golden/make_golden.py --only-w4  # workload 4: scaled counts, full-size shards
                                                   # (--shards-only: the shards alone)
+    python tests/golden/make_golden.py --only-w5  # workload 5: the serving query mix (3-10
+                                                  # vertices) over the scaled workload-3 database
 
 Reference values come from the reference's own cpumine, compiled from its
 sources into oracle/_ref/cpumine (`make -C oracle ref`), run on files written
@@ -192,7 +194,36 @@ def workload4_scaled(scale=0.01, nq=24):
               open(os.path.join(HERE, "workload4_scaled.json"), "w"), indent=1)
 
 
+def workload5(scale=0.01, nq=64):
+    """The serving stream's queries (random walks of 3-10 vertices, the
+    workload-5 generator) over workload 3's database at `scale`: oracle counts,
+    matched graph ids and algorithmic bytes, and the reference cpumine count
+    where it finishes quickly."""
+    ds = configs.workload3(scale=scale, nqueries=0)
+    qs = configs.workload5_queries(ds, nq)
+    ogw = O.OracleGraph.of(ds)
+    d = tempfile.mkdtemp()
+    pw = write(ds, d, "w5")
+    cases = []
+    for qt in qs:
+        r = ogw.mine(qt, threads=os.cpu_count())
+        c = {"query": qt, "count": r.count, "alg_bytes": r.alg_bytes,
+             "graph_ids": sorted(r.graph_ids)}
+        if r.count < 2_000_000 and os.path.exists(O.REF_CPUMINE):
+            _, ref = ref_run(pw, qt, timeout=120)
+            if ref is not None:
+                c["ref_count"] = ref
+                assert ref == r.count, (ref, r.count, qt)
+        cases.append(c)
+        print("workload5:", len(qt.splitlines()), r.count, len(r.graph_ids), flush=True)
+    shutil.rmtree(d, ignore_errors=True)
+    json.dump({"workload": 5, "scale": scale, "nqueries": nq, "cases": cases},
+              open(os.path.join(HERE, "workload5_scaled.json"), "w"), indent=1)
+
+
 def main():
+    if "--only-w5" in sys.argv:
+        return workload5()
     if "--only-full2" in sys.argv:
         return full_workload2()
     if "--only-w4" in sys.argv:

@@ -73,6 +73,14 @@ def test_row_cap_defers_slots_without_changing_results(w3):
     eng.batch_config(slots=64, slot_bytes=8 << 20, row_cap=1 << 14)
     res, st = eng.mine_batch(qs)
     assert [r.count for r in res] == want
+    # sustained admission under the same row cap: slots that keep waiting
+    # age and run ahead of refilled lower slots, and every request finishes
+    # with its own result
+    rng = np.random.default_rng(9)
+    arr = np.cumsum(rng.exponential(1 / 20000.0, size=len(qs))).tolist()
+    lat, cnt, st = eng.serve(qs, arr, batched=True)
+    print("serve under row cap", st, flush=True)
+    assert cnt == want
     eng.batch_config(slots=64, slot_bytes=8 << 20, row_cap=1 << 22)
 
 
@@ -103,3 +111,61 @@ def test_serving_results_match(w3):
     assert all(l > 0 for l in lat)
     lat2, cnt2, st2 = eng.serve(qs, arr, batched=False)
     assert cnt2 == want
+
+
+@pytest.fixture(scope="module")
+def w5(require_gpu):
+    """The serving query mix (random walks of 3-10 vertices) over the scaled
+    workload-3 database, with oracle counts, graph ids and bytes (and reference
+    cpumine counts) in tests/golden/workload5_scaled.json."""
+    gold = json.load(open(os.path.join(GOLDEN, "workload5_scaled.json")))
+    ds = configs.workload3(scale=gold["scale"], nqueries=0)
+    qs = configs.workload5_queries(ds, gold["nqueries"])
+    assert qs == [c["query"] for c in gold["cases"]], "generator drifted from the fixture"
+    eng = G.Engine(ds)
+    eng.batch_config(slots=64, slot_bytes=8 << 20, row_cap=1 << 22)
+    return ds, eng, gold["cases"]
+
+
+def _check(cases, counts, ids, what):
+    for i, c in enumerate(cases):
+        assert counts[i] == c["count"], (what, i, counts[i], c["count"])
+        assert sorted(ids[i]) == c["graph_ids"], (what, i)
+        if "ref_count" in c:
+            assert counts[i] == c["ref_count"], (what, i)
+
+
+def test_workload5_batch_and_plain_match_oracle(w5):
+    ds, eng, cases = w5
+    qs = [c["query"] for c in cases]
+    print("workload 5 mix:", len(qs), "queries", flush=True)
+    res, st = eng.mine_batch(qs)
+    print("batch stats", st, flush=True)
+    _check(cases, [r.count for r in res], [r.graph_ids for r in res], "batch")
+    assert [r.alg_bytes for r in res] == [c["alg_bytes"] for c in cases]
+    plain = [eng.mine(q) for q in qs]
+    _check(cases, [r.count for r in plain], [r.graph_ids for r in plain], "plain")
+    assert [r.alg_bytes for r in plain] == [c["alg_bytes"] for c in cases]
+
+
+def test_workload5_serving_matches_oracle(w5):
+    ds, eng, cases = w5
+    qs = [c["query"] for c in cases]
+    rng = np.random.default_rng(55)
+    arr = np.cumsum(rng.exponential(1 / 3000.0, size=len(qs))).tolist()
+    stats = {}
+    for batched, budget in ((True, 16 << 30), (False, 16 << 30), (True, 1)):
+        print("serve batched" if batched else "serve plain", "budget", budget, flush=True)
+        lat, cnt, st, ids = eng.serve(qs, arr, batched=batched, mem_budget=budget, with_graphs=True)
+        print("  stats", st, "p50 %.3f p99 %.3f" % (np.percentile(lat, 50), np.percentile(lat, 99)),
+              flush=True)
+        _check(cases, cnt, ids, ("serve", batched, budget))
+        assert all(l > 0 for l in lat)
+        stats[(batched, budget)] = st
+    # an admission bound below every materialising request's estimate: such
+    # requests enter the batch engine one at a time (requests whose plan
+    # materialises no table have an estimate of 0 and still join), so the
+    # same stream takes more iterations, with unchanged results
+    tight, loose = stats[(True, 1)], stats[(True, 16 << 30)]
+    assert tight["iterations"] > loose["iterations"]
+    assert tight["max_batched"] <= loose["max_batched"]

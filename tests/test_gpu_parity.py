@@ -87,19 +87,45 @@ def test_random_databases_against_live_oracle(require_gpu):
         eng.close()
 
 
+HUB_ISECT_QUERIES = [
+    # the counted vertex has two back edges into the parent prefix, so the
+    # parents' lists are intersected once per parent (k_isect, k_isect_big):
+    # a 4-clique, a diamond and a 4-clique with a pendant of a fourth label
+    "v 0 1\nv 1 1\nv 2 1\nv 3 1\ne 0 1 1\ne 0 2 1\ne 0 3 1\ne 1 2 1\ne 1 3 1\ne 2 3 1\n",
+    "v 0 1\nv 1 2\nv 2 1\nv 3 2\ne 0 1 1\ne 1 2 1\ne 2 3 1\ne 3 0 1\ne 0 2 1\n",
+    "v 0 1\nv 1 1\nv 2 2\nv 3 1\nv 4 4\ne 0 1 1\ne 0 2 1\ne 0 3 1\ne 1 2 1\ne 1 3 1\ne 2 3 1\n"
+    "e 3 4 1\n",
+]
+HUB_CHAIN_QUERIES = [
+    # a triangle and a 4-cycle with a two-edge tail of labels no other query
+    # vertex has: counted from per-vertex chain weights (largest degree > 256)
+    "v 0 1\nv 1 1\nv 2 1\nv 3 2\nv 4 3\ne 0 1 1\ne 1 2 1\ne 0 2 1\ne 2 3 1\ne 3 4 1\n",
+    "v 0 1\nv 1 1\nv 2 1\nv 3 1\nv 4 2\nv 5 3\ne 0 1 1\ne 1 2 1\ne 2 3 1\ne 3 0 1\ne 0 4 1\n"
+    "e 4 5 1\n",
+]
+
+
 def test_hub_graph_and_chunked_frontiers(require_gpu):
     """A hub with ~20k neighbours (one label group > 4096, skip index in the
-    stored block) and a tiny level budget that forces many extension chunks:
-    counts must not depend on the chunking."""
+    stored block), a few secondary hubs of ~1,000 neighbours, and a tiny level
+    budget that forces many extension chunks: counts must not depend on the
+    chunking.  The prefix-shared intersections (with parents of more than 256
+    anchor entries and other lists longer than the 4096-entry LDS stage) and
+    the pendant-chain counting must run, and agree with the oracle; with the
+    tiny budget the intersection buffer overflows and the query is re-run in
+    chunks."""
     import oracle as O
     rng = np.random.default_rng(7)
     n = 30000
-    vl = np.concatenate([[0], rng.integers(1, 4, size=n)]).astype(np.uint32)
+    vl = np.concatenate([[0], rng.integers(1, 5, size=n)]).astype(np.uint32)
+    vl[1] = 1
     # the hub touches every vertex, so the sources form one id interval
     hub = [(1, v, 1) for v in range(2, n + 1)]
+    sec = [(int(h), int(v), 1) for h in range(2, 12)
+           for v in rng.choice(np.arange(12, n + 1), size=1000, replace=False)]
     rest = rng.integers(2, n + 1, size=(40000, 2))
     rest = [(int(a), int(b), 1) for a, b in rest if a != b]
-    ds = G.Dataset.from_arrays(vl, np.array(hub + rest, dtype=np.uint32))
+    ds = G.Dataset.from_arrays(vl, np.array(hub + sec + rest, dtype=np.uint32))
     qs = ["v 0 1\nv 1 2\nv 2 2\ne 0 1 1\ne 0 2 1\n",
           "v 0 1\nv 1 1\nv 2 2\nv 3 3\ne 0 1 1\ne 0 2 1\ne 1 2 1\ne 2 3 1\n",
           "v 0 2\nv 1 1\nv 2 2\nv 3 3\ne 0 1 1\ne 1 2 1\ne 2 3 1\ne 3 0 1\n",
@@ -107,13 +133,21 @@ def test_hub_graph_and_chunked_frontiers(require_gpu):
     og = O.OracleGraph.of(ds)
     big = G.Engine(ds)
     small = G.Engine(ds, level_bytes=1 << 16)
-    for q in qs:
+    for q in qs + HUB_ISECT_QUERIES + HUB_CHAIN_QUERIES:
         print("hub query", q.replace("\n", " "), flush=True)
-        want = og.mine(q)
-        a, b = big.mine(q), small.mine(q)
+        want = og.mine(q, threads=8)
+        big.profile(True, count_bytes=False)
+        a = big.mine(q)
+        prof = big.profile_read()
+        big.profile(False)
+        b = small.mine(q)
         assert a.count == b.count == want.count
         assert a.alg_bytes == b.alg_bytes == want.alg_bytes
         assert b.launches >= a.launches
+        if q in HUB_ISECT_QUERIES:
+            assert prof["k_isect"][0] > 0, prof
+        if q in HUB_CHAIN_QUERIES:
+            assert prof["k_count_chain"][0] > 0, prof
 
 
 def test_absent_labels_give_zero(w1):
@@ -148,7 +182,9 @@ ROW_COUNT_QUERIES = [
     ("v 0 2\nv 1 1\nv 2 1\nv 3 1\ne 0 1 1\ne 0 2 1\ne 0 3 1\n", 3),
     ("v 0 1\nv 1 1\nv 2 2\nv 3 1\ne 0 1 1\ne 0 2 1\ne 0 3 1\n", 3),
     # pendant chains: a triangle and a 4-cycle with a two-edge tail of labels
-    # no other vertex has
+    # no other vertex has (these graphs have no vertex of degree > 256, so the
+    # tail is walked here; test_hub_graph_and_chunked_frontiers runs the same
+    # shapes through the chain-weight kernels)
     ("v 0 1\nv 1 1\nv 2 1\nv 3 2\nv 4 3\ne 0 1 1\ne 1 2 1\ne 0 2 1\ne 2 3 1\ne 3 4 1\n", 4),
     ("v 0 1\nv 1 1\nv 2 1\nv 3 1\nv 4 2\nv 5 3\ne 0 1 1\ne 1 2 1\ne 2 3 1\ne 3 0 1\ne 0 4 1\n"
      "e 4 5 1\n", 4),
