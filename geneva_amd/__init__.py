@@ -76,6 +76,14 @@ def lib():
                                      C.POINTER(dbl)]),
         "gh_engine_serve": (i32, [vp, C.POINTER(cs), C.POINTER(dbl), u32, i32, u64, C.POINTER(dbl),
                                   C.POINTER(u64), _u32p(), u32, C.POINTER(dbl)]),
+        "gh_phase_mine": (i32, [cs, cs, cs, i32, i32, u64, u32, i32, C.POINTER(vp)]),
+        "gh_phase_count": (u64, [vp]),
+        "gh_phase_width": (u32, [vp]),
+        "gh_phase_order": (_u32p(), [vp]),
+        "gh_phase_rows": (_u32p(), [vp]),
+        "gh_phase_launches": (u64, [vp]),
+        "gh_phase_spills": (u64, [vp]),
+        "gh_phase_free": (None, [vp]),
         "gh_engine_profile": (i32, [vp, i32]),
         "gh_engine_profile_reset": (i32, [vp]),
         "gh_engine_profile_read": (i32, [vp, i32, C.POINTER(u64), C.POINTER(dbl), C.POINTER(u64)]),
@@ -197,6 +205,53 @@ def dv_phase_count(query_text):
     if n < 0:
         raise _err("dv_phase_count", n)
     return n
+
+
+class PhaseRun:
+    """Result of the reference-shaped phase driver: count, the final rows as an
+    (n, k) uint32 array with column u = data vertex of query vertex u (None
+    unless rows were kept), launcher calls and spills to host memory."""
+    __slots__ = ("count", "rows", "launches", "spills")
+
+    def __init__(self, count, rows, launches, spills):
+        self.count, self.rows, self.launches, self.spills = count, rows, launches, spills
+
+    def __repr__(self):
+        return f"PhaseRun(count={self.count}, launches={self.launches}, spills={self.spills})"
+
+
+def phase_mine(mygraph, metadata, query_file, single_v=False, device=0, arena_bytes=0, max_rows=0,
+               keep_rows=True):
+    """Mine the query in query_file over a .mygraph/.metadata pair the way the
+    reference drivers do (src/gpumine.cpp, src/gpumine_singleV.cpp), through
+    the reference-named launchers (include/geneva_launchers.hpp) on the
+    reference's device layout.  max_rows caps maxRowNum (forces the overflow
+    protocol on small inputs)."""
+    L = lib()
+    h = C.c_void_p()
+    rc = L.gh_phase_mine(mygraph.encode(), metadata.encode(), query_file.encode(), int(single_v), device,
+                         arena_bytes, max_rows, int(keep_rows), C.byref(h))
+    if rc:
+        raise _err("phase_mine", rc)
+    try:
+        n, k = L.gh_phase_count(h), L.gh_phase_width(h)
+        rows = None
+        if keep_rows:
+            rows = np.zeros((n, k), dtype=np.uint32)
+            if n:
+                order = np.ctypeslib.as_array(L.gh_phase_order(h), shape=(k,)).copy()
+                raw = np.ctypeslib.as_array(L.gh_phase_rows(h), shape=(n, k))
+                rows[:, order] = raw
+        return PhaseRun(n, rows, L.gh_phase_launches(h), L.gh_phase_spills(h))
+    finally:
+        L.gh_phase_free(h)
+
+
+def sorted_rows(rows):
+    """Rows sorted lexicographically (for multiset comparisons)."""
+    if rows is None or len(rows) == 0:
+        return rows
+    return rows[np.lexsort(rows.T[::-1])]
 
 
 class MineResult:

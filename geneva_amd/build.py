@@ -16,10 +16,10 @@ import sys
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 PKG = os.path.join(ROOT, "geneva_amd")
 LIB = os.path.join(PKG, "lib")
-HIP_SRC = [os.path.join(PKG, "csrc", "hip", f) for f in ("geneva_hip.hip", "batch.hip")]
+HIP_SRC = [os.path.join(PKG, "csrc", "hip", f) for f in ("geneva_hip.hip", "batch.hip", "launchers.hip")]
 HOST_SRC = [os.path.join(PKG, "csrc", "host", f) for f in
             ("genMatchOrder.cpp", "loadDataBase.cpp", "gen.cpp", "miner.cpp",
-             "batch.cpp", "serve.cpp", "host_capi.cpp")]
+             "batch.cpp", "serve.cpp", "phase_driver.cpp", "host_capi.cpp")]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("GENEVA_ARCH", "gfx950")
 

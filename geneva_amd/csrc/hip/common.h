@@ -1114,7 +1114,13 @@ struct gv_ctx {
     uint32_t* plist = nullptr;            // parents with rows: small ones, then large ones from plist_cap / 2
     uint64_t plist_cap = 0;
     uint32_t chain_epoch = 0;
+    // workspace and resume state of the reference-layout launchers
+    // (launchers.hip), created on their first call
+    struct RefState* ref = nullptr;
 };
+
+// Frees ctx->ref (launchers.hip); called by gv_ctx_destroy.
+void ref_state_free(gv_ctx* c);
 
 struct gv_graph {
     gv_ctx* ctx = nullptr;
@@ -1164,7 +1170,7 @@ DevGraph dev_view(const gv_graph* g) {
 }
 
 enum { kKindSize = 0, kKindScan = 1, kKindExt = 2, kKindCount = 3, kKindBatch = 4, kKindRows = 5,
-       kKindIsect = 6, kKindChain = 7, kKinds = 8 };
+       kKindIsect = 6, kKindChain = 7, kKindRef = 8, kKinds = 9 };
 
 hipEvent_t pool_get(gv_ctx* c) {
     if (!c->pool.empty()) {

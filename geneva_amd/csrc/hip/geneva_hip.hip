@@ -1094,6 +1094,7 @@ int gv_ctx_destroy(gv_ctx_t c) {
     if (c->ilen) hipFree(c->ilen);
     if (c->pmark) hipFree(c->pmark);
     if (c->plist) hipFree(c->plist);
+    ref_state_free(c);
     for (auto& r : c->recs) {
         hipEventDestroy(r.a);
         hipEventDestroy(r.b);
@@ -2020,8 +2021,19 @@ int gv_counters_read(gv_ctx_t c, uint64_t* count, uint64_t* bytes, uint32_t* bit
 int gv_copy_partial_emb(gv_ctx_t c, const uint32_t* src, uint32_t* dst, uint64_t n) {
     if (!c || (n && (!src || !dst))) return GV_ERR_ARG;
     if (n == 0 || src == dst) return GV_OK;
-    if (dst < src || dst >= src + n) {
+    if (dst + n <= src || dst >= src + n) {   // disjoint
         HIPCHK(hipMemcpyAsync(dst, src, n * 4, hipMemcpyDeviceToDevice, c->stream));
+        return GV_OK;
+    }
+    if (dst < src) {
+        // dst overlaps the head of src: copy in chunks from the front, each
+        // no longer than the shift, so no chunk overwrites what a later one
+        // still has to read
+        const uint64_t shift = (uint64_t)(src - dst);
+        for (uint64_t beg = 0; beg < n; beg += shift) {
+            const uint64_t len = std::min(shift, n - beg);
+            HIPCHK(hipMemcpyAsync(dst + beg, src + beg, len * 4, hipMemcpyDeviceToDevice, c->stream));
+        }
         return GV_OK;
     }
     // dst overlaps the tail of src: copy in chunks from the end, each chunk

@@ -13,6 +13,7 @@
 #include "loadDataBase.h"
 #include "batch.h"
 #include "miner.h"
+#include "phase_driver.h"
 
 namespace {
 thread_local std::string g_err;
@@ -522,5 +523,42 @@ int gh_engine_profile_units(gh_engine_t e, int kind, uint64_t* units) {
 int gh_engine_profile_alg_bytes(gh_engine_t e, int kind, uint64_t* bytes) {
     return e ? gv_profile_alg_bytes(e->ctx, kind, bytes) : GV_ERR_ARG;
 }
+
+struct gh_phase_run {
+    geneva::PhaseRunResult r;
+};
+
+int gh_phase_mine(const char* mygraph, const char* metadata, const char* query_file, int single_v, int device,
+                  uint64_t arena_bytes, uint32_t max_rows, int keep_rows, gh_phase_run_t* out) {
+    if (!mygraph || !metadata || !query_file || !out) return GV_ERR_ARG;
+    *out = nullptr;
+    gv_ctx_t ctx = nullptr;
+    int rc = gv_ctx_create(device, &ctx);
+    if (rc != GV_OK) {
+        setErr(std::string("gv_ctx_create: ") + gv_status_string(rc));
+        return rc;
+    }
+    std::unique_ptr<gh_phase_run> run(new gh_phase_run());
+    geneva::PhaseDriverOptions opt;
+    if (arena_bytes) opt.arenaBytes = arena_bytes;
+    opt.maxRows = max_rows;
+    opt.keepRows = keep_rows != 0;
+    rc = geneva::runPhaseDriver(ctx, mygraph, metadata, query_file, single_v != 0, opt, run->r);
+    gv_ctx_destroy(ctx);
+    if (rc != GV_OK) {
+        setErr("phase driver: " + std::string(gv_status_string(rc)) + " " + run->r.error + " " + gv_last_error());
+        return rc;
+    }
+    *out = run.release();
+    return GV_OK;
+}
+
+uint64_t gh_phase_count(gh_phase_run_t r) { return r ? r->r.count : 0; }
+uint32_t gh_phase_width(gh_phase_run_t r) { return r ? r->r.width : 0; }
+const uint32_t* gh_phase_order(gh_phase_run_t r) { return r ? r->r.order.data() : nullptr; }
+const uint32_t* gh_phase_rows(gh_phase_run_t r) { return r && !r->r.rows.empty() ? r->r.rows.data() : nullptr; }
+uint64_t gh_phase_launches(gh_phase_run_t r) { return r ? r->r.launches : 0; }
+uint64_t gh_phase_spills(gh_phase_run_t r) { return r ? r->r.spills : 0; }
+void gh_phase_free(gh_phase_run_t r) { delete r; }
 
 }  // extern "C"

@@ -118,6 +118,28 @@ int gh_engine_serve(gh_engine_t e, const char* const* queries, const double* arr
                     int mode, uint64_t mem_budget, double* latency_ms, uint64_t* counts,
                     uint32_t* graph_bits, uint32_t nwords, double* stats);
 
+/* ---- the reference's phase driver over the reference-named launchers ------
+ * Mines the query in query_file over a .mygraph / .metadata pair the way the
+ * reference drivers do (src/gpumine.cpp:866-1072 with single_v = 0,
+ * src/gpumine_singleV.cpp:707-928 with single_v = 1): loadDataBase, the
+ * reference planner, one device arena of arena_bytes (0 = 1 GiB) on a context
+ * of its own, every phase through the launchers of geneva_launchers.hpp with
+ * a counter read-back after each, overflow rows moved to host memory and the
+ * launch resumed with isContinue.  max_rows (0 = none) caps maxRowNum so the
+ * overflow path can be exercised on small inputs.  The result holds the
+ * final rows (keep_rows), spilled parts included; column c holds the data
+ * vertex of query vertex order[c]. */
+typedef struct gh_phase_run* gh_phase_run_t;
+int gh_phase_mine(const char* mygraph, const char* metadata, const char* query_file, int single_v, int device,
+                  uint64_t arena_bytes, uint32_t max_rows, int keep_rows, gh_phase_run_t* out);
+uint64_t gh_phase_count(gh_phase_run_t r);
+uint32_t gh_phase_width(gh_phase_run_t r);
+const uint32_t* gh_phase_order(gh_phase_run_t r);     /* width entries */
+const uint32_t* gh_phase_rows(gh_phase_run_t r);      /* count * width entries, or NULL */
+uint64_t gh_phase_launches(gh_phase_run_t r);
+uint64_t gh_phase_spills(gh_phase_run_t r);
+void gh_phase_free(gh_phase_run_t r);
+
 /* Per-kernel profiling of the engine's context (see gv_profile_* in
  * geneva_hip.h): enable/disable, reset, and read one kernel kind. */
 int gh_engine_profile(gh_engine_t e, int on);

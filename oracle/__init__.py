@@ -35,6 +35,8 @@ def lib():
                                   C.c_uint64, u32p, C.c_uint32, C.POINTER(C.c_uint8), C.POINTER(C.c_uint64),
                                   C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
                                   C.POINTER(C.c_double)]
+        L.oracle_rows.restype = C.c_int64
+        L.oracle_rows.argtypes = [C.c_void_p, C.c_char_p, u32p, C.c_uint64, C.POINTER(C.c_uint32)]
         _lib = L
     return _lib
 
@@ -74,6 +76,19 @@ class OracleGraph:
         h, self._h = getattr(self, "_h", None), None
         if h and _lib is not None:
             _lib.oracle_graph_destroy(h)
+
+    def rows(self, query_text):
+        """The faithful mode's embeddings as an (n, k) uint32 array, column u =
+        data vertex of query vertex u, rows sorted."""
+        k = C.c_uint32(0)
+        n = lib().oracle_rows(self._h, query_text.encode(), None, 0, C.byref(k))
+        if n < 0:
+            raise ValueError("oracle: malformed query")
+        out = np.zeros((n, k.value), dtype=np.uint32)
+        if n:
+            lib().oracle_rows(self._h, query_text.encode(), out.ctypes.data_as(C.POINTER(C.c_uint32)),
+                              out.size, C.byref(k))
+        return out
 
     def mine(self, query_text, mode="count", threads=0, start_begin=0, start_end=0, stride=1):
         cnt, by, ns = C.c_uint64(0), C.c_uint64(0), C.c_uint64(0)

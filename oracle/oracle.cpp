@@ -213,9 +213,11 @@ void dfs(const Graph& g, const Compiled& c, unsigned t, unsigned* row, Acc& acc,
     else if (withBytes) acc.bytes += 4ull * (t + 1) * surv;
 }
 
-// cpumine's phase-by-phase execution on in-memory tables.
+// cpumine's phase-by-phase execution on in-memory tables.  finalRows
+// (nullable) receives the last table with its columns put in query-vertex
+// order (column u = the data vertex of query vertex u).
 uint64_t faithful(const Graph& g, const OPlan& P, const std::vector<unsigned>* graphOf,
-                  std::vector<uint8_t>* bits) {
+                  std::vector<uint8_t>* bits, std::vector<unsigned>* finalRows = nullptr) {
     std::map<unsigned, unsigned> pos;  // posOfVisitedV
     auto ql = [&](unsigned v) { return P.queryLabel.at(v); };
     auto restr = [&](unsigned v) {
@@ -344,6 +346,11 @@ uint64_t faithful(const Graph& g, const OPlan& P, const std::vector<unsigned>* g
     }
     if (graphOf && bits)
         for (size_t r = 0; r < rows.size() / len; ++r) (*bits)[(*graphOf)[rows[r * len]]] = 1;
+    if (finalRows) {
+        finalRows->assign(rows.size(), 0);
+        for (size_t r = 0; r < rows.size() / len; ++r)
+            for (auto& kv : pos) (*finalRows)[r * len + kv.first] = rows[r * len + kv.second];
+    }
     return rows.size() / len;
 }
 
@@ -489,6 +496,32 @@ int oracle_mine(void* h, const char* query, int mode, int threads, uint64_t star
     if (secs) *secs = std::chrono::duration<double>(t1 - t0).count();
     if (bits && !gb.empty()) memcpy(bits, gb.data(), ngraphs);
     return 0;
+}
+
+// The embeddings of the faithful mode, one row per embedding with column u
+// holding the data vertex of query vertex u, rows sorted ascending
+// (lexicographic).  Returns the number of rows; out receives them when
+// cap >= rows * k (k = query vertices, written to *kOut).  Negative on a
+// malformed query.
+int64_t oracle_rows(void* h, const char* query, uint32_t* out, uint64_t cap, uint32_t* kOut) {
+    Graph& g = *(Graph*)h;
+    OPlan P;
+    std::string err;
+    if (!planQuery(query, P, err)) return -1;
+    std::vector<unsigned> rows;
+    const uint64_t n = faithful(g, P, nullptr, nullptr, &rows);
+    const unsigned k = P.vnum;
+    if (kOut) *kOut = k;
+    std::vector<uint64_t> idx(n);
+    for (uint64_t i = 0; i < n; ++i) idx[i] = i;
+    std::sort(idx.begin(), idx.end(), [&](uint64_t a, uint64_t b) {
+        return std::lexicographical_compare(rows.begin() + a * k, rows.begin() + (a + 1) * k,
+                                            rows.begin() + b * k, rows.begin() + (b + 1) * k);
+    });
+    if (out && cap >= n * k)
+        for (uint64_t i = 0; i < n; ++i)
+            for (unsigned j = 0; j < k; ++j) out[i * k + j] = rows[idx[i] * k + j];
+    return (int64_t)n;
 }
 
 }  // extern "C"

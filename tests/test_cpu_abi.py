@@ -34,6 +34,33 @@ def test_hip_library_exports_every_declared_symbol():
         getattr(lib, n)
 
 
+def test_launcher_abi_exports_every_declared_symbol():
+    """include/geneva_launchers.h: one entry per reference launcher
+    (include/miningPattern.h:1-31, include/reduction.h:1-4), and the shim
+    include/geneva_launchers.hpp defines the reference's names."""
+    names = _declared("geneva_launchers.h")
+    ref = ["initEmb_3V_1and2_NoHash", "initEmb_2V_NoHash", "extEmb_2V_2src_1and2_sameLabel_NoHash",
+           "extEmb_2V_2src_1and2_notSameLabel_NoHash", "extEmb_2V_1src_1and2_sameLabel_NoHash",
+           "extEmb_2V_1src_1and2_notSameLabel_NoHash", "extEmb_1V_NoHash", "reductionPhase",
+           "copyPartialEmb"]
+    assert sorted("gv_" + r for r in ref) == sorted(n for n in names if n != "gv_launchers_reserve")
+    exp = _exports(G.HIP_LIB)
+    assert not [n for n in names if n not in exp]
+    shim = open(os.path.join(ROOT, "include", "geneva_launchers.hpp")).read()
+    for r in ref:
+        assert re.search(r"inline void " + r + r"\(", shim), r
+
+
+def test_launcher_argument_checks_without_a_device():
+    lib = C.CDLL(G.HIP_LIB)
+    # no context, no partition: rejected before any device work
+    assert lib.gv_extEmb_1V_NoHash(None, None, None, None, None, None, None, None, None, 1, 0, 3,
+                                   False, False, False, False, False, 10) == -1
+    assert lib.gv_reductionPhase(None, None, None, None, None, None, None, None, 1, 0, 3, False) == -1
+    assert lib.gv_initEmb_2V_NoHash(None, None, None, None, None, None, None, 300, 1, 1, 1, 0, 0,
+                                    False, 10) == -1   # more than 256 intervals
+
+
 def test_host_library_exports_every_declared_symbol():
     names = _declared("geneva_host.h")
     assert len(names) >= 20

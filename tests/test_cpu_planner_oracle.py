@@ -222,6 +222,102 @@ def test_oracle_algorithmic_bytes_formula_on_one_edge():
     assert og.mine(q).alg_bytes == expect
 
 
+def _plan_order_and_restricts(q):
+    """Column order (vertices as the plan's E phases introduce them) and the
+    restriction map of the planner (restrict: u<v means f(u) < f(v))."""
+    order, res = [], {}
+    for line in G.plan_dump(q).splitlines():
+        if line.startswith("restrict: "):
+            a, b = line[len("restrict: "):].split("<")
+            if int(b) != 0xffffffff:
+                res[int(a)] = int(b)
+        elif line.startswith("svid:"):
+            toks = line.replace("svid:", " ").replace("evid:", " ").split()
+            for i in range(0, len(toks), 2):
+                s_, e_ = int(toks[i]), int(toks[i + 1])
+                if not order:
+                    order.append(s_)
+                if e_ not in order:
+                    order.append(e_)
+    return order, res
+
+
+def test_oracle_rows_match_brute_force_embeddings():
+    """The faithful mode's final rows (the table the row-multiset comparisons
+    use) equal every map of Appendix-B semantics enumerated here directly:
+    injective, label-preserving, every query edge on a data edge of its label,
+    the planner's restrictions."""
+    ds = G.Dataset.database(40, 5, 10, 3.5, 0.5, 3, 0.0, 17)
+    ds.add_walk_queries(10, 3, 5, seed=8)
+    ds.add_walk_queries(4, 4, 5, induced=True, seed=9)
+    og = O.OracleGraph.of(ds)
+    vl = ds.vlabel
+    adj = {}
+    for a, b, l in ds.edges:
+        adj.setdefault(int(a), set()).add((int(b), int(l)))
+        adj.setdefault(int(b), set()).add((int(a), int(l)))
+    for q in ds.queries:
+        qv = {int(l.split()[1]): int(l.split()[2]) for l in q.splitlines() if l.startswith("v ")}
+        qe = [tuple(map(int, l.split()[1:4])) for l in q.splitlines() if l.startswith("e ")]
+        k = len(qv)
+        _, res = _plan_order_and_restricts(q)
+        found = []
+
+        def rec(f):
+            u = len(f)
+            if u == k:
+                if all(f[a] < f[b] for a, b in res.items()):
+                    found.append(tuple(f))
+                return
+            for x in range(1, ds.nv + 1):
+                if vl[x] != qv[u] or x in f:
+                    continue
+                if all((f[b if a == u else a], l) in adj.get(x, ()) for a, b, l in qe
+                       if (a == u and b < u) or (b == u and a < u)):
+                    rec(f + [x])
+
+        rec([])
+        got = og.rows(q)
+        assert sorted(found) == [tuple(int(v) for v in r) for r in got], q
+
+
+def test_oracle_algorithmic_bytes_two_edge_path():
+    """B on a two-edge path of three labels, counted by hand here: the first
+    extension reads each start row (4 B), the row-pointer pair and segment of
+    its list (8 + 4|N|) and writes every survivor as a two-column row
+    (8 B each); the last extension reads each two-column row (8 B) and the
+    segments of its back neighbours, and writes nothing."""
+    ds = G.Dataset.database(120, 6, 20, 4.0, 0.5, 3, 0.0, 31)
+    og = O.OracleGraph.of(ds)
+    vl = ds.vlabel
+    nb = {}
+    for a, b, _ in ds.edges:
+        nb.setdefault(int(a), []).append(int(b))
+        nb.setdefault(int(b), []).append(int(a))
+    q = "v 0 1\nv 1 2\nv 2 3\ne 0 1 1\ne 1 2 1\n"
+    order, res = _plan_order_and_restricts(q)
+    assert not res
+    lab = {0: 1, 1: 2, 2: 3}
+    qadj = {0: {1}, 1: {0, 2}, 2: {1}}
+
+    def seg(v, l):
+        return [x for x in nb.get(v, []) if vl[x] == l]
+
+    u0, u1, u2 = order
+    expect = 0
+    for v in range(1, ds.nv + 1):
+        if vl[v] != lab[u0]:
+            continue
+        n1 = seg(v, lab[u1])
+        expect += 4 + 8 + 4 * len(n1) + 8 * len(n1)       # read row, read list, write survivors
+        for x in n1:
+            f = {u0: v, u1: x}
+            expect += 8                                    # read the two-column row
+            for w in qadj[u2]:
+                expect += 8 + 4 * len(seg(f[w], lab[u2]))  # each back neighbour's segment
+    assert og.mine(q).alg_bytes == expect
+
+
 def test_golden_counts_workload1_match_reference_cpumine():
     gold = _load("workload1.json")
     ds = configs.workload1()

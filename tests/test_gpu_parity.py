@@ -90,9 +90,11 @@ def test_random_databases_against_live_oracle(require_gpu):
 HUB_ISECT_QUERIES = [
     # the counted vertex has two back edges into the parent prefix, so the
     # parents' lists are intersected once per parent (k_isect, k_isect_big):
-    # a 4-clique, a diamond and a 4-clique with a pendant of a fourth label
+    # a 4-clique, a triangle with a fourth vertex on two of its corners (the
+    # plan places it last, joined to columns 0 and 1) and a 4-clique with a
+    # pendant of a fourth label
     "v 0 1\nv 1 1\nv 2 1\nv 3 1\ne 0 1 1\ne 0 2 1\ne 0 3 1\ne 1 2 1\ne 1 3 1\ne 2 3 1\n",
-    "v 0 1\nv 1 2\nv 2 1\nv 3 2\ne 0 1 1\ne 1 2 1\ne 2 3 1\ne 3 0 1\ne 0 2 1\n",
+    "v 0 1\nv 1 2\nv 2 3\nv 3 4\ne 0 1 1\ne 0 2 1\ne 1 2 1\ne 3 0 1\ne 3 1 1\n",
     "v 0 1\nv 1 1\nv 2 2\nv 3 1\nv 4 4\ne 0 1 1\ne 0 2 1\ne 0 3 1\ne 1 2 1\ne 1 3 1\ne 2 3 1\n"
     "e 3 4 1\n",
 ]
