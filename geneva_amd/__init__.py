@@ -69,6 +69,8 @@ def lib():
         "gh_engine_mine_shard": (i32, [vp, cs, i32, u64, u64, u64, C.POINTER(u64), C.POINTER(u64),
                                        C.POINTER(dbl), _u32p(), u32, C.POINTER(u64)]),
         "gh_engine_count_by_rows": (i32, [vp, i32]),
+        "gh_engine_time_limit": (i32, [vp, dbl]),
+        "gh_engine_last_progress": (i32, [vp, C.POINTER(i32), C.POINTER(dbl)]),
         "gh_engine_batch_config": (i32, [vp, u32, u64, u64, u32, u32]),
         "gh_engine_batch_mine": (i32, [vp, C.POINTER(cs), u32, i32, C.POINTER(u64), C.POINTER(u64),
                                        _u32p(), u32, C.POINTER(dbl)]),
@@ -83,7 +85,10 @@ def lib():
         "gh_phase_rows": (_u32p(), [vp]),
         "gh_phase_launches": (u64, [vp]),
         "gh_phase_spills": (u64, [vp]),
+        "gh_phase_single_v": (i32, [vp]),
         "gh_phase_free": (None, [vp]),
+        "gh_engine_batch_profile": (i32, [vp, i32]),
+        "gh_engine_batch_profile_bytes": (i32, [vp, C.POINTER(u64), C.POINTER(u64)]),
         "gh_engine_profile": (i32, [vp, i32]),
         "gh_engine_profile_reset": (i32, [vp]),
         "gh_engine_profile_read": (i32, [vp, i32, C.POINTER(u64), C.POINTER(dbl), C.POINTER(u64)]),
@@ -211,10 +216,11 @@ class PhaseRun:
     """Result of the reference-shaped phase driver: count, the final rows as an
     (n, k) uint32 array with column u = data vertex of query vertex u (None
     unless rows were kept), launcher calls and spills to host memory."""
-    __slots__ = ("count", "rows", "launches", "spills")
+    __slots__ = ("count", "rows", "launches", "spills", "single_v")
 
-    def __init__(self, count, rows, launches, spills):
+    def __init__(self, count, rows, launches, spills, single_v):
         self.count, self.rows, self.launches, self.spills = count, rows, launches, spills
+        self.single_v = single_v
 
     def __repr__(self):
         return f"PhaseRun(count={self.count}, launches={self.launches}, spills={self.spills})"
@@ -225,11 +231,13 @@ def phase_mine(mygraph, metadata, query_file, single_v=False, device=0, arena_by
     """Mine the query in query_file over a .mygraph/.metadata pair the way the
     reference drivers do (src/gpumine.cpp, src/gpumine_singleV.cpp), through
     the reference-named launchers (include/geneva_launchers.hpp) on the
-    reference's device layout.  max_rows caps maxRowNum (forces the overflow
-    protocol on small inputs)."""
+    reference's device layout.  single_v: False (DV plan), True (SV plan) or
+    None (DV when the DV plan has a two-vertex phase).  max_rows caps
+    maxRowNum (forces the overflow protocol on small inputs)."""
     L = lib()
     h = C.c_void_p()
-    rc = L.gh_phase_mine(mygraph.encode(), metadata.encode(), query_file.encode(), int(single_v), device,
+    sv = -1 if single_v is None else int(bool(single_v))
+    rc = L.gh_phase_mine(mygraph.encode(), metadata.encode(), query_file.encode(), sv, device,
                          arena_bytes, max_rows, int(keep_rows), C.byref(h))
     if rc:
         raise _err("phase_mine", rc)
@@ -242,7 +250,8 @@ def phase_mine(mygraph, metadata, query_file, single_v=False, device=0, arena_by
                 order = np.ctypeslib.as_array(L.gh_phase_order(h), shape=(k,)).copy()
                 raw = np.ctypeslib.as_array(L.gh_phase_rows(h), shape=(n, k))
                 rows[:, order] = raw
-        return PhaseRun(n, rows, L.gh_phase_launches(h), L.gh_phase_spills(h))
+        return PhaseRun(n, rows, L.gh_phase_launches(h), L.gh_phase_spills(h),
+                        bool(L.gh_phase_single_v(h)))
     finally:
         L.gh_phase_free(h)
 
@@ -370,6 +379,36 @@ class Engine:
         ids = [_bits_to_ids(words[i * nwords:(i + 1) * nwords], self.ngraphs) if nwords else frozenset()
                for i in range(n)]
         return list(lat)[:n], list(cnt)[:n], stats, ids
+
+    def batch_profile(self, on=True):
+        """Per-kernel algorithmic bytes of the batch engine (re-captures its
+        iteration); with profile() on, batch iterations run kernel by kernel
+        under HIP events."""
+        rc = lib().gh_engine_batch_profile(self._h, int(on))
+        if rc:
+            raise _err("batch_profile", rc)
+
+    def batch_profile_bytes(self):
+        """(B charged to k_b_size, B charged to k_b_walk) since batch_profile(True)."""
+        a, b = C.c_uint64(0), C.c_uint64(0)
+        rc = lib().gh_engine_batch_profile_bytes(self._h, C.byref(a), C.byref(b))
+        if rc:
+            raise _err("batch_profile_bytes", rc)
+        return a.value, b.value
+
+    def time_limit(self, seconds):
+        """Stop plain-path queries after `seconds` of wall time (0 = no limit);
+        see last_progress()."""
+        rc = lib().gh_engine_time_limit(self._h, float(seconds))
+        if rc:
+            raise _err("time_limit", rc)
+
+    def last_progress(self):
+        """(finished, fraction of the first extension's candidates mined) of the
+        last plain-path query."""
+        f, x = C.c_int(0), C.c_double(0)
+        lib().gh_engine_last_progress(self._h, C.byref(f), C.byref(x))
+        return bool(f.value), x.value
 
     def count_by_rows(self, on=True):
         """Count single-back-edge last steps per row (default) or by a candidate walk."""

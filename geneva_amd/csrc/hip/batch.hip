@@ -17,6 +17,8 @@
 // captured once and replayed without host synchronisation.
 #include "common.h"
 
+#include <optional>
+
 namespace {
 
 // Rows per thread of the engine's sizing passes.  Its launches size many
@@ -85,6 +87,9 @@ struct BArgs {
     // slot; blocks add to line blockIdx % kBLines of their slot and k_b_post
     // folds the lines into the slot's state
     unsigned long long* sacc;
+    // algorithmic bytes B charged per kernel (nullable; profiling): [0] k_b_size
+    // (rows read, back lists read, rows counted there), [1] k_b_walk
+    unsigned long long* kb;
 };
 
 constexpr uint32_t kBLines = 16;
@@ -220,6 +225,7 @@ __global__ __launch_bounds__(kBlock) void k_b_size(BArgs a) {
         unsigned long long* line = slot_line(a, cur);
         if (lane == 0 && c) atomicAdd(line, (unsigned long long)c);
         if (lane == 0 && w) atomicAdd(line + 1, (unsigned long long)w);
+        if (a.kb && lane == 0 && w) atomicAdd(a.kb, (unsigned long long)w);
         acc = cacc = 0;
     };
     for (uint64_t RT = blockIdx.x; RT < ntiles; RT += gridDim.x) {
@@ -437,6 +443,7 @@ __global__ __launch_bounds__(kBlock) void k_b_walk(BArgs a) {
         if (lane == 0) {
             if (c) atomicAdd(line, (unsigned long long)c);
             if (b) atomicAdd(line + 1, (unsigned long long)b);
+            if (a.kb && b) atomicAdd(a.kb + 1, (unsigned long long)b);
         }
         acc = TileAcc();
     };
@@ -558,6 +565,8 @@ struct gv_batch {
     BState* h_zero = nullptr;           // pinned, all zero: the state of a released slot
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    unsigned long long* d_kb = nullptr;   // per-kernel B (profiling), 2 words
+    bool kb_on = false;
 };
 
 namespace {
@@ -584,7 +593,45 @@ BArgs bargs(gv_batch* b) {
     a.gwords = b->gwords;
     a.sacc = b->d_sacc;
     a.tile = 64ull * b->tune.ipt;
+    a.kb = b->kb_on ? b->d_kb : nullptr;
     return a;
+}
+
+// The kernels of one iteration on the context's stream; with `prof` each is
+// bracketed by HIP events of its kind (prepare / tiles / retire: batch;
+// sizing; scan; walk: extension).
+void launch_iteration(gv_batch* b, bool prof) {
+    gv_ctx* c = b->ctx;
+    BArgs a = bargs(b);
+    const int size_blocks = c->num_cu * 8;
+    const int scan_blocks = c->num_cu * 4;
+    const int walk_blocks = c->num_cu * (int)b->tune.walk_blocks_per_cu;
+    // events only when profiling (never inside a capture)
+    std::optional<ProfScope> ps;
+    auto timed = [&](int kind) {
+        ps.reset();
+        if (prof) ps.emplace(c, kind);
+    };
+    timed(kKindBatch);
+    hipLaunchKernelGGL(k_b_prep, dim3(1), dim3(1024), 0, c->stream, a);
+    timed(kKindSize);
+    hipLaunchKernelGGL(k_b_size, dim3(size_blocks), dim3(kBlock), 0, c->stream, a);
+    timed(kKindScan);
+    hipLaunchKernelGGL(k_b_scan_reduce, dim3(scan_blocks), dim3(kBlock), 0, c->stream, a);
+    hipLaunchKernelGGL(k_b_scan_tiles, dim3(1), dim3(kBlock), 0, c->stream, a);
+    hipLaunchKernelGGL(k_b_scan_final, dim3(scan_blocks), dim3(kBlock), 0, c->stream, a);
+    timed(kKindBatch);
+    hipLaunchKernelGGL(k_b_tiles, dim3(1), dim3(1024), 0, c->stream, a);
+    timed(kKindExt);
+    switch (b->tune.ipt) {
+        case 1: hipLaunchKernelGGL(k_b_walk<1>, dim3(walk_blocks), dim3(kBlock), 0, c->stream, a); break;
+        case 2: hipLaunchKernelGGL(k_b_walk<2>, dim3(walk_blocks), dim3(kBlock), 0, c->stream, a); break;
+        case 8: hipLaunchKernelGGL(k_b_walk<8>, dim3(walk_blocks), dim3(kBlock), 0, c->stream, a); break;
+        default: hipLaunchKernelGGL(k_b_walk<4>, dim3(walk_blocks), dim3(kBlock), 0, c->stream, a); break;
+    }
+    timed(kKindBatch);
+    hipLaunchKernelGGL(k_b_post, dim3(1), dim3(1024), 0, c->stream, a);
+    ps.reset();
 }
 
 int capture(gv_batch* b) {
@@ -597,24 +644,8 @@ int capture(gv_batch* b) {
         (void)hipGraphDestroy(b->graph);
         b->graph = nullptr;
     }
-    BArgs a = bargs(b);
-    const int size_blocks = c->num_cu * 8;
-    const int scan_blocks = c->num_cu * 4;
-    const int walk_blocks = c->num_cu * (int)b->tune.walk_blocks_per_cu;
     HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-    hipLaunchKernelGGL(k_b_prep, dim3(1), dim3(1024), 0, c->stream, a);
-    hipLaunchKernelGGL(k_b_size, dim3(size_blocks), dim3(kBlock), 0, c->stream, a);
-    hipLaunchKernelGGL(k_b_scan_reduce, dim3(scan_blocks), dim3(kBlock), 0, c->stream, a);
-    hipLaunchKernelGGL(k_b_scan_tiles, dim3(1), dim3(kBlock), 0, c->stream, a);
-    hipLaunchKernelGGL(k_b_scan_final, dim3(scan_blocks), dim3(kBlock), 0, c->stream, a);
-    hipLaunchKernelGGL(k_b_tiles, dim3(1), dim3(1024), 0, c->stream, a);
-    switch (b->tune.ipt) {
-        case 1: hipLaunchKernelGGL(k_b_walk<1>, dim3(walk_blocks), dim3(kBlock), 0, c->stream, a); break;
-        case 2: hipLaunchKernelGGL(k_b_walk<2>, dim3(walk_blocks), dim3(kBlock), 0, c->stream, a); break;
-        case 8: hipLaunchKernelGGL(k_b_walk<8>, dim3(walk_blocks), dim3(kBlock), 0, c->stream, a); break;
-        default: hipLaunchKernelGGL(k_b_walk<4>, dim3(walk_blocks), dim3(kBlock), 0, c->stream, a); break;
-    }
-    hipLaunchKernelGGL(k_b_post, dim3(1), dim3(1024), 0, c->stream, a);
+    launch_iteration(b, false);
     hipError_t e1 = hipStreamEndCapture(c->stream, &b->graph);
     if (e1 != hipSuccess) {
         g_last_error = std::string("capture: ") + hipGetErrorString(e1);
@@ -731,7 +762,7 @@ int gv_batch_destroy(gv_batch_t b) {
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
     void* ptrs[] = {b->d_desc, b->d_st, b->d_half[0], b->d_half[1], b->d_anc_beg, b->d_cnt,
-                    b->d_anc_i, b->d_csum, b->d_tiles, b->d_glob, b->d_gbits, b->d_sacc};
+                    b->d_anc_i, b->d_csum, b->d_tiles, b->d_glob, b->d_gbits, b->d_sacc, b->d_kb};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (b->h_desc) (void)hipHostFree(b->h_desc);
@@ -803,8 +834,13 @@ int gv_batch_run(gv_batch_t b, uint32_t iterations) {
     int rc = flush(b);
     if (rc != GV_OK) return rc;
     for (uint32_t i = 0; i < iterations; ++i) {
-        ProfScope ps(b->ctx, kKindBatch);
-        HIPCHK(hipGraphLaunch(b->exec, b->ctx->stream));
+        if (b->ctx->prof) {
+            // profiling: the same kernels launched one by one, each timed
+            launch_iteration(b, true);
+            HIPCHK(hipGetLastError());
+        } else {
+            HIPCHK(hipGraphLaunch(b->exec, b->ctx->stream));
+        }
     }
     return GV_OK;
 }
@@ -856,6 +892,29 @@ int gv_batch_release(gv_batch_t b, uint32_t s, uint32_t* bits, uint32_t nwords) 
         if (rc != GV_OK) return rc;
     }
     b->used[s] = 0;
+    return GV_OK;
+}
+
+int gv_batch_profile(gv_batch_t b, int on) {
+    if (!b) return GV_ERR_ARG;
+    int rc = sync(b);
+    if (rc != GV_OK) return rc;
+    if (on && !b->d_kb && hipMalloc(&b->d_kb, 16) != hipSuccess) return GV_ERR_NOMEM;
+    if (on) HIPCHK(hipMemsetAsync(b->d_kb, 0, 16, b->ctx->stream));
+    b->kb_on = on != 0;
+    return capture(b);
+}
+
+int gv_batch_profile_bytes(gv_batch_t b, uint64_t* size_bytes, uint64_t* walk_bytes) {
+    if (!b) return GV_ERR_ARG;
+    uint64_t v[2] = {0, 0};
+    if (b->d_kb) {
+        HIPCHK(hipMemcpyAsync(v, b->d_kb, 16, hipMemcpyDeviceToHost, b->ctx->stream));
+        int rc = sync(b);
+        if (rc != GV_OK) return rc;
+    }
+    if (size_bytes) *size_bytes = v[0];
+    if (walk_bytes) *walk_bytes = v[1];
     return GV_OK;
 }
 

@@ -37,6 +37,9 @@ struct gh_engine {
     // the serving loop's plain-path worker: its own context (own stream)
     gv_ctx_t sideCtx = nullptr;
     std::unique_ptr<geneva::PlainMiner> side;
+    // progress of the last plain-path query (time limit)
+    bool lastFinished = true;
+    double lastFraction = 1.0;
 };
 
 namespace {
@@ -371,6 +374,8 @@ int gh_engine_mine_shard(gh_engine_t e, const char* text, int single_v, uint64_t
         setErr(r.error.empty() ? std::string(gv_status_string(rc)) : r.error);
         return rc;
     }
+    e->lastFinished = r.finished;
+    e->lastFraction = r.fraction;
     if (count) *count = r.count;
     if (alg_bytes) *alg_bytes = r.algBytes;
     if (ms) *ms = r.ms;
@@ -378,6 +383,19 @@ int gh_engine_mine_shard(gh_engine_t e, const char* text, int single_v, uint64_t
     if (bits && nwords) {
         for (uint32_t i = 0; i < nwords; ++i) bits[i] = i < r.graphBits.size() ? r.graphBits[i] : 0;
     }
+    return GV_OK;
+}
+
+int gh_engine_time_limit(gh_engine_t e, double seconds) {
+    if (!e || seconds < 0) return GV_ERR_ARG;
+    e->plain->setTimeLimit(seconds);
+    return GV_OK;
+}
+
+int gh_engine_last_progress(gh_engine_t e, int* finished, double* fraction) {
+    if (!e) return GV_ERR_ARG;
+    if (finished) *finished = e->lastFinished ? 1 : 0;
+    if (fraction) *fraction = e->lastFraction;
     return GV_OK;
 }
 
@@ -505,6 +523,18 @@ int gh_engine_serve(gh_engine_t e, const char* const* queries, const double* arr
     return GV_OK;
 }
 
+int gh_engine_batch_profile(gh_engine_t e, int on) {
+    if (!e) return GV_ERR_ARG;
+    int rc = ensureBatch(e);
+    if (rc != GV_OK) return rc;
+    return gv_batch_profile(e->batch->handle(), on);
+}
+
+int gh_engine_batch_profile_bytes(gh_engine_t e, uint64_t* size_bytes, uint64_t* walk_bytes) {
+    if (!e || !e->batch) return GV_ERR_ARG;
+    return gv_batch_profile_bytes(e->batch->handle(), size_bytes, walk_bytes);
+}
+
 int gh_engine_profile(gh_engine_t e, int on) {
     return e ? gv_profile_enable(e->ctx, on) : GV_ERR_ARG;
 }
@@ -543,7 +573,8 @@ int gh_phase_mine(const char* mygraph, const char* metadata, const char* query_f
     if (arena_bytes) opt.arenaBytes = arena_bytes;
     opt.maxRows = max_rows;
     opt.keepRows = keep_rows != 0;
-    rc = geneva::runPhaseDriver(ctx, mygraph, metadata, query_file, single_v != 0, opt, run->r);
+    rc = geneva::runPhaseDriver(ctx, mygraph, metadata, query_file, single_v < 0 ? -1 : (single_v ? 1 : 0),
+                                opt, run->r);
     gv_ctx_destroy(ctx);
     if (rc != GV_OK) {
         setErr("phase driver: " + std::string(gv_status_string(rc)) + " " + run->r.error + " " + gv_last_error());
@@ -559,6 +590,7 @@ const uint32_t* gh_phase_order(gh_phase_run_t r) { return r ? r->r.order.data() 
 const uint32_t* gh_phase_rows(gh_phase_run_t r) { return r && !r->r.rows.empty() ? r->r.rows.data() : nullptr; }
 uint64_t gh_phase_launches(gh_phase_run_t r) { return r ? r->r.launches : 0; }
 uint64_t gh_phase_spills(gh_phase_run_t r) { return r ? r->r.spills : 0; }
+int gh_phase_single_v(gh_phase_run_t r) { return r ? (r->r.singleV ? 1 : 0) : -1; }
 void gh_phase_free(gh_phase_run_t r) { delete r; }
 
 }  // extern "C"

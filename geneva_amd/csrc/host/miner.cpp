@@ -5,12 +5,26 @@
 #include <cstring>
 
 #include <algorithm>
+#include <chrono>
 #include <sstream>
 
 namespace geneva {
 
 namespace {
 constexpr uint64_t kCountChunk = 1ull << 31;  // candidates per counting launch
+// internal status of a query stopped at its time limit
+constexpr int kTimedOut = -100;
+double nowS() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+}  // namespace
+
+bool PlainMiner::timedOut() const { return deadline_ > 0 && nowS() > deadline_; }
+
+int PlainMiner::snapshot() {
+    if (deadline_ <= 0) return GV_OK;
+    return gv_counters_read(ctx_, &snapCount_, &snapBytes_, snapBits_.empty() ? nullptr : snapBits_.data(),
+                            (uint32_t)snapBits_.size());
 }
 
 bool parseQueryPattern(const std::string& text, QueryPattern& q, std::string* err) {
@@ -230,14 +244,21 @@ int PlainMiner::run(const gv_query_t& q, uint32_t width, const uint32_t* rows, u
     uint64_t total = 0;
     rc = gv_size_rows(ctx_, g_, &st, rows, width, n, fr, &total);
     if (rc != GV_OK || total == 0) return rc;
+    if (width == 1) l1Total_ = total;
     if (last || fusedTail) {
         // count-only: no output table, so any range works; launches are cut at
         // kCountChunk candidates to keep each kernel short
         const gv_step_t* tail = fusedTail ? &q.step[width + 1] : nullptr;
         for (uint64_t c0 = 0; c0 < total; c0 += kCountChunk) {
+            if (timedOut()) return kTimedOut;
             const uint64_t c1 = std::min(total, c0 + kCountChunk);
             out.launches++;
             rc = gv_count_emb(ctx_, g_, &st, tail, rows, width, n, fr, c0, c1);
+            if (rc == GV_OK && deadline_ > 0) rc = gv_ctx_synchronize(ctx_);
+            if (rc == GV_OK && width == 1) {
+                l1Done_ = c1;
+                rc = snapshot();
+            }
             if (rc != GV_OK) return rc;
         }
         return GV_OK;
@@ -246,6 +267,7 @@ int PlainMiner::run(const gv_query_t& q, uint32_t width, const uint32_t* rows, u
     if (rc != GV_OK) return rc;
     Level& nx = lv_[width + 1];
     for (uint64_t c0 = 0; c0 < total; c0 += nx.cap) {
+        if (timedOut()) return kTimedOut;
         uint64_t c1 = std::min(total, c0 + nx.cap);
         uint64_t written = 0;
         out.launches++;
@@ -254,6 +276,10 @@ int PlainMiner::run(const gv_query_t& q, uint32_t width, const uint32_t* rows, u
         if (rc != GV_OK) return rc;
         out.peakRows = std::max(out.peakRows, written);
         rc = run(q, width + 1, nx.rows, written, out);
+        if (rc == GV_OK && width == 1) {
+            l1Done_ = c1;
+            rc = snapshot();
+        }
         if (rc != GV_OK) return rc;
     }
     return GV_OK;
@@ -402,7 +428,29 @@ int PlainMiner::mine(const CompiledQuery& cq, const Shard& sh, MineResult& out) 
                 n = pick.size();
             }
         }
-        if (rc == GV_OK && n) {
+        if (rc == GV_OK && n && limitS_ > 0) {
+            // under a time limit: the chunked path, launch by launch
+            deadline_ = nowS() + limitS_;
+            l1Total_ = l1Done_ = 0;
+            snapCount_ = snapBytes_ = 0;
+            snapBits_.assign(out.graphBits.size(), 0);
+            uint32_t* bits = out.graphBits.empty() ? nullptr : out.graphBits.data();
+            rc = gv_counters_reset(ctx_, g_);
+            if (rc == GV_OK) rc = run(cq.q, 1, rows, n, out);
+            deadline_ = 0;
+            if (rc == kTimedOut) {
+                // what the completed first-extension chunks counted
+                out.finished = false;
+                out.fraction = l1Total_ ? (double)l1Done_ / (double)l1Total_ : 0.0;
+                out.count = snapCount_;
+                out.algBytes = snapBytes_;
+                out.graphBits = snapBits_;
+                rc = gv_ctx_synchronize(ctx_);
+            } else if (rc == GV_OK) {
+                rc = gv_counters_read(ctx_, &out.count, &out.algBytes, bits, (uint32_t)out.graphBits.size());
+            }
+            counted = rc == GV_OK;
+        } else if (rc == GV_OK && n) {
             rc = runLevels(cq, rows, n, out);
             counted = rc == GV_OK;
         }

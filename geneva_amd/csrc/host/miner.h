@@ -53,6 +53,8 @@ struct MineResult {
     uint64_t chunks = 0;         // extension chunks (capacity splits)
     uint64_t peakRows = 0;       // largest materialised table (rows)
     uint64_t overflowRetries = 0;   // whole-query runs redone chunked
+    bool finished = true;        // false: stopped at the deadline (counts are partial)
+    double fraction = 1.0;       // share of the first extension's candidates mined
     std::vector<uint32_t> graphBits;
 };
 
@@ -84,6 +86,15 @@ public:
     // instead of walking its candidates (on by default).
     void setCountByRows(bool on) { countByRows_ = on; }
 
+    // Stop a query once `seconds` of wall time have passed (0 = no limit).
+    // With a limit the query runs on the chunked path, whose launches are
+    // bounded (a level's capacity, or kCountChunk candidates), and the host
+    // waits for each launch; at the limit the query stops after the launch
+    // in flight and MineResult holds the exact count of the embeddings whose
+    // first-extension candidate lies in the part mined (finished = false,
+    // fraction = that part's share of the first extension's candidates).
+    void setTimeLimit(double seconds) { limitS_ = seconds; }
+
 private:
     struct Level {
         uint32_t* rows = nullptr;   // table of width `col` (rows produced by step col-1)
@@ -102,6 +113,14 @@ private:
     uint64_t levelBytes_;
     Level lv_[GV_MAXK + 1];
     bool countByRows_ = true;
+    double limitS_ = 0;
+    double deadline_ = 0;           // steady-clock seconds; 0 = none
+    uint64_t l1Total_ = 0, l1Done_ = 0;
+    // counters after the last completed first-extension chunk (time limit)
+    uint64_t snapCount_ = 0, snapBytes_ = 0;
+    std::vector<uint32_t> snapBits_;
+    bool timedOut() const;
+    int snapshot();
     uint32_t* shardRows_ = nullptr;   // start table of a strided shard
     uint64_t shardCap_ = 0;
 };

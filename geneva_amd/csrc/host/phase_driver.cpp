@@ -482,10 +482,40 @@ int Driver<PhaseT>::run(const std::string& queryFile, bool singleV, PhaseRunResu
 
 }  // namespace
 
+int choosePlanKind(const std::string& queryFile, bool& singleV, std::string* err) {
+    // DV plans whenever the query has a two-vertex phase: the reference's own
+    // timings (scripts/ReadData.py:254-262) show the DV driver at or below the
+    // SV driver for every dataset and DV-phase count from one up; with none
+    // the two plans are the same phase list
+    std::ifstream f(queryFile);
+    if (!f) {
+        if (err) *err = "cannot open " + queryFile;
+        return GV_ERR_ARG;
+    }
+    std::vector<Phase> mo;
+    std::map<uint64_t, uint> rf;
+    std::map<uint, uint> rs, ql;
+    std::map<uint, std::map<uint, bool>> ls;
+    try {
+        planDV(f, true, mo, rf, rs, ls, ql, nullptr, nullptr, nullptr, nullptr);
+    } catch (const PlanError& e) {
+        if (err) *err = e.what;
+        return GV_ERR_ARG;
+    }
+    singleV = calDVPhaseCount(mo) == 0;
+    return GV_OK;
+}
+
 int runPhaseDriver(gv_ctx_t ctx, const std::string& mygraph, const std::string& metadata,
-                   const std::string& queryFile, bool singleV, const PhaseDriverOptions& opt,
+                   const std::string& queryFile, int planKind, const PhaseDriverOptions& opt,
                    PhaseRunResult& out) {
     out = PhaseRunResult();
+    bool singleV = planKind == 1;
+    if (planKind < 0) {
+        int rc0 = choosePlanKind(queryFile, singleV, &out.error);
+        if (rc0 != GV_OK) return rc0;
+    }
+    out.singleV = singleV;
     std::map<uint, uint*> db = loadDataBase(mygraph, metadata);
     if (db.find(0xffffffffu) == db.end()) {
         freeDataBase(db);

@@ -24,13 +24,20 @@ struct PhaseRunResult {
     std::vector<uint32_t> rows;         // count * width (keepRows)
     uint64_t launches = 0;              // launcher calls (one counter read-back each)
     uint64_t spills = 0;                // tables or parts moved to host memory
+    bool singleV = false;               // the plan kind that ran
     std::string error;
 };
 
 // Mine the query in queryFile over the .mygraph / .metadata pair with the DV
-// plan (genMatchOrder) or the SV plan (genMatchOrderSingleV).
+// plan (planKind 0, genMatchOrder), the SV plan (1, genMatchOrderSingleV) or
+// the kind chosen from the DV-phase count (-1, choosePlanKind).
 int runPhaseDriver(gv_ctx_t ctx, const std::string& mygraph, const std::string& metadata,
-                   const std::string& queryFile, bool singleV, const PhaseDriverOptions& opt,
+                   const std::string& queryFile, int planKind, const PhaseDriverOptions& opt,
                    PhaseRunResult& out);
+
+// The plan kind for a query (tools/classifyExtPhase.cpp:27-38 classifies
+// queries by calDVPhaseCount): DV when its DV plan has a two-vertex phase,
+// SV otherwise.
+int choosePlanKind(const std::string& queryFile, bool& singleV, std::string* err);
 
 }  // namespace geneva

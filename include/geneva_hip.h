@@ -303,6 +303,16 @@ int gv_batch_release(gv_batch_t b, uint32_t slot, uint32_t* graph_bits, uint32_t
  * synchronisation for all of them. */
 int gv_batch_release_many(gv_batch_t b, uint32_t n, const uint32_t* slots, uint32_t* bits,
                           uint32_t nwords);
+/* Per-kernel algorithmic bytes of the batch engine: on = 1 re-captures the
+ * iteration with counters of the bytes B charged to k_b_size (rows and back
+ * lists read, rows counted there) and to k_b_walk (rows written, lists of the
+ * counted walk), zeroed here; on = 0 re-captures without them.  While the
+ * context's profiling (gv_profile_enable) is on, gv_batch_run launches the
+ * iteration's kernels one by one, each bracketed by HIP events of its kind
+ * (k_b_size: sizing, k_b_scan_*: scan, k_b_walk: extension, the others:
+ * batch). */
+int gv_batch_profile(gv_batch_t b, int on);
+int gv_batch_profile_bytes(gv_batch_t b, uint64_t* size_bytes, uint64_t* walk_bytes);
 /* Slots, rows a slot table holds at a given width, free slots. */
 int gv_batch_info(gv_batch_t b, uint32_t* slots, uint64_t* slot_bytes, uint32_t* free_slots);
 

@@ -83,6 +83,15 @@ int gh_engine_mine_shard(gh_engine_t e, const char* query_text, int single_v,
                          uint64_t* count, uint64_t* alg_bytes, double* device_ms,
                          uint32_t* graph_bits, uint32_t nwords, uint64_t* launches);
 
+/* Plain path: stop each query after `seconds` of wall time (0 = no limit).
+ * Under a limit a query runs on the chunked path, launch by launch; a query
+ * stopped at the limit returns the exact count, bytes and graph ids of the
+ * embeddings whose first-extension candidate lies in the part mined, and
+ * gh_engine_last_progress reports finished = 0 and that part's share of the
+ * first extension's candidates. */
+int gh_engine_time_limit(gh_engine_t e, double seconds);
+int gh_engine_last_progress(gh_engine_t e, int* finished, double* fraction);
+
 /* Plain path: count a last step with a single back edge per row
  * (gv_count_rows, default on) or by walking its candidates (off). */
 int gh_engine_count_by_rows(gh_engine_t e, int on);
@@ -125,7 +134,9 @@ int gh_engine_serve(gh_engine_t e, const char* const* queries, const double* arr
  * reference planner, one device arena of arena_bytes (0 = 1 GiB) on a context
  * of its own, every phase through the launchers of geneva_launchers.hpp with
  * a counter read-back after each, overflow rows moved to host memory and the
- * launch resumed with isContinue.  max_rows (0 = none) caps maxRowNum so the
+ * launch resumed with isContinue.  single_v: 0 DV plan, 1 SV plan, -1 the
+ * kind chosen from the DV-phase count (DV when the plan has a two-vertex
+ * phase; gh_phase_single_v reports the choice).  max_rows (0 = none) caps maxRowNum so the
  * overflow path can be exercised on small inputs.  The result holds the
  * final rows (keep_rows), spilled parts included; column c holds the data
  * vertex of query vertex order[c]. */
@@ -138,7 +149,13 @@ const uint32_t* gh_phase_order(gh_phase_run_t r);     /* width entries */
 const uint32_t* gh_phase_rows(gh_phase_run_t r);      /* count * width entries, or NULL */
 uint64_t gh_phase_launches(gh_phase_run_t r);
 uint64_t gh_phase_spills(gh_phase_run_t r);
+int gh_phase_single_v(gh_phase_run_t r);
 void gh_phase_free(gh_phase_run_t r);
+
+/* Batch engine: per-kernel algorithmic bytes (gv_batch_profile) on or off,
+ * and the bytes counted since it was turned on. */
+int gh_engine_batch_profile(gh_engine_t e, int on);
+int gh_engine_batch_profile_bytes(gh_engine_t e, uint64_t* size_bytes, uint64_t* walk_bytes);
 
 /* Per-kernel profiling of the engine's context (see gv_profile_* in
  * geneva_hip.h): enable/disable, reset, and read one kernel kind. */

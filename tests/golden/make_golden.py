@@ -6,6 +6,8 @@
                                                   # (--shards-only: the shards alone)
     python tests/golden/make_golden.py --only-w5  # workload 5: the serving query mix (3-10
                                                   # vertices) over the scaled workload-3 database
+    python tests/golden/make_golden.py --only-w3full  # workload 3 at full size (100,000 graphs):
+                                                      # counts, bytes and matched graph ids
 
 Reference values come from the reference's own cpumine, compiled from its
 sources into oracle/_ref/cpumine (`make -C oracle ref`), run on files written
@@ -221,7 +223,27 @@ def workload5(scale=0.01, nq=64):
               open(os.path.join(HERE, "workload5_scaled.json"), "w"), indent=1)
 
 
+def workload3_full():
+    """Workload 3 at full size: the oracle's count, algorithmic bytes and
+    matched graph ids of all 256 queries (graph ids stored as a sorted list
+    and as the SHA-256 of its decimal text, one id per line)."""
+    import hashlib
+    ds = configs.workload3()
+    ogw = O.OracleGraph.of(ds)
+    cases = []
+    for qt in ds.queries:
+        r = ogw.mine(qt, threads=os.cpu_count())
+        ids = sorted(r.graph_ids)
+        cases.append({"query": qt, "count": r.count, "alg_bytes": r.alg_bytes, "ngraphs": len(ids),
+                      "graph_ids_sha256": hashlib.sha256("\n".join(map(str, ids)).encode()).hexdigest()})
+    json.dump({"workload": 3, "scale": 1.0, "cases": cases},
+              open(os.path.join(HERE, "workload3_full.json"), "w"), indent=1)
+    print("workload3 full:", sum(c["count"] for c in cases), "embeddings")
+
+
 def main():
+    if "--only-w3full" in sys.argv:
+        return workload3_full()
     if "--only-w5" in sys.argv:
         return workload5()
     if "--only-full2" in sys.argv:

@@ -318,6 +318,29 @@ def test_oracle_algorithmic_bytes_two_edge_path():
     assert og.mine(q).alg_bytes == expect
 
 
+def test_dv_phase_count_matches_reference_plan_dumps():
+    """gh_dv_phase_count (tools/classifyExtPhase.cpp:27-38: E phases with two
+    extension pairs) equals the count read off the reference cpumine's plan
+    dump of every corpus query, and is 0 exactly when the SV plan has as many
+    E phases as the DV plan."""
+    cases = json.load(open(os.path.join(GOLDEN, "plans.json")))
+    seen_dv = 0
+    for c in cases:
+        lines = c["ref_plan"].splitlines()
+        want = 0
+        for i, l in enumerate(lines):
+            if l.startswith("phase ") and " type E " in l:
+                pairs = lines[i + 1].count("svid:")
+                want += pairs == 2
+        got = G.dv_phase_count(c["query"])
+        assert got == want, c["query"]
+        seen_dv += got > 0
+        ne = lambda sv: sum(1 for l in G.plan_dump(c["query"], single_v=sv).splitlines()
+                            if l.startswith("phase ") and " type E " in l)
+        assert (ne(True) == ne(False)) == (got == 0), c["query"]
+    assert seen_dv > 50
+
+
 def test_golden_counts_workload1_match_reference_cpumine():
     gold = _load("workload1.json")
     ds = configs.workload1()

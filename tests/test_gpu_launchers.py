@@ -46,6 +46,10 @@ def test_workload1_phase_driver_counts_and_rows(w1_files):
         qf = _query_file(d, i, c["query"])
         want = og.rows(c["query"])
         assert len(want) == c["ref_count"]
+        auto = G.phase_mine(os.path.join(d, "w1.mygraph"), os.path.join(d, "w1.metadata"), qf,
+                            single_v=None, keep_rows=False)
+        assert auto.count == c["ref_count"]
+        assert auto.single_v == (G.dv_phase_count(c["query"]) == 0)
         for sv in (True, False):
             # the reference bound, then a bound of 16 rows that makes every
             # phase overflow, spill and resume with isContinue
@@ -81,3 +85,41 @@ def test_enron_phase_driver(require_gpu):
                     assert np.array_equal(G.sorted_rows(r.rows), og.rows(c["query"])), (i, sv)
     finally:
         shutil.rmtree(d, ignore_errors=True)
+
+
+def test_copy_partial_emb_overlapping_and_disjoint(require_gpu):
+    """gv_copy_partial_emb / gv_copyPartialEmb (src/reduction.cu:280-296):
+    after the copy, dst holds what src held, for destinations above and below
+    an overlapping source and for disjoint ranges (numpy's copy semantics)."""
+    import ctypes as C
+    hip = C.CDLL(G.HIP_LIB)
+    ctx = C.c_void_p()
+    assert hip.gv_ctx_create(0, C.byref(ctx)) == 0
+    n_all = 1 << 20
+    buf = C.c_void_p()
+    assert hip.gv_malloc(ctx, C.c_uint64(4 * n_all), C.byref(buf)) == 0
+    try:
+        rng = np.random.default_rng(3)
+        base = rng.integers(1, 1 << 31, size=n_all, dtype=np.uint32)
+        cases = [(1000, 1500, 300_000), (1500, 1000, 300_000), (0, 400_000, 300_000),
+                 (400_000, 0, 300_000), (10, 11, 500_000), (11, 10, 500_000), (5, 5, 100), (7, 9, 0)]
+        for fn in ("gv_copy_partial_emb", "gv_copyPartialEmb"):
+            f = getattr(hip, fn)
+            for s, d, n in cases:
+                print(fn, "src", s, "dst", d, "n", n, flush=True)
+                assert hip.gv_memcpy_h2d(ctx, buf, base.ctypes.data_as(C.c_void_p), C.c_uint64(4 * n_all)) == 0
+                src = C.c_void_p(buf.value + 4 * s)
+                dst = C.c_void_p(buf.value + 4 * d)
+                if fn == "gv_copy_partial_emb":
+                    rc = f(ctx, src, dst, C.c_uint64(n))
+                else:
+                    rc = f(ctx, src, dst, C.c_uint32(n))
+                assert rc == 0
+                out = np.zeros(n_all, dtype=np.uint32)
+                assert hip.gv_memcpy_d2h(ctx, out.ctypes.data_as(C.c_void_p), buf, C.c_uint64(4 * n_all)) == 0
+                want = base.copy()
+                want[d:d + n] = base[s:s + n]
+                assert np.array_equal(out, want), (fn, s, d, n)
+    finally:
+        hip.gv_free(ctx, buf)
+        hip.gv_ctx_destroy(ctx)
