@@ -89,6 +89,9 @@ def lib():
         "gh_phase_free": (None, [vp]),
         "gh_engine_batch_profile": (i32, [vp, i32]),
         "gh_engine_batch_profile_bytes": (i32, [vp, C.POINTER(u64), C.POINTER(u64)]),
+        "gh_engine_batch_pair_start": (i32, [vp, i32]),
+        "gh_engine_batch_history": (i32, [vp, C.POINTER(u64), C.POINTER(u64), u32, C.POINTER(u32)]),
+        "gh_engine_profile_records": (i32, [vp, C.POINTER(i32), C.POINTER(C.c_float), u64, C.POINTER(u64)]),
         "gh_engine_profile": (i32, [vp, i32]),
         "gh_engine_profile_reset": (i32, [vp]),
         "gh_engine_profile_read": (i32, [vp, i32, C.POINTER(u64), C.POINTER(dbl), C.POINTER(u64)]),
@@ -315,7 +318,8 @@ class Engine:
         return lib().gh_engine_graph_bytes(self._h)
 
     KERNELS = {0: "k_size", 1: "k_scan", 2: "k_walk<extend>", 3: "k_walk<count>",
-               4: "batch_iteration", 5: "k_count_rows", 6: "k_isect", 7: "k_count_chain"}
+               4: "batch_iteration", 5: "k_count_rows", 6: "k_isect", 7: "k_count_chain",
+               8: "ref_launchers"}
 
     def batch_config(self, slots=0, slot_bytes=0, row_cap=0, ipt=0, blocks_per_cu=0):
         """Configure the batch engine (0 keeps a setting); rebuilt on next use."""
@@ -409,6 +413,31 @@ class Engine:
         f, x = C.c_int(0), C.c_double(0)
         lib().gh_engine_last_progress(self._h, C.byref(f), C.byref(x))
         return bool(f.value), x.value
+
+    def batch_pair_start(self, on=True):
+        """Start qualifying batch queries on the label-pair edge tables."""
+        rc = lib().gh_engine_batch_pair_start(self._h, int(on))
+        if rc:
+            raise _err("batch_pair_start", rc)
+
+    def batch_history(self):
+        """[(rows sized, candidates walked)] of each batch iteration run while
+        profiling was on; clears the history."""
+        n = C.c_uint32(0)
+        lib().gh_engine_batch_history(self._h, None, None, 0, C.byref(n))
+        r = (C.c_uint64 * max(n.value, 1))()
+        c = (C.c_uint64 * max(n.value, 1))()
+        lib().gh_engine_batch_history(self._h, r, c, n.value, C.byref(n))
+        return list(zip(list(r)[:n.value], list(c)[:n.value]))
+
+    def profile_records(self):
+        """[(kernel kind name, device ms)] of every profiled launch, in order."""
+        n = C.c_uint64(0)
+        lib().gh_engine_profile_records(self._h, None, None, 0, C.byref(n))
+        k = (C.c_int * max(n.value, 1))()
+        m = (C.c_float * max(n.value, 1))()
+        lib().gh_engine_profile_records(self._h, k, m, n.value, C.byref(n))
+        return [(self.KERNELS.get(k[i], str(k[i])), m[i]) for i in range(n.value)]
 
     def count_by_rows(self, on=True):
         """Count single-back-edge last steps per row (default) or by a candidate walk."""

@@ -34,6 +34,10 @@ struct BDesc {               // written by the host at admission
     uint32_t start_n;
     uint32_t has_leaves;     // leaves: counted per row at width leaves.width
     gv_leaves_t leaves;
+    // a first step that only lists the label-pair edges (one back edge to
+    // column 0, at most the restriction below it) starts the slot at width 2
+    // on the graph's label-pair edge table: its rows, never written
+    const uint32_t* init_rows;
 };
 
 struct BState {              // dynamic, polled by the host (mirrors gv_slot_info_t)
@@ -101,6 +105,7 @@ __device__ __forceinline__ unsigned long long* slot_line(const BArgs& a, uint32_
 __device__ __forceinline__ const uint32_t* slot_rows(const BArgs& a, uint32_t s, const BState& S,
                                                      const BDesc& D) {
     if (S.width == 1) return a.lverts + D.start_off;
+    if (S.width == 2 && D.init_rows) return D.init_rows;
     return (S.sel ? a.half1 : a.half0) + (uint64_t)s * a.slot_uints;
 }
 
@@ -567,6 +572,9 @@ struct gv_batch {
     hipGraphExec_t exec = nullptr;
     unsigned long long* d_kb = nullptr;   // per-kernel B (profiling), 2 words
     bool kb_on = false;
+    bool pair_start = true;               // first steps from the label-pair edge tables
+    // rows sized and candidates walked by each profiled iteration
+    std::vector<uint64_t> hist;
 };
 
 namespace {
@@ -823,6 +831,34 @@ int gv_batch_admit_leaves(gv_batch_t b, const gv_query_t* q, const gv_leaves_t* 
     st.state = GV_SLOT_RUNNING;
     st.width = 1;
     st.rows = d.start_n;
+    // the first step from the label-pair edge table: a step that extends
+    // (not the counted last step, not a fused tail, not a leaf core of width
+    // 1) with one back edge, to column 0, and no restriction or the one below
+    // column 0 of a same-label step
+    const gv_graph* G = b->g;
+    const gv_step_t& s1 = q->step[1];
+    const bool extends = q->k >= 3 && !(q->k == 3 && q->step[2].nback == 1) && !(leaves && leaves->width == 1);
+    const uint32_t la = q->label0, lb = s1.label, NL = G->nlabels;
+    if (b->pair_start && extends && s1.nback == 1 && s1.back_col[0] == 0 &&
+        (s1.restrict_col == GV_NONE || (s1.restrict_col == 0 && la == lb)) && !G->pe_off.empty()) {
+        const uint32_t p = s1.back_part[0];
+        const uint64_t e0 = G->pe_off[((uint64_t)p * NL + la) * NL + lb];
+        const uint64_t e1 = G->pe_off[((uint64_t)p * NL + la) * NL + lb + 1];
+        uint64_t r0 = e0, r1 = e1;
+        const uint32_t* base = G->d_pe;
+        if (s1.restrict_col == 0) {
+            r0 = G->pel_off[(uint64_t)p * NL + la];
+            r1 = G->pel_off[(uint64_t)p * NL + la + 1];
+            base = G->d_pel;
+        }
+        d.init_rows = base + 2 * r0;
+        st.width = 2;
+        st.rows = r1 - r0;
+        // the first step's algorithmic bytes: every start row read (4) with
+        // its row-pointer pair (8), its label-lb segment (4 per entry) and
+        // each row it writes (8)
+        st.bytes = 12ull * d.start_n + 4ull * (e1 - e0) + 8ull * (r1 - r0);
+    }
     b->dirty[s] = 1;
     b->used[s] = 1;
     *slot = s;
@@ -835,9 +871,16 @@ int gv_batch_run(gv_batch_t b, uint32_t iterations) {
     if (rc != GV_OK) return rc;
     for (uint32_t i = 0; i < iterations; ++i) {
         if (b->ctx->prof) {
-            // profiling: the same kernels launched one by one, each timed
+            // profiling: the same kernels launched one by one, each timed,
+            // and the iteration's extents read back
             launch_iteration(b, true);
             HIPCHK(hipGetLastError());
+            uint64_t g[4];
+            HIPCHK(hipMemcpyAsync(g, b->d_glob, sizeof(g), hipMemcpyDeviceToHost, b->ctx->stream));
+            int rc2 = sync(b);
+            if (rc2 != GV_OK) return rc2;
+            b->hist.push_back(g[0]);
+            b->hist.push_back(g[1]);
         } else {
             HIPCHK(hipGraphLaunch(b->exec, b->ctx->stream));
         }
@@ -915,6 +958,23 @@ int gv_batch_profile_bytes(gv_batch_t b, uint64_t* size_bytes, uint64_t* walk_by
     }
     if (size_bytes) *size_bytes = v[0];
     if (walk_bytes) *walk_bytes = v[1];
+    return GV_OK;
+}
+
+int gv_batch_pair_start(gv_batch_t b, int on) {
+    if (!b) return GV_ERR_ARG;
+    b->pair_start = on != 0;
+    return GV_OK;
+}
+
+int gv_batch_history(gv_batch_t b, uint64_t* rows, uint64_t* candidates, uint32_t cap, uint32_t* n) {
+    if (!b || !n) return GV_ERR_ARG;
+    *n = (uint32_t)(b->hist.size() / 2);
+    for (uint32_t i = 0; i < *n && i < cap; ++i) {
+        if (rows) rows[i] = b->hist[2 * i];
+        if (candidates) candidates[i] = b->hist[2 * i + 1];
+    }
+    if (rows || candidates) b->hist.clear();
     return GV_OK;
 }
 

@@ -1141,6 +1141,12 @@ struct gv_graph {
     uint32_t* d_lverts = nullptr;
     uint32_t* d_gof = nullptr;
     uint32_t ngraphs = 0;
+    // label-pair edge tables (rows (v, u) of width 2): pe for partition p,
+    // labels a, b at rows [pe_off[(p*NL + a)*NL + b], pe_off[... + 1]); pel
+    // (u < v, labels a = a) at rows [pel_off[p*NL + a], pel_off[p*NL + a + 1])
+    std::vector<uint64_t> pe_off, pel_off;
+    uint32_t* d_pe = nullptr;
+    uint32_t* d_pel = nullptr;
 };
 
 struct gv_frontier {

@@ -1255,10 +1255,60 @@ int gv_graph_create(gv_ctx_t c, const uint32_t* vLabel, uint32_t nblocks,
             for (uint64_t i = 0; i < spn; ++i) plane[i] = lo[(uint64_t)lverts[i] * nl1 + l];
         }
     }
+    // label-pair edge tables: for partition p and labels a, b, the rows
+    // (v, u) with label(v) = a, u a label-b neighbour of v, v then u
+    // ascending (pe), and for a = b the rows with u < v (pel): the width-2
+    // table of a first step from label a, unrestricted or restricted below
+    // column 0
+    const uint32_t np1 = std::max<uint32_t>(nblocks, 1);
+    std::vector<uint64_t> pe_off((uint64_t)np1 * NL * NL + 1, 0), pel_off((uint64_t)np1 * NL + 1, 0);
+    for (uint32_t p = 0; p < np1; ++p) {
+        const uint32_t* lo = loff.data() + (uint64_t)p * nv1 * nl1;
+        for (uint32_t a = 0; a < NL; ++a) {
+            uint64_t low = 0;
+            for (uint64_t i = lvoff[a]; i < lvoff[a + 1]; ++i) {
+                const uint64_t v = lverts[i];
+                const uint32_t b0 = lo[v * nl1 + a], b1 = lo[v * nl1 + a + 1];
+                low += std::lower_bound(nbr.begin() + b0, nbr.begin() + b1, (uint32_t)v) - (nbr.begin() + b0);
+            }
+            pel_off[(uint64_t)p * NL + a + 1] = low;
+            for (uint32_t b = 0; b < NL; ++b)
+                pe_off[((uint64_t)p * NL + a) * NL + b + 1] = pairs[((uint64_t)p * NL + a) * NL + b];
+        }
+    }
+    for (uint64_t i = 1; i < pe_off.size(); ++i) pe_off[i] += pe_off[i - 1];
+    for (uint64_t i = 1; i < pel_off.size(); ++i) pel_off[i] += pel_off[i - 1];
+    std::vector<uint32_t> pe(2 * pe_off.back() + 2, 0u), pel(2 * pel_off.back() + 2, 0u);
+    for (uint32_t p = 0; p < np1; ++p) {
+        const uint32_t* lo = loff.data() + (uint64_t)p * nv1 * nl1;
+        for (uint32_t a = 0; a < NL; ++a) {
+            uint64_t* cur = &pe_off[((uint64_t)p * NL + a) * NL];
+            std::vector<uint64_t> at(cur, cur + NL);
+            uint64_t lat = pel_off[(uint64_t)p * NL + a];
+            for (uint64_t i = lvoff[a]; i < lvoff[a + 1]; ++i) {
+                const uint32_t v = lverts[i];
+                for (uint32_t b = 0; b < NL; ++b) {
+                    const uint32_t s0 = lo[(uint64_t)v * nl1 + b], s1 = lo[(uint64_t)v * nl1 + b + 1];
+                    for (uint32_t s = s0; s < s1; ++s) {
+                        pe[2 * at[b]] = v;
+                        pe[2 * at[b] + 1] = nbr[s];
+                        ++at[b];
+                        if (b == a && nbr[s] < v) {
+                            pel[2 * lat] = v;
+                            pel[2 * lat + 1] = nbr[s];
+                            ++lat;
+                        }
+                    }
+                }
+            }
+        }
+    }
     gv_graph* g = new gv_graph();
     g->ctx = c;
     g->nv1 = nv1;
     g->sp_n = spn;
+    g->pe_off = pe_off;
+    g->pel_off = pel_off;
     g->nlabels = NL;
     g->nparts = nblocks;
     g->nnbr = nbr.size();
@@ -1281,6 +1331,8 @@ int gv_graph_create(gv_ctx_t c, const uint32_t* vLabel, uint32_t nblocks,
     up(&g->d_nbr, nbr.data(), nbr.size() * sizeof(uint32_t));
     up(&g->d_sp, sp.data(), sp.size() * sizeof(uint32_t));
     up(&g->d_lverts, lverts.data(), lverts.size() * sizeof(uint32_t));
+    up(&g->d_pe, pe.data(), pe.size() * sizeof(uint32_t));
+    up(&g->d_pel, pel.data(), pel.size() * sizeof(uint32_t));
     if (rc != GV_OK) {
         gv_graph_destroy(g);
         return rc;
@@ -1296,6 +1348,8 @@ int gv_graph_destroy(gv_graph_t g) {
     if (g->d_nbr) hipFree(g->d_nbr);
     if (g->d_sp) hipFree(g->d_sp);
     if (g->d_lverts) hipFree(g->d_lverts);
+    if (g->d_pe) hipFree(g->d_pe);
+    if (g->d_pel) hipFree(g->d_pel);
     if (g->d_gof) hipFree(g->d_gof);
     delete g;
     return GV_OK;
@@ -2129,6 +2183,21 @@ int gv_profile_read(gv_ctx_t c, int kind, uint64_t* launches, double* total_ms,
     if (launches) *launches = n;
     if (total_ms) *total_ms = ms;
     if (req_bytes) *req_bytes = rq;
+    return GV_OK;
+}
+
+int gv_profile_records(gv_ctx_t c, int* kinds, float* ms, uint64_t cap, uint64_t* n) {
+    if (!c || !n) return GV_ERR_ARG;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    *n = c->recs.size();
+    for (uint64_t i = 0; i < c->recs.size() && i < cap; ++i) {
+        if (kinds) kinds[i] = c->recs[i].kind;
+        if (ms) {
+            float t = 0;
+            HIPCHK(hipEventElapsedTime(&t, c->recs[i].a, c->recs[i].b));
+            ms[i] = t;
+        }
+    }
     return GV_OK;
 }
 

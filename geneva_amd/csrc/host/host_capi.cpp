@@ -535,6 +535,22 @@ int gh_engine_batch_profile_bytes(gh_engine_t e, uint64_t* size_bytes, uint64_t*
     return gv_batch_profile_bytes(e->batch->handle(), size_bytes, walk_bytes);
 }
 
+int gh_engine_batch_pair_start(gh_engine_t e, int on) {
+    if (!e) return GV_ERR_ARG;
+    int rc = ensureBatch(e);
+    if (rc != GV_OK) return rc;
+    return gv_batch_pair_start(e->batch->handle(), on);
+}
+
+int gh_engine_batch_history(gh_engine_t e, uint64_t* rows, uint64_t* candidates, uint32_t cap, uint32_t* n) {
+    if (!e || !e->batch) return GV_ERR_ARG;
+    return gv_batch_history(e->batch->handle(), rows, candidates, cap, n);
+}
+
+int gh_engine_profile_records(gh_engine_t e, int* kinds, float* ms, uint64_t cap, uint64_t* n) {
+    return e ? gv_profile_records(e->ctx, kinds, ms, cap, n) : GV_ERR_ARG;
+}
+
 int gh_engine_profile(gh_engine_t e, int on) {
     return e ? gv_profile_enable(e->ctx, on) : GV_ERR_ARG;
 }

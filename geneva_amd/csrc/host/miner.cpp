@@ -266,9 +266,14 @@ int PlainMiner::run(const gv_query_t& q, uint32_t width, const uint32_t* rows, u
     rc = ensureRows(width + 1);
     if (rc != GV_OK) return rc;
     Level& nx = lv_[width + 1];
-    for (uint64_t c0 = 0; c0 < total; c0 += nx.cap) {
+    // under a time limit the first extension goes in at least 256 pieces, so
+    // a stopped query reports its progress in steps of at most 1/256
+    const uint64_t piece = deadline_ > 0 && width == 1
+                               ? std::min<uint64_t>(nx.cap, std::max<uint64_t>(1, (total + 255) / 256))
+                               : nx.cap;
+    for (uint64_t c0 = 0; c0 < total; c0 += piece) {
         if (timedOut()) return kTimedOut;
-        uint64_t c1 = std::min(total, c0 + nx.cap);
+        uint64_t c1 = std::min(total, c0 + piece);
         uint64_t written = 0;
         out.launches++;
         out.chunks++;

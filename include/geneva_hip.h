@@ -313,6 +313,17 @@ int gv_batch_release_many(gv_batch_t b, uint32_t n, const uint32_t* slots, uint3
  * batch). */
 int gv_batch_profile(gv_batch_t b, int on);
 int gv_batch_profile_bytes(gv_batch_t b, uint64_t* size_bytes, uint64_t* walk_bytes);
+/* Queries admitted from now on start on the graph's label-pair edge table
+ * when their first step only lists the edges between two labels (one back
+ * edge to column 0; no restriction, or the one below column 0 of a
+ * same-label step) and is not their counted last step: the slot starts at
+ * width 2 with that table as its input, the first iteration is saved, and
+ * the step's algorithmic bytes are added in closed form.  On by default. */
+int gv_batch_pair_start(gv_batch_t b, int on);
+/* Rows sized and candidates walked by each iteration run while the
+ * context's profiling was on (oldest first; cleared when read into rows or
+ * candidates). */
+int gv_batch_history(gv_batch_t b, uint64_t* rows, uint64_t* candidates, uint32_t cap, uint32_t* n);
 /* Slots, rows a slot table holds at a given width, free slots. */
 int gv_batch_info(gv_batch_t b, uint32_t* slots, uint64_t* slot_bytes, uint32_t* free_slots);
 
@@ -346,6 +357,10 @@ int gv_profile_units(gv_ctx_t ctx, int kind, uint64_t* units);
 /* Algorithmic bytes B (the reference figure, accumulated exactly as for the
  * counters) of a kind since the last reset, with on = 2. */
 int gv_profile_alg_bytes(gv_ctx_t ctx, int kind, uint64_t* bytes);
+
+/* Every profiled launch since the last reset, in launch order: its kind and
+ * device milliseconds (up to cap of them; *n receives the number). */
+int gv_profile_records(gv_ctx_t ctx, int* kinds, float* ms, uint64_t cap, uint64_t* n);
 
 /* Timing on the context's stream (HIP events): record a mark, and the
  * milliseconds between two marks (synchronises on the second). */

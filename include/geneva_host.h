@@ -156,6 +156,14 @@ void gh_phase_free(gh_phase_run_t r);
  * and the bytes counted since it was turned on. */
 int gh_engine_batch_profile(gh_engine_t e, int on);
 int gh_engine_batch_profile_bytes(gh_engine_t e, uint64_t* size_bytes, uint64_t* walk_bytes);
+/* Batch engine: first steps from the label-pair edge tables on or off
+ * (gv_batch_pair_start). */
+int gh_engine_batch_pair_start(gh_engine_t e, int on);
+/* Rows sized and candidates walked per profiled batch iteration
+ * (gv_batch_history). */
+int gh_engine_batch_history(gh_engine_t e, uint64_t* rows, uint64_t* candidates, uint32_t cap, uint32_t* n);
+/* Every profiled launch in order: kind and device ms (gv_profile_records). */
+int gh_engine_profile_records(gh_engine_t e, int* kinds, float* ms, uint64_t cap, uint64_t* n);
 
 /* Per-kernel profiling of the engine's context (see gv_profile_* in
  * geneva_hip.h): enable/disable, reset, and read one kernel kind. */

@@ -97,6 +97,12 @@ def test_tunings_agree(w3):
     print("autotune", tuned, flush=True)
     res, _ = eng.mine_batch(qs)
     assert [r.count for r in res] == want
+    # first steps from the start tables instead of the label-pair edge tables
+    eng.batch_pair_start(False)
+    res2, _ = eng.mine_batch(qs)
+    eng.batch_pair_start(True)
+    assert [r.count for r in res2] == want
+    assert [r.alg_bytes for r in res2] == [r.alg_bytes for r in res]
 
 
 def test_serving_results_match(w3):
