@@ -59,6 +59,9 @@ def lib():
         "gh_ds_nqueries": (u32, [vp]),
         "gh_ds_query": (cs, [vp, u32]),
         "gh_ds_write": (i32, [vp, cs, i32]),
+        "gh_ds_reorder_intervals": (i32, [vp]),
+        "gh_ds_partition_runs": (i32, [vp, _u32p(), _u32p(), u32]),
+        "gh_ds_old_ids": (_u32p(), [vp]),
         "gh_plan_dump": (u64, [cs, i32, C.c_char_p, u64]),
         "gh_dv_phase_count": (i32, [cs]),
         "gh_engine_create": (vp, [vp, i32, u64, C.POINTER(i32)]),
@@ -190,6 +193,29 @@ class Dataset:
     def queries(self):
         L = lib()
         return [L.gh_ds_query(self._h, i).decode() for i in range(L.gh_ds_nqueries(self._h))]
+
+    def reorder_intervals(self):
+        """Renumber vertices so every edge-label partition's sources form at
+        most 256 id runs; returns the largest run count."""
+        rc = lib().gh_ds_reorder_intervals(self._h)
+        if rc < 0:
+            raise _err("reorder_intervals", rc)
+        return rc
+
+    def partition_runs(self):
+        """{edge label: source id runs its partition block needs}."""
+        n = lib().gh_ds_partition_runs(self._h, None, None, 0)
+        lab = (C.c_uint32 * max(n, 1))()
+        run = (C.c_uint32 * max(n, 1))()
+        lib().gh_ds_partition_runs(self._h, lab, run, n)
+        return {lab[i]: run[i] for i in range(n)}
+
+    @property
+    def old_ids(self):
+        p = lib().gh_ds_old_ids(self._h)
+        if not p:
+            return np.arange(self.nv + 1, dtype=np.uint32)
+        return np.ctypeslib.as_array(p, shape=(self.nv + 1,)).copy()
 
     def write(self, prefix, skip_index=False):
         rc = lib().gh_ds_write(self._h, prefix.encode(), int(skip_index))

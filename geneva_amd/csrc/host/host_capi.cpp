@@ -231,6 +231,42 @@ int gh_ds_write(gh_dataset_t d, const char* prefix, int skip_index) {
     return GV_OK;
 }
 
+int gh_ds_reorder_intervals(gh_dataset_t d) {
+    if (!d) return GV_ERR_ARG;
+    std::vector<uint> newOf;
+    std::string err;
+    const uint worst = geneva::reorderForIntervals(d->ds.g, newOf, &err);
+    if (worst == 0 && !err.empty()) {
+        setErr("reorder: " + err);
+        return GV_ERR_FORMAT;
+    }
+    d->ds.g = geneva::relabel(d->ds.g, newOf);
+    if (d->ds.graphOf.size() == newOf.size()) {
+        std::vector<uint32_t> go(newOf.size(), 0);
+        for (size_t v = 1; v < newOf.size(); ++v) go[newOf[v]] = d->ds.graphOf[v];
+        d->ds.graphOf.swap(go);
+    }
+    return (int)worst;
+}
+
+int gh_ds_partition_runs(gh_dataset_t d, uint32_t* labels, uint32_t* runs, uint32_t cap) {
+    if (!d) return GV_ERR_ARG;
+    auto m = geneva::partitionRuns(d->ds.g);
+    uint32_t i = 0;
+    for (auto& kv : m) {
+        if (i < cap) {
+            if (labels) labels[i] = kv.first;
+            if (runs) runs[i] = kv.second;
+        }
+        ++i;
+    }
+    return (int)i;
+}
+
+const uint32_t* gh_ds_old_ids(gh_dataset_t d) {
+    return d && !d->ds.g.oldId.empty() ? d->ds.g.oldId.data() : nullptr;
+}
+
 uint64_t gh_plan_dump(const char* text, int single_v, char* out, uint64_t cap) {
     if (!text) return 0;
     std::map<uint64_t, uint> rf;

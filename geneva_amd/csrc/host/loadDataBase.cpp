@@ -241,9 +241,106 @@ bool writeGraphText(const std::string& path, const DataGraph& g) {
 bool writeMetadata(const std::string& path, const DataGraph& g) {
     FILE* f = fopen(path.c_str(), "w");
     if (!f) return false;
-    for (uint v = 1; v <= g.nv; ++v) fprintf(f, "%u %u %u\n", v, g.vlabel[v], v);
+    for (uint v = 1; v <= g.nv; ++v)
+        fprintf(f, "%u %u %u\n", v, g.vlabel[v], g.oldId.size() > v ? g.oldId[v] : v);
     fclose(f);
     return true;
+}
+
+namespace {
+// each vertex's edge-label signature: bit b set when it has an edge of the
+// b-th label of `labels`
+std::vector<uint64_t> signatures(const DataGraph& g, const std::vector<uint>& labels) {
+    std::vector<uint64_t> sig(g.nv + 1, 0);
+    for (auto& e : g.edges) {
+        if (e.a == e.b) continue;
+        const uint64_t bit = 1ull << (std::lower_bound(labels.begin(), labels.end(), e.label) - labels.begin());
+        sig[e.a] |= bit;
+        sig[e.b] |= bit;
+    }
+    return sig;
+}
+std::vector<uint> edgeLabels(const DataGraph& g) {
+    std::vector<uint> labels;
+    for (auto& e : g.edges)
+        if (e.a != e.b) labels.push_back(e.label);
+    std::sort(labels.begin(), labels.end());
+    labels.erase(std::unique(labels.begin(), labels.end()), labels.end());
+    return labels;
+}
+}  // namespace
+
+std::map<uint, uint> partitionRuns(const DataGraph& g) {
+    const std::vector<uint> labels = edgeLabels(g);
+    std::map<uint, uint> runs;
+    if (labels.size() > 64) return runs;
+    const std::vector<uint64_t> sig = signatures(g, labels);
+    for (size_t b = 0; b < labels.size(); ++b) {
+        uint r = 0;
+        bool in = false;
+        for (uint v = 1; v <= g.nv; ++v) {
+            const bool s = (sig[v] >> b) & 1;
+            if (s && !in) ++r;
+            in = s;
+        }
+        runs[labels[b]] = r;
+    }
+    return runs;
+}
+
+uint reorderForIntervals(const DataGraph& g, std::vector<uint>& newOf, std::string* err) {
+    const std::vector<uint> labels = edgeLabels(g);
+    const size_t K = labels.size();
+    if (K > 64) {
+        if (err) *err = "more than 64 edge labels";
+        return 0;
+    }
+    const std::vector<uint64_t> sig = signatures(g, labels);
+    // bit positions: the label with the most sources takes the highest bit
+    std::vector<uint64_t> srcs(K, 0);
+    for (uint v = 1; v <= g.nv; ++v)
+        for (size_t b = 0; b < K; ++b) srcs[b] += (sig[v] >> b) & 1;
+    std::vector<size_t> byCount(K);
+    for (size_t b = 0; b < K; ++b) byCount[b] = b;
+    std::stable_sort(byCount.begin(), byCount.end(), [&](size_t x, size_t y) { return srcs[x] > srcs[y]; });
+    std::vector<uint> bitOf(K);
+    for (size_t i = 0; i < K; ++i) bitOf[byCount[i]] = (uint)(K - 1 - i);
+    // rank of a signature read as a Gray code: prefix XOR from the top bit
+    auto rank = [&](uint64_t s) {
+        uint64_t code = 0;
+        for (size_t b = 0; b < K; ++b)
+            if ((s >> b) & 1) code |= 1ull << bitOf[b];
+        uint64_t r = code;
+        for (uint sh = 1; sh < 64; sh <<= 1) r ^= r >> sh;
+        return r;
+    };
+    std::vector<std::pair<uint64_t, uint>> key;
+    key.reserve(g.nv);
+    for (uint v = 1; v <= g.nv; ++v) key.emplace_back(rank(sig[v]), v);
+    std::sort(key.begin(), key.end());
+    newOf.assign(g.nv + 1, 0);
+    for (uint i = 0; i < key.size(); ++i) newOf[key[i].second] = i + 1;
+    uint worst = 0;
+    for (auto& kv : partitionRuns(relabel(g, newOf))) worst = std::max(worst, kv.second);
+    if (worst > 256) {
+        if (err) *err = "a partition needs " + std::to_string(worst) + " source intervals after reordering";
+        return 0;
+    }
+    return worst;
+}
+
+DataGraph relabel(const DataGraph& g, const std::vector<uint>& newOf) {
+    DataGraph h;
+    h.nv = g.nv;
+    h.vlabel.assign(g.nv + 1, 0);
+    h.oldId.assign(g.nv + 1, 0);
+    for (uint v = 1; v <= g.nv; ++v) {
+        h.vlabel[newOf[v]] = g.vlabel[v];
+        h.oldId[newOf[v]] = g.oldId.size() > v ? g.oldId[v] : v;
+    }
+    h.edges.reserve(g.edges.size());
+    for (auto& e : g.edges) h.edges.push_back({newOf[e.a], newOf[e.b], e.label});
+    return h;
 }
 
 bool writeMyGraph(const std::string& path, const DataGraph& g, std::string* err) {

@@ -30,7 +30,33 @@ struct DataGraph {
     uint nv = 0;
     std::vector<uint> vlabel;  // size nv+1
     std::vector<Edge> edges;
+    // original id of each vertex (size nv+1) after reorderForIntervals; empty
+    // while ids are the original ones (written as the .metadata third column)
+    std::vector<uint> oldId;
 };
+
+// New vertex ids under which every edge-label partition's source vertices
+// form at most 256 runs of consecutive ids, the limit of the partition block
+// (tools/genDatabase.cpp:451-460; the reference reorders ids for the same
+// purpose, tools/genDatabase.cpp:259-413).  Each vertex's signature is the set
+// of edge labels it has edges of; vertices are ordered by the rank of their
+// signature read as a binary-reflected Gray code (bit order: the partition
+// with the most sources in the highest bit), then by id, and isolated
+// vertices come first.  In that order the i-th largest partition's bit
+// (i = 0 for the largest) is set in at most max(1, 2^(i-1)) runs over all
+// 2^K signatures, and dropping absent signatures only merges runs: always
+// within 256 for K <= 10 partitions, and for any K when the signatures
+// present are few.  newOf[old] = new id (newOf[0] = 0).  Returns the largest run count; err is set and 0
+// returned when some partition would still need more than 256 runs or
+// there are more than 64 edge labels.
+uint reorderForIntervals(const DataGraph& g, std::vector<uint>& newOf, std::string* err);
+
+// The graph under new ids (vertex labels and edges moved; oldId records the
+// original ids).
+DataGraph relabel(const DataGraph& g, const std::vector<uint>& newOf);
+
+// Source runs (intervals) each edge label's partition block would need.
+std::map<uint, uint> partitionRuns(const DataGraph& g);
 
 // The partition block of one edge label in the .mygraph layout
 // (tools/genDatabase.cpp:437-580): u0..u4 header, array0 (interval starts and

@@ -55,6 +55,23 @@ const char* gh_ds_query(gh_dataset_t ds, uint32_t i);
  * groups unless skip_index) and prefix_<i>.query. */
 int gh_ds_write(gh_dataset_t ds, const char* prefix, int skip_index);
 
+/* Renumber the data set's vertices so that every edge-label partition's
+ * source vertices form at most 256 runs of consecutive ids, the limit of the
+ * partition block (tools/genDatabase.cpp:451-460; the reference reorders for
+ * the same purpose, tools/genDatabase.cpp:259-413): vertices ordered by their
+ * edge-label signature as a Gray code (reorderForIntervals in
+ * loadDataBase.h).  Vertex labels, edges and owner graphs move with their
+ * vertices; the original ids are kept (gh_ds_old_ids, and the third column of
+ * the .metadata gh_ds_write writes).  Returns the largest run count, or a
+ * negative status when some partition would still need more than 256. */
+int gh_ds_reorder_intervals(gh_dataset_t ds);
+/* Source runs each edge label's partition needs under the current ids:
+ * labels[i], runs[i] for i < min(cap, returned count). */
+int gh_ds_partition_runs(gh_dataset_t ds, uint32_t* labels, uint32_t* runs, uint32_t cap);
+/* Original id of each vertex after gh_ds_reorder_intervals (nv+1 entries),
+ * or NULL. */
+const uint32_t* gh_ds_old_ids(gh_dataset_t ds);
+
 /* ---- planner ---------------------------------------------------------------
  * Plan a query with genMatchOrder (single_v = 0) or genMatchOrderSingleV and
  * print it in the layout the reference cpumine prints

@@ -428,3 +428,57 @@ def test_mygraph_writer_round_trip(tmp_path):
     norm = lambda E: {(min(int(a), int(b)), max(int(a), int(b)), int(l)) for a, b, l in E}
     assert norm(back.edges) == norm(e)
     assert np.array_equal(back.vlabel[1:], vl[1:])
+
+
+def test_reorder_intervals_fits_scattered_multi_label_sources(tmp_path):
+    """A graph whose eight edge labels give every partition thousands of
+    scattered source runs: the partition block writer refuses it (more than
+    256 intervals, tools/genDatabase.cpp:454-460), the Gray-code renumbering
+    brings every partition within 256 runs, the renumbered files load back to
+    the same graph under the recorded id map, and mining the loaded files
+    gives the oracle's counts."""
+    rng = np.random.default_rng(44)
+    n = 3000
+    vl = np.concatenate([[0], rng.integers(1, 4, size=n)]).astype(np.uint32)
+    m = 12000
+    a = rng.integers(1, n + 1, size=m)
+    b = rng.integers(1, n + 1, size=m)
+    lab = rng.integers(1, 9, size=m)
+    keep = a != b
+    edges = np.stack([a[keep], b[keep], lab[keep]], axis=1).astype(np.uint32)
+    ds = G.Dataset.from_arrays(vl, edges)
+    before = ds.partition_runs()
+    assert max(before.values()) > 256
+    with pytest.raises(G.GenevaError):
+        ds.write(str(tmp_path / "raw"))
+    worst = ds.reorder_intervals()
+    after = ds.partition_runs()
+    assert worst == max(after.values()) <= 256
+    assert sorted(after) == sorted(before)
+    old = ds.old_ids
+    assert sorted(old[1:].tolist()) == list(range(1, n + 1))
+    # same graph under the id map: labels and the edge set
+    assert np.array_equal(ds.vlabel[1:], vl[old[1:]])
+    new_of = np.zeros(n + 1, dtype=np.int64)
+    new_of[old] = np.arange(n + 1)
+    want = {(min(new_of[x], new_of[y]), max(new_of[x], new_of[y]), l) for x, y, l in edges.tolist()}
+    got = {(min(x, y), max(x, y), l) for x, y, l in ds.edges.tolist()}
+    assert got == want
+    # the written files: headers within 256 intervals, a round trip through
+    # loadDataBase, the .metadata's third column the original id
+    ds.write(str(tmp_path / "re"))
+    raw = np.fromfile(str(tmp_path / "re.mygraph"), dtype=np.uint32)
+    off, nblk = 1, 0
+    while off < len(raw):
+        assert raw[off + 2] <= 256
+        off += raw[off]
+        nblk += 1
+    assert nblk == len(after)
+    meta = np.loadtxt(str(tmp_path / "re.metadata"), dtype=np.int64)
+    assert np.array_equal(meta[:, 2], old[1:])
+    back = G.Dataset.load(str(tmp_path / "re.mygraph"), str(tmp_path / "re.metadata"))
+    assert {(min(x, y), max(x, y), l) for x, y, l in back.edges.tolist()} == got
+    og_a, og_b = O.OracleGraph.of(ds), O.OracleGraph.of(back)
+    for q in ("v 0 1\nv 1 2\ne 0 1 3\n", "v 0 1\nv 1 2\nv 2 3\ne 0 1 1\ne 1 2 2\n",
+              "v 0 1\nv 1 1\nv 2 2\ne 0 1 4\ne 1 2 5\ne 0 2 6\n"):
+        assert og_a.mine(q).count == og_b.mine(q).count
