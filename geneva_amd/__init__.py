@@ -79,6 +79,7 @@ def lib():
                                        _u32p(), u32, C.POINTER(dbl)]),
         "gh_engine_autotune": (i32, [vp, C.POINTER(cs), u32, C.POINTER(u32), C.POINTER(u32),
                                      C.POINTER(dbl)]),
+        "gh_engine_autotune_classes": (u64, [vp, C.c_char_p, u64]),
         "gh_engine_serve": (i32, [vp, C.POINTER(cs), C.POINTER(dbl), u32, i32, u64, C.POINTER(dbl),
                                   C.POINTER(u64), _u32p(), u32, C.POINTER(dbl)]),
         "gh_phase_mine": (i32, [cs, cs, cs, i32, i32, u64, u32, i32, C.POINTER(vp)]),
@@ -384,7 +385,15 @@ class Engine:
                                       C.byref(bpc), C.byref(ms))
         if rc:
             raise _err("autotune", rc)
-        return {"ipt": ipt.value, "blocks_per_cu": bpc.value, "device_ms": ms.value}
+        n = lib().gh_engine_autotune_classes(self._h, None, 0)
+        buf = C.create_string_buffer(n + 1)
+        lib().gh_engine_autotune_classes(self._h, buf, n + 1)
+        classes = {}
+        for ent in buf.value.decode().split(";"):
+            if ent:
+                k, i, m = ent.split(":")
+                classes[k] = {"ipt": int(i), "device_ms": round(float(m), 4)}
+        return {"ipt": ipt.value, "blocks_per_cu": bpc.value, "device_ms": ms.value, "classes": classes}
 
     def serve(self, queries, arrival_s, batched=True, mem_budget=16 << 30, with_graphs=False):
         """Serve requests arriving at arrival_s (seconds).  Returns (latency_ms,

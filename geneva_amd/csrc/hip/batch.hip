@@ -38,6 +38,9 @@ struct BDesc {               // written by the host at admission
     // column 0, at most the restriction below it) starts the slot at width 2
     // on the graph's label-pair edge table: its rows, never written
     const uint32_t* init_rows;
+    // candidates per walk tile of this slot (64 x its candidates per lane, at
+    // most the kernel's); 0 = the batch's
+    uint32_t tile;
 };
 
 struct BState {              // dynamic, polled by the host (mirrors gv_slot_info_t)
@@ -416,7 +419,8 @@ __global__ __launch_bounds__(1024) void k_b_tiles(BArgs a) {
             const uint64_t re = S.row_end, rb = re - rows;
             cb = rb ? a.csum[rb - 1] : 0;
             ce = a.csum[re - 1];
-            nt = (ce - cb + a.tile - 1) / a.tile;
+            const uint64_t tl = a.desc[s].tile ? a.desc[s].tile : a.tile;
+            nt = (ce - cb + tl - 1) / tl;
         }
     }
     uint64_t tot;
@@ -471,10 +475,11 @@ __global__ __launch_bounds__(kBlock) void k_b_walk(BArgs a) {
         const BDesc& D = a.desc[s];
         const uint32_t L = S.width;
         const uint64_t rows = L == 1 ? D.start_n : S.rows;
-        const uint64_t ntile_s = (S.cand_end - S.cand_base + a.tile - 1) / a.tile;
+        const uint64_t tl = D.tile ? D.tile : a.tile;
+        const uint64_t ntile_s = (S.cand_end - S.cand_base + tl - 1) / tl;
         const uint64_t k = T - (tile_end[s] - ntile_s);
-        const uint64_t t0 = S.cand_base + k * a.tile;
-        const uint64_t t1 = t0 + a.tile < S.cand_end ? t0 + a.tile : S.cand_end;
+        const uint64_t t0 = S.cand_base + k * tl;
+        const uint64_t t1 = t0 + tl < S.cand_end ? t0 + tl : S.cand_end;
         TileIn in;
         in.rows = slot_rows(a, s, S, D);
         in.anc = a.g.nbr;
@@ -794,6 +799,12 @@ int gv_batch_admit(gv_batch_t b, const gv_query_t* q, uint64_t tag, uint32_t* sl
 
 int gv_batch_admit_leaves(gv_batch_t b, const gv_query_t* q, const gv_leaves_t* leaves, uint64_t tag,
                           uint32_t* slot) {
+    return gv_batch_admit_tuned(b, q, leaves, 0, tag, slot);
+}
+
+int gv_batch_admit_tuned(gv_batch_t b, const gv_query_t* q, const gv_leaves_t* leaves, uint32_t ipt,
+                         uint64_t tag, uint32_t* slot) {
+    if (b && ipt && (ipt > b->tune.ipt || (ipt & (ipt - 1)))) return GV_ERR_ARG;
     if (!b || !q || !slot || q->k < 2 || q->k > GV_MAXK || q->label0 >= b->g->nlabels)
         return GV_ERR_ARG;
     if (leaves && (!valid_leaves(leaves, b->g) || leaves->k != q->k)) return GV_ERR_ARG;
@@ -819,6 +830,7 @@ int gv_batch_admit_leaves(gv_batch_t b, const gv_query_t* q, const gv_leaves_t* 
     BDesc& d = b->h_desc[s];
     memset(&d, 0, sizeof(d));
     d.q = *q;
+    d.tile = 64u * ipt;
     d.start_off = (uint32_t)b->g->lvoff[q->label0];
     d.start_n = (uint32_t)(b->g->lvoff[q->label0 + 1] - b->g->lvoff[q->label0]);
     if (leaves) {

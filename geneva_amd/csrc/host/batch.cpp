@@ -70,6 +70,21 @@ int BatchMiner::setTune(const gv_tune_t& t) {
     return rc;
 }
 
+std::string BatchMiner::classKey(const CompiledQuery& cq) {
+    uint32_t cyc = 0;
+    for (uint32_t t = 1; t < cq.q.k; ++t) cyc += cq.q.step[t].nback > 1;
+    return "k" + std::to_string(cq.q.k) + "c" + std::to_string(cyc) + (cq.hasLeaves ? "l" : "");
+}
+
+int BatchMiner::admit(const CompiledQuery& cq, uint64_t tag, uint32_t* slot) {
+    uint32_t ipt = 0;
+    if (!classIpt_.empty()) {
+        auto it = classIpt_.find(classKey(cq));
+        if (it != classIpt_.end() && it->second <= opt_.tune.ipt) ipt = it->second;
+    }
+    return gv_batch_admit_tuned(b_, &cq.q, cq.hasLeaves ? &cq.leaves : nullptr, ipt, tag, slot);
+}
+
 int BatchMiner::run(const std::vector<CompiledQuery>& qs, std::vector<MineResult>& out,
                     BatchStats* st) {
     int rc = init();
@@ -96,8 +111,7 @@ int BatchMiner::run(const std::vector<CompiledQuery>& qs, std::vector<MineResult
         while (!pending.empty()) {
             uint32_t slot = 0;
             const CompiledQuery& cq = qs[pending.front()];
-            rc = gv_batch_admit_leaves(b_, &cq.q, cq.hasLeaves ? &cq.leaves : nullptr, pending.front(),
-                                       &slot);
+            rc = admit(cq, pending.front(), &slot);
             if (rc == GV_ERR_CAPACITY) break;
             if (rc != GV_OK) return rc;
             slotQuery[slot] = pending.front();
@@ -166,8 +180,12 @@ std::mutex& tuneMutex() {
     static std::mutex m;
     return m;
 }
-std::map<std::string, gv_tune_t>& tuneCache() {
-    static std::map<std::string, gv_tune_t> m;
+struct CachedTune {
+    gv_tune_t tune;
+    std::map<std::string, uint32_t> classes;
+};
+std::map<std::string, CachedTune>& tuneCache() {
+    static std::map<std::string, CachedTune> m;
     return m;
 }
 std::string graphKey(gv_graph_t g, const BatchOptions& o) {
@@ -201,10 +219,13 @@ int autotune(BatchMiner& bm, const std::vector<CompiledQuery>& sample, TuneResul
         std::lock_guard<std::mutex> l(tuneMutex());
         auto it = tuneCache().find(key);
         if (it != tuneCache().end()) {
-            out.tune = it->second;
+            out.tune = it->second.tune;
+            bm.classTune() = it->second.classes;
+            for (auto& kv : it->second.classes) out.classes[kv.first] = {kv.second, 0.0};
             return bm.setTune(out.tune);
         }
     }
+    bm.classTune().clear();
     const uint32_t ipts[] = {1, 2, 4, 8};
     const uint32_t bpcs[] = {4, 8, 16, 32};
     double best = 1e300;
@@ -228,11 +249,36 @@ int autotune(BatchMiner& bm, const std::vector<CompiledQuery>& sample, TuneResul
         }
     }
     out.bestMs = best;
+    int rc = bm.setTune(out.tune);
+    if (rc != GV_OK) return rc;
+    // per class: candidates per lane up to the chosen kernel's
+    std::map<std::string, std::vector<CompiledQuery>> byClass;
+    for (auto& q : sample)
+        if (!q.empty) byClass[BatchMiner::classKey(q)].push_back(q);
+    for (auto& kv : byClass) {
+        if (kv.second.size() < 4) continue;
+        double cbest = 1e300;
+        uint32_t cipt = out.tune.ipt;
+        for (uint32_t ipt : ipts) {
+            if (ipt > out.tune.ipt) break;
+            bm.classTune()[kv.first] = ipt;
+            BatchStats st;
+            rc = bm.run(kv.second, res, nullptr);
+            if (rc == GV_OK) rc = bm.run(kv.second, res, &st);
+            if (rc != GV_OK) return rc;
+            if (st.deviceMs < cbest) {
+                cbest = st.deviceMs;
+                cipt = ipt;
+            }
+        }
+        bm.classTune()[kv.first] = cipt;
+        out.classes[kv.first] = {cipt, cbest};
+    }
     {
         std::lock_guard<std::mutex> l(tuneMutex());
-        tuneCache()[key] = out.tune;
+        tuneCache()[key] = CachedTune{out.tune, bm.classTune()};
     }
-    return bm.setTune(out.tune);
+    return GV_OK;
 }
 
 }  // namespace geneva

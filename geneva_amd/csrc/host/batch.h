@@ -61,6 +61,17 @@ public:
     // get status GV_ERR_ARG; overflowing ones are re-run on the plain path.
     int run(const std::vector<CompiledQuery>& qs, std::vector<MineResult>& out, BatchStats* st);
 
+    // Place a query in a free slot with its class's candidates per lane
+    // (classTune); GV_ERR_CAPACITY when no slot is free.
+    int admit(const CompiledQuery& cq, uint64_t tag, uint32_t* slot);
+
+    // Query class of the per-class tuning: vertices, steps with more than
+    // one back edge, trailing leaf groups.
+    static std::string classKey(const CompiledQuery& cq);
+    // candidates per lane per walk tile of each class (0 or absent: the
+    // batch's own)
+    std::map<std::string, uint32_t>& classTune() { return classIpt_; }
+
 private:
     gv_ctx_t ctx_;
     gv_graph_t g_;
@@ -68,15 +79,22 @@ private:
     BatchOptions opt_;
     gv_batch_t b_ = nullptr;
     uint32_t nwords_ = 0;
+    std::map<std::string, uint32_t> classIpt_;
 };
 
 // Autotuner: times the batch engine under each candidate tuning on a sample of
-// queries and keeps the fastest, remembered per shape key (query sizes of the
-// sample).  Returns the chosen tuning.
+// queries and keeps the fastest (candidates per lane of the walk kernel x its
+// blocks per CU), then, for each query class of the sample with at least four
+// queries, times that class alone under each candidates-per-lane value up to
+// the chosen one and keeps the fastest for the class (slots of the class run
+// their walk tiles at that size).  Remembered per graph, engine options and
+// sample shape.
 struct TuneResult {
     gv_tune_t tune{4, 16};
     double bestMs = 0;
     std::vector<std::pair<gv_tune_t, double>> tried;
+    // per query class: the candidates per lane chosen and its device ms
+    std::map<std::string, std::pair<uint32_t, double>> classes;
 };
 int autotune(BatchMiner& bm, const std::vector<CompiledQuery>& sample, TuneResult& out);
 

@@ -40,6 +40,8 @@ struct gh_engine {
     // progress of the last plain-path query (time limit)
     bool lastFinished = true;
     double lastFraction = 1.0;
+    // per-class tuning of the last autotune: "class:ipt:ms;" entries
+    std::string tuneTable;
 };
 
 namespace {
@@ -505,10 +507,25 @@ int gh_engine_autotune(gh_engine_t e, const char* const* queries, uint32_t n, ui
     rc = geneva::autotune(*e->batch, qs, tr);
     if (rc != GV_OK) return rc;
     e->batchOpt.tune = tr.tune;
+    e->tuneTable.clear();
+    for (auto& kv : tr.classes)
+        e->tuneTable += kv.first + ":" + std::to_string(kv.second.first) + ":" +
+                        std::to_string(kv.second.second) + ";";
     if (ipt) *ipt = tr.tune.ipt;
     if (bpc) *bpc = tr.tune.walk_blocks_per_cu;
     if (best_ms) *best_ms = tr.bestMs;
     return GV_OK;
+}
+
+uint64_t gh_engine_autotune_classes(gh_engine_t e, char* out, uint64_t cap) {
+    if (!e) return 0;
+    const std::string& t = e->tuneTable;
+    if (out && cap) {
+        const uint64_t n = std::min<uint64_t>(t.size(), cap - 1);
+        memcpy(out, t.data(), n);
+        out[n] = 0;
+    }
+    return t.size();
 }
 
 int gh_engine_serve(gh_engine_t e, const char* const* queries, const double* arrival_s, uint32_t n,

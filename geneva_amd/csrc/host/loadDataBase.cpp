@@ -182,7 +182,7 @@ uint* buildPartitionBlock(const DataGraph& g, uint edgeLabel, std::string* err) 
         if (err) *err = "partition too large for 32-bit offsets";
         return nullptr;
     }
-    uint* blk = (uint*)malloc(sizeof(uint) * (size_t)tot);
+    uint* blk = new uint[(size_t)tot];   // freed by freeDataBase, as loadDataBase's blocks
     blk[0] = (uint)tot;
     blk[1] = S;
     blk[2] = n;

@@ -66,7 +66,8 @@ std::map<uint, uint> partitionRuns(const DataGraph& g);
 // tools/genDatabase.cpp:415-436,741-743).  Sources are the vertices with at
 // least one edge of this label; their ids must form at most 256 runs of
 // consecutive ids (the reference drops runs past 256; this writer refuses).
-// Returns a malloc'd array (element 0 = total uints).
+// Returns a new[]-allocated array (element 0 = total uints), released by
+// freeDataBase like loadDataBase's blocks.
 uint* buildPartitionBlock(const DataGraph& g, uint edgeLabel, std::string* err);
 
 // Same result as loadDataBase for an in-memory graph: one block per non-empty

@@ -198,8 +198,7 @@ int serveBatched(BatchMiner& bm, const std::vector<CompiledQuery>& reqs,
             }
             if (running > 0 && inUse + est[next] > memBudget) break;
             uint32_t slot = 0;
-            rc = gv_batch_admit_leaves(b, &reqs[next].q, reqs[next].hasLeaves ? &reqs[next].leaves : nullptr,
-                                       next, &slot);
+            rc = bm.admit(reqs[next], next, &slot);
             if (rc == GV_ERR_CAPACITY) break;
             if (rc != GV_OK) return rc;
             slotReq[slot] = next;

@@ -287,6 +287,11 @@ int gv_batch_admit(gv_batch_t b, const gv_query_t* q, uint64_t tag, uint32_t* sl
  * its leaf groups per core row when its table reaches leaves->width. */
 int gv_batch_admit_leaves(gv_batch_t b, const gv_query_t* q, const gv_leaves_t* leaves,
                           uint64_t tag, uint32_t* slot);
+/* As gv_batch_admit_leaves, with this slot's candidates per lane per walk
+ * tile (a power of two up to the batch's gv_tune_t.ipt; 0 = the batch's):
+ * the per-query-class tuning of the wavefront split. */
+int gv_batch_admit_tuned(gv_batch_t b, const gv_query_t* q, const gv_leaves_t* leaves, uint32_t ipt,
+                         uint64_t tag, uint32_t* slot);
 /* Launch `iterations` iterations (asynchronous). */
 int gv_batch_run(gv_batch_t b, uint32_t iterations);
 /* Copy every slot's state to info[0..slots) (synchronises). */

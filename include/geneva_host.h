@@ -129,6 +129,10 @@ int gh_engine_batch_mine(gh_engine_t e, const char* const* queries, uint32_t n, 
  * lane and blocks per CU and its device ms. */
 int gh_engine_autotune(gh_engine_t e, const char* const* queries, uint32_t n, uint32_t* ipt,
                        uint32_t* blocks_per_cu, double* best_ms);
+/* The per-query-class choices of the last autotune as text, one
+ * "class:candidates per lane:device ms;" entry per class (class = k<vertices>
+ * c<steps with several back edges>[l if leaf groups]).  Returns its length. */
+uint64_t gh_engine_autotune_classes(gh_engine_t e, char* out, uint64_t cap);
 /* Serve n requests arriving at arrival_s[i] seconds (non-decreasing).
  * mode 0: continuous batching with admission under mem_budget bytes of
  * estimated peak tables (requests that leave the batch engine are mined on

@@ -93,10 +93,14 @@ def test_tunings_agree(w3):
             eng.batch_config(slots=64, slot_bytes=8 << 20, row_cap=1 << 22, ipt=ipt, blocks_per_cu=bpc)
             res, _ = eng.mine_batch(qs)
             assert [r.count for r in res] == want, (ipt, bpc)
-    tuned = eng.autotune(qs)
+    tuned = eng.autotune([c["query"] for c in cases])
     print("autotune", tuned, flush=True)
     res, _ = eng.mine_batch(qs)
     assert [r.count for r in res] == want
+    # per-class candidates per lane (the tuning of each query class) change
+    # nothing but speed
+    assert tuned["classes"], tuned
+    assert all(c["ipt"] <= tuned["ipt"] for c in tuned["classes"].values())
     # first steps from the start tables instead of the label-pair edge tables
     eng.batch_pair_start(False)
     res2, _ = eng.mine_batch(qs)
