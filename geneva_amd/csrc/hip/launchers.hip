@@ -378,6 +378,17 @@ __global__ void k_ref_overflow(uint32_t* counter, uint32_t max_rows) {
 // ---------------------------------------------------------------------------
 // host side
 
+// Resume point of the last launch that stopped before its last candidate.
+struct RefResume {
+    bool active = false;
+    int kind = -1;
+    const void* in = nullptr;
+    const void* out = nullptr;
+    uint64_t nrows = 0;
+    uint64_t total = 0;
+    uint64_t next = 0;
+};
+
 struct RefState {
     uint64_t cap = 0;         // rows the workspace holds
     uint32_t* gb = nullptr;
@@ -389,14 +400,7 @@ struct RefState {
     uint64_t scratch_cap = 0; // words
     uint64_t* hval = nullptr; // pinned: total candidates
     uint32_t* hcnt = nullptr; // pinned: totWriteRowNum
-    // resume point of the last launch that stopped early
-    bool active = false;
-    int kind = -1;
-    const void* in = nullptr;
-    const void* out = nullptr;
-    uint64_t nrows = 0;
-    uint64_t total = 0;
-    uint64_t next = 0;
+    RefResume res;
 };
 
 void ref_state_free(gv_ctx* c) {
@@ -447,13 +451,12 @@ int ref_reserve_rows(gv_ctx* c, RefState* r, uint64_t n) {
     return GV_OK;
 }
 
-int ref_scan(gv_ctx* c, RefState* r, const uint64_t* in, uint64_t n) {
+int ref_scan(gv_ctx* c, uint64_t* bsum, uint64_t* csum, const uint64_t* in, uint64_t n) {
     if (n == 0) return GV_OK;
     const uint64_t nb = (n + kRScanTile - 1) / kRScanTile;
-    hipLaunchKernelGGL(k_rscan_reduce, dim3((unsigned)nb), dim3(kBlock), 0, c->stream, in, n, r->bsum);
-    hipLaunchKernelGGL(k_rscan_tiles, dim3(1), dim3(kBlock), 0, c->stream, r->bsum, nb);
-    hipLaunchKernelGGL(k_rscan_final, dim3((unsigned)nb), dim3(kBlock), 0, c->stream, in, n, r->bsum,
-                       r->csum);
+    hipLaunchKernelGGL(k_rscan_reduce, dim3((unsigned)nb), dim3(kBlock), 0, c->stream, in, n, bsum);
+    hipLaunchKernelGGL(k_rscan_tiles, dim3(1), dim3(kBlock), 0, c->stream, bsum, nb);
+    hipLaunchKernelGGL(k_rscan_final, dim3((unsigned)nb), dim3(kBlock), 0, c->stream, in, n, bsum, csum);
     HIPCHK(hipGetLastError());
     return GV_OK;
 }
@@ -505,37 +508,44 @@ int aux_colls(const uint32_t* aux, uint32_t in_w, bool same, RefNew& n1, RefNew*
     return GV_OK;
 }
 
+// Working arrays of one launch and the host words its sizes are read into.
+struct RefWork {
+    uint32_t* gb;
+    uint32_t* gl;
+    uint64_t* cnt;
+    uint64_t* csum;
+    uint64_t* bsum;
+    uint64_t* hval;   // total candidates
+    uint32_t* hcnt;   // totWriteRowNum[0]
+    RefResume* res;
+};
+
 // One launch of an extension or init launcher: size and scan on a fresh
-// call, then emit the candidates that fit below maxRowNum.
-int ref_extend(gv_ctx* c, RefArgs& a, int kind, uint32_t* counter, uint32_t maxRowNum, bool isContinue) {
-    if (!c || !counter || !a.out || (a.in_w + a.nnew) > GV_MAXK + 1) return GV_ERR_ARG;
-    HIPCHK(hipSetDevice(c->device));
-    RefState* r = nullptr;
-    int rc = ref_state(c, r);
-    if (rc != GV_OK) return rc;
-    ProfScope ps(c, kKindRef);
+// call, then emit the candidates that fit below maxRowNum.  The workspace
+// must hold a.nrows rows.
+int ref_extend_on(gv_ctx* c, RefArgs& a, int kind, uint32_t* counter, uint32_t maxRowNum, bool isContinue,
+                  const RefWork& w) {
+    RefResume* r = w.res;
+    a.gb = w.gb;
+    a.gl = w.gl;
+    a.cnt = w.cnt;
+    a.csum = w.csum;
     if (!isContinue) {
         r->active = false;
-        rc = ref_reserve_rows(c, r, a.nrows);
-        if (rc != GV_OK) return rc;
-        a.gb = r->gb;
-        a.gl = r->gl;
-        a.cnt = r->cnt;
-        a.csum = r->csum;
         r->total = 0;
         if (a.nrows) {
             hipLaunchKernelGGL(k_ref_size, dim3(grid_for(a.nrows, kBlock, c->num_cu * 16)), dim3(kBlock), 0,
                                c->stream, a);
             HIPCHK(hipGetLastError());
-            rc = ref_scan(c, r, r->cnt, a.nrows);
+            int rc = ref_scan(c, w.bsum, w.csum, w.cnt, a.nrows);
             if (rc != GV_OK) return rc;
-            HIPCHK(hipMemcpyAsync(r->hval, r->csum + a.nrows - 1, 8, hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(hipMemcpyAsync(w.hval, w.csum + a.nrows - 1, 8, hipMemcpyDeviceToHost, c->stream));
         } else {
-            r->hval[0] = 0;
+            *w.hval = 0;
         }
-        HIPCHK(hipMemcpyAsync(r->hcnt, counter, 4, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipMemcpyAsync(w.hcnt, counter, 4, hipMemcpyDeviceToHost, c->stream));
         HIPCHK(hipStreamSynchronize(c->stream));
-        r->total = r->hval[0];
+        r->total = *w.hval;
         r->next = 0;
         r->kind = kind;
         r->in = a.in;
@@ -544,14 +554,10 @@ int ref_extend(gv_ctx* c, RefArgs& a, int kind, uint32_t* counter, uint32_t maxR
     } else {
         if (!r->active || r->kind != kind || r->in != a.in || r->out != a.out || r->nrows != a.nrows)
             return GV_ERR_ARG;
-        a.gb = r->gb;
-        a.gl = r->gl;
-        a.cnt = r->cnt;
-        a.csum = r->csum;
-        HIPCHK(hipMemcpyAsync(r->hcnt, counter, 4, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipMemcpyAsync(w.hcnt, counter, 4, hipMemcpyDeviceToHost, c->stream));
         HIPCHK(hipStreamSynchronize(c->stream));
     }
-    const uint64_t written = r->hcnt[0];
+    const uint64_t written = *w.hcnt;
     const uint64_t room = (uint64_t)maxRowNum > written + 1 ? maxRowNum - 1 - written : 0;
     const uint64_t end = std::min<uint64_t>(r->total, r->next + room);
     if (end > r->next) {
@@ -569,6 +575,59 @@ int ref_extend(gv_ctx* c, RefArgs& a, int kind, uint32_t* counter, uint32_t maxR
         HIPCHK(hipGetLastError());
         if (room == 0) return GV_ERR_CAPACITY;   // no row fits: the caller would loop forever
     }
+    return GV_OK;
+}
+
+// The launcher entries: the context's workspace, grown to the input.
+int ref_extend(gv_ctx* c, RefArgs& a, int kind, uint32_t* counter, uint32_t maxRowNum, bool isContinue) {
+    if (!c || !counter || !a.out || (a.in_w + a.nnew) > GV_MAXK + 1) return GV_ERR_ARG;
+    HIPCHK(hipSetDevice(c->device));
+    RefState* r = nullptr;
+    int rc = ref_state(c, r);
+    if (rc != GV_OK) return rc;
+    if (!isContinue) {
+        r->res.active = false;
+        rc = ref_reserve_rows(c, r, a.nrows);
+        if (rc != GV_OK) return rc;
+    }
+    ProfScope ps(c, kKindRef);
+    const RefWork w{r->gb, r->gl, r->cnt, r->csum, r->bsum, r->hval, r->hcnt, &r->res};
+    return ref_extend_on(c, a, kind, counter, maxRowNum, isContinue, w);
+}
+
+// ---- the extension launcher without allocations --------------------------
+// The working arrays of gv_extEmb_1V_NoHash_noalloc live in one file-scope
+// device array per GPU, `scratch`, of kNoAllocBytes: 32 bytes per input row
+// (two group starts, two lengths, a count and its scan) and the scan's tile
+// sums, so at most kNoAllocRows input rows per call.  Its address on a
+// device comes from hipGetSymbolAddress at the first call on that device.
+constexpr uint64_t kNoAllocRows = 2000000;
+constexpr uint64_t kNoAllocTiles = kNoAllocRows / kRScanTile + 2;
+constexpr uint64_t kNoAllocBytes = 32 * kNoAllocRows + 8 * kNoAllocTiles;
+__device__ __align__(256) uint8_t scratch[kNoAllocBytes];
+
+constexpr int kNoAllocDevices = 64;
+struct NoAllocDevice {
+    uint8_t* ptr = nullptr;     // this device's copy of scratch
+    RefResume res;
+};
+NoAllocDevice g_noalloc[kNoAllocDevices];
+std::mutex g_noalloc_mu;
+
+int noalloc_device(int device, NoAllocDevice*& out) {
+    if (device < 0 || device >= kNoAllocDevices) return GV_ERR_ARG;
+    std::lock_guard<std::mutex> lock(g_noalloc_mu);
+    NoAllocDevice& d = g_noalloc[device];
+    if (!d.ptr) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            g_last_error = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+            return GV_ERR_HIP;
+        }
+        d.ptr = static_cast<uint8_t*>(p);
+    }
+    out = &d;
     return GV_OK;
 }
 
@@ -782,7 +841,7 @@ int gv_reductionPhase(gv_ctx_t c, uint32_t* vLabel, uint32_t* totWriteRowNum, ui
     RefState* r = nullptr;
     int rc = ref_state(c, r);
     if (rc != GV_OK) return rc;
-    r->active = false;
+    r->res.active = false;
     rc = ref_reserve_rows(c, r, partialRowNum);
     if (rc != GV_OK) return rc;
     const uint64_t words = (uint64_t)partialRowNum * embLen;
@@ -807,7 +866,7 @@ int gv_reductionPhase(gv_ctx_t c, uint32_t* vLabel, uint32_t* totWriteRowNum, ui
         hipLaunchKernelGGL(k_ref_red_flag, dim3(grid_for(partialRowNum, kBlock, c->num_cu * 16)), dim3(kBlock),
                            0, c->stream, a);
         HIPCHK(hipGetLastError());
-        rc = ref_scan(c, r, r->cnt, partialRowNum);
+        rc = ref_scan(c, r->bsum, r->csum, r->cnt, partialRowNum);
         if (rc != GV_OK) return rc;
         hipLaunchKernelGGL(k_ref_red_scatter, dim3(grid_for(partialRowNum, kBlock, c->num_cu * 16)),
                            dim3(kBlock), 0, c->stream, a);
@@ -820,6 +879,53 @@ int gv_reductionPhase(gv_ctx_t c, uint32_t* vLabel, uint32_t* totWriteRowNum, ui
 
 int gv_copyPartialEmb(gv_ctx_t c, uint32_t* src_dev, uint32_t* dst_dev, uint32_t numUints) {
     return gv_copy_partial_emb(c, src_dev, dst_dev, numUints);
+}
+
+int gv_extEmb_1V_NoHash_noalloc(gv_ctx_t c, uint32_t* vLabel_dev, uint32_t* totWriteRowNum_dev,
+                                uint32_t* edgeLabelPartition_dev, uint32_t* neighborsData_dev,
+                                uint32_t* recordPos_dev, uint32_t* auxArray_dev, uint32_t* newEmb_dev,
+                                uint32_t* partialEmb_dev, uint32_t intervalNum, uint32_t partialRowNum,
+                                uint32_t embLen, bool isExtRestrict, bool isRecord, bool useRecord,
+                                bool isLastPhase, bool isContinue, uint32_t maxRowNum) {
+    (void)useRecord;
+    (void)isLastPhase;
+    RefArgs a;
+    memset(&a, 0, sizeof(a));
+    if (!c || !totWriteRowNum_dev || !newEmb_dev ||
+        !ref_graph(a.g, vLabel_dev, edgeLabelPartition_dev, neighborsData_dev, intervalNum) || !auxArray_dev ||
+        embLen < 2 || embLen > GV_MAXK + 1 || (partialRowNum && !partialEmb_dev))
+        return GV_ERR_ARG;
+    if (partialRowNum > kNoAllocRows) {
+        g_last_error = "gv_extEmb_1V_NoHash_noalloc: " + std::to_string(partialRowNum) +
+                       " input rows, at most " + std::to_string(kNoAllocRows);
+        return GV_ERR_CAPACITY;
+    }
+    HIPCHK(hipSetDevice(c->device));
+    NoAllocDevice* d = nullptr;
+    int rc = noalloc_device(c->device, d);
+    if (rc != GV_OK) return rc;
+    uint32_t aux[64];
+    HIPCHK(hipMemcpyAsync(aux, auxArray_dev, sizeof(aux), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    const uint32_t in_w = embLen - 1;
+    if (aux[4] >= in_w || (isExtRestrict && aux[2] >= in_w)) return GV_ERR_ARG;
+    a.in = partialEmb_dev;
+    a.nrows = partialRowNum;
+    a.in_w = in_w;
+    a.nnew = 1;
+    new_col(a.nv[0], aux[0], aux[4], isExtRestrict ? aux[2] : kNoCol, isRecord ? recordPos_dev : nullptr);
+    rc = aux_colls(aux, in_w, false, a.nv[0], nullptr);
+    if (rc != GV_OK) return rc;
+    a.out = newEmb_dev;
+    // the working arrays in this device's scratch (fixed layout)
+    uint8_t* p = d->ptr;
+    uint64_t hval = 0;
+    uint32_t hcnt = 0;
+    const RefWork w{reinterpret_cast<uint32_t*>(p), reinterpret_cast<uint32_t*>(p + 8 * kNoAllocRows),
+                    reinterpret_cast<uint64_t*>(p + 16 * kNoAllocRows),
+                    reinterpret_cast<uint64_t*>(p + 24 * kNoAllocRows),
+                    reinterpret_cast<uint64_t*>(p + 32 * kNoAllocRows), &hval, &hcnt, &d->res};
+    return ref_extend_on(c, a, kExt1V, totWriteRowNum_dev, maxRowNum, isContinue, w);
 }
 
 int gv_launchers_reserve(gv_ctx_t c, uint64_t rows, uint64_t reduction_words) {

@@ -137,6 +137,21 @@ int gv_reductionPhase(gv_ctx_t ctx, uint32_t* vLabel, uint32_t* totWriteRowNum, 
  * numUints uints, overlapping ranges allowed in either direction. */
 int gv_copyPartialEmb(gv_ctx_t ctx, uint32_t* src_dev, uint32_t* dst_dev, uint32_t numUints);
 
+/* gv_extEmb_1V_NoHash for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, hosts that forbid allocation): no
+ * counterpart in the reference.  Same arguments, results and overflow
+ * protocol; the working arrays live in a fixed device array per GPU (about
+ * 64 MB, taken once per process), so at most 2,000,000 input rows
+ * (partialRowNum; more gives GV_ERR_CAPACITY), and the call allocates no
+ * device, pinned or host-heap memory.  All calls on one GPU share that array
+ * and one resume point: calls that overlap on one GPU race. */
+int gv_extEmb_1V_NoHash_noalloc(gv_ctx_t ctx, uint32_t* vLabel_dev, uint32_t* totWriteRowNum_dev,
+                                uint32_t* edgeLabelPartition_dev, uint32_t* neighborsData_dev,
+                                uint32_t* recordPos_dev, uint32_t* auxArray_dev, uint32_t* newEmb_dev,
+                                uint32_t* partialEmb_dev, uint32_t intervalNum, uint32_t partialRowNum,
+                                uint32_t embLen, bool isExtRestrict, bool isRecord, bool useRecord,
+                                bool isLastPhase, bool isContinue, uint32_t maxRowNum);
+
 /* Size the context's launcher workspace for input tables of up to `rows`
  * rows and reductions of up to `reduction_words` uints, so later calls
  * allocate nothing.  No counterpart in the reference. */
