@@ -238,6 +238,7 @@ def section_w3(args):
                        "against": "oracle fixture tests/golden/workload3_full.json (counts, bytes, graph-id sets)"})
     # one accounting run: every kernel of the iteration launched on its own
     # under HIP events, and the bytes B charged to k_b_size / k_b_walk
+    shared = dict(zip(("shared", "restarted"), eng.batch_share_stats()))
     eng.batch_profile(True)
     eng.profile(True, count_bytes=False)
     acct_res, acct_st = eng.mine_batch(qs)
@@ -266,8 +267,13 @@ def section_w3(args):
                     kinds["k_b_size"][1] / max(kinds["k_b_size"][0], 1), 4),
                     "b_bytes_per_launch": b_size // max(kinds["k_b_size"][0], 1),
                     "frac": round(b_size / (max(kinds["k_b_size"][1], 1e-9) / 1e3) / 1e9 / HBM_PEAK_GBS, 4)},
-                "iteration": {"b_bytes": b_total, "device_ms": round(dev_ms, 3),
-                              "frac": round(b_total / (max(dev_ms, 1e-9) / 1e3) / 1e9 / HBM_PEAK_GBS, 4)}}
+                # B the kernels executed (k_b_size + k_b_walk) over the whole
+                # iteration's device time; the queries' own B also counts the
+                # steps shared between queries once per query and the
+                # closed-form first steps of pair starts
+                "iteration": {"b_bytes": b_size + b_walk, "device_ms": round(dev_ms, 3),
+                              "frac": round((b_size + b_walk) / (max(dev_ms, 1e-9) / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
+                              "b_bytes_of_queries": b_total}}
     kern = {k: {"launches": v[0], "ms": round(v[1], 3)} for k, v in kinds.items()}
     out = {"embeddings_per_step": emb, "queries": len(qs), "ms_per_step": ms_step,
            "batched_queries_per_s": round(len(qs) / (ms_step / 1e3), 1),
@@ -276,6 +282,7 @@ def section_w3(args):
            "speedup_vs_plain": round(tp / ms_step, 3),
            "overflowed_to_plain": int(st["overflowed"]), "graph_launches": int(st["graph_launches"]),
            "batch_device_ms": round(st["device_ms"], 3), "tune": tune, "parity": parity,
+           "shared_tables": shared,
            "roofline": roofline, "kernels": kern,
            "plain_device_ms_sum": round(sum(r.ms for r in plain), 3)}
     return ds, eng, out
@@ -516,6 +523,8 @@ def main():
     ap.add_argument("--workload", type=int, default=3, help="workload of --pmc-summary")
     ap.add_argument("--cal-mib", type=int, default=512, help="MIB of the calibration run (--pmc-summary)")
     ap.add_argument("--pmc-table", metavar="CSV", help="only print per-kernel means of a rocprofv3 --pmc CSV")
+    ap.add_argument("--only", choices=("w2", "w4"),
+                    help="only run the workload-2 or workload-4 section and print its JSON (profiling runs)")
     args = ap.parse_args()
     if args.copy_calibration:
         return copy_calibration(args.copy_calibration)
@@ -528,6 +537,10 @@ def main():
             "single MI355X; sharding over several GPUs is a design note, not part of the build.")
         sys.exit(2)
 
+    if args.only:
+        _, sec = (section_w2 if args.only == "w2" else section_w4)(args)
+        print(json.dumps({args.only: sec}), flush=True)
+        return
     w3ds, eng, w3 = section_w3(args)
     log(f"workload 3: {w3['batched_queries_per_s']} q/s batched vs {w3['plain_queries_per_s']} plain")
     served, streams = (None, {}) if args.no_served else section_served(args, w3ds, eng)

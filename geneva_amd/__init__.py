@@ -94,6 +94,8 @@ def lib():
         "gh_engine_batch_profile": (i32, [vp, i32]),
         "gh_engine_batch_profile_bytes": (i32, [vp, C.POINTER(u64), C.POINTER(u64)]),
         "gh_engine_batch_pair_start": (i32, [vp, i32]),
+        "gh_engine_batch_share": (i32, [vp, i32]),
+        "gh_engine_batch_share_stats": (i32, [vp, C.POINTER(u64), C.POINTER(u64)]),
         "gh_engine_batch_history": (i32, [vp, C.POINTER(u64), C.POINTER(u64), u32, C.POINTER(u32)]),
         "gh_engine_profile_records": (i32, [vp, C.POINTER(i32), C.POINTER(C.c_float), u64, C.POINTER(u64)]),
         "gh_engine_profile": (i32, [vp, i32]),
@@ -454,6 +456,22 @@ class Engine:
         rc = lib().gh_engine_batch_pair_start(self._h, int(on))
         if rc:
             raise _err("batch_pair_start", rc)
+
+    def batch_share(self, on=True):
+        """Let batch queries with the same first steps share one table
+        (applies to queries admitted from now on)."""
+        rc = lib().gh_engine_batch_share(self._h, int(on))
+        if rc:
+            raise _err("batch_share", rc)
+
+    def batch_share_stats(self):
+        """(tables shared, followers restarted on their own) since the batch
+        engine was made."""
+        a, b = C.c_uint64(0), C.c_uint64(0)
+        rc = lib().gh_engine_batch_share_stats(self._h, C.byref(a), C.byref(b))
+        if rc:
+            raise _err("batch_share_stats", rc)
+        return a.value, b.value
 
     def batch_history(self):
         """[(rows sized, candidates walked)] of each batch iteration run while

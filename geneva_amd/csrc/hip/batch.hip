@@ -41,7 +41,23 @@ struct BDesc {               // written by the host at admission
     // candidates per walk tile of this slot (64 x its candidates per lane, at
     // most the kernel's); 0 = the batch's
     uint32_t tile;
+    // prefix sharing: a follower's first steps equal those of the slot
+    // `leader` (admitted with serial leader_serial) up to the table of width
+    // fork_width.  It waits until the leader's input table has that width and
+    // then extends the leader's table in the same iteration as the leader,
+    // taking over the leader's counters; kNoSlot = none
+    uint32_t leader;
+    uint32_t fork_width;
+    uint64_t leader_serial;
+    // where the slot starts (width, rows, bytes of a pair start): a follower
+    // that cannot share restarts from here
+    uint32_t init_width;
+    uint32_t pad_;
+    uint64_t init_n;
+    uint64_t init_bytes;
 };
+
+constexpr uint32_t kNoSlot = 0xffffffffu;
 
 struct BState {              // dynamic, polled by the host (mirrors gv_slot_info_t)
     uint64_t tag;
@@ -63,7 +79,10 @@ struct BState {              // dynamic, polled by the host (mirrors gv_slot_inf
     uint64_t tile_end;       // end of this slot's tiles (tiles never straddle slots)
     uint64_t rtile_end;      // end of this slot's sizing tiles (kBSizeTile rows each)
     uint32_t defer;          // iterations in a row this slot waited for row space
-    uint32_t pad_;
+    uint32_t wait;           // follower still waiting for its leader's table
+    uint64_t serial;         // admission number (never 0 for an admitted slot)
+    uint32_t src;            // slot and half holding this iteration's input table
+    uint32_t src_sel;
 };
 
 // A slot that waited this many iterations in a row for row space is placed
@@ -86,7 +105,8 @@ struct BArgs {
     uint64_t* csum;
     uint64_t row_cap;
     uint64_t* tiles;
-    unsigned long long* glob;   // [0] rows, [1] candidates, [2] tiles, [3] sizing tiles
+    unsigned long long* glob;   // [0] rows, [1] candidates, [2] tiles, [3] sizing tiles,
+                                // [4] shared tables (forks), [5] followers restarted
     uint64_t tile;              // candidates per walk tile (64 * IPT)
     uint32_t* gbits;            // nslots * gwords
     uint32_t gwords;
@@ -109,7 +129,7 @@ __device__ __forceinline__ const uint32_t* slot_rows(const BArgs& a, uint32_t s,
                                                      const BDesc& D) {
     if (S.width == 1) return a.lverts + D.start_off;
     if (S.width == 2 && D.init_rows) return D.init_rows;
-    return (S.sel ? a.half1 : a.half0) + (uint64_t)s * a.slot_uints;
+    return (S.src_sel ? a.half1 : a.half0) + (uint64_t)S.src * a.slot_uints;
 }
 
 __device__ __forceinline__ uint32_t* slot_out(const BArgs& a, uint32_t s, const BState& S) {
@@ -152,50 +172,155 @@ __device__ uint64_t block_scan_1024(uint64_t v, uint64_t* wsum, uint64_t& total)
     return off + x - v;
 }
 
+// Mode of a slot whose input table has width w: 1 extend, 2 count the last
+// step, 3 count the last two (single-edge tail), 4 leaf groups; rmode = the
+// per-row counting mode of a counted step, or -1 (walk).
+__device__ uint32_t slot_mode(const BDesc& D, uint32_t w, int32_t& rmode) {
+    const bool last = w + 1 == D.q.k;
+    const bool tail = !last && w + 2 == D.q.k && D.q.step[w + 1].nback == 1;
+    uint32_t mode = last ? 2 : (tail ? 3 : 1);
+    if (D.has_leaves && w == D.leaves.width) mode = 4;
+    rmode = mode == 1 || mode == 4
+                ? -1
+                : count_rows_mode(&D.q.step[w], mode == 3 ? &D.q.step[w + 1] : nullptr, w);
+    return mode;
+}
+
+// A running slot's mode and rows this iteration; finishes a slot whose table
+// is empty or complete and flags one whose table can never fit an iteration.
+__device__ uint32_t plan_running(const BArgs& a, BState& S, const BDesc& D, uint64_t& rows,
+                                 int32_t& rmode) {
+    rows = 0;
+    rmode = -1;
+    const uint64_t n = S.width == 1 ? D.start_n : S.rows;
+    if (n == 0 || S.width >= D.q.k) {
+        S.state = GV_SLOT_DONE;
+        return 0;
+    }
+    if (n > a.row_cap) {
+        S.state = GV_SLOT_OVERFLOW;
+        return 0;
+    }
+    rows = n;
+    return slot_mode(D, S.width, rmode);
+}
+
 __global__ __launch_bounds__(1024) void k_b_prep(BArgs a) {
     __shared__ uint64_t wsum[16];
+    __shared__ uint32_t nfollow[kMaxSlots];   // followers sharing each leader's table now
+    __shared__ uint8_t ran[kMaxSlots];
     const uint32_t s = threadIdx.x;
+    const bool in = s < a.nslots;
+    nfollow[s] = 0;
     uint64_t rows = 0;
     uint32_t mode = 0;
-    if (s < a.nslots) {
+    int32_t rmode = -1;
+    bool waiting = false;
+    if (in) {
         BState& S = a.st[s];
-        const BDesc& D = a.desc[s];
         S.out_rows = 0;
         if (S.state == GV_SLOT_RUNNING) {
-            const uint64_t n = S.width == 1 ? D.start_n : S.rows;
-            if (n == 0 || S.width >= D.q.k) {
-                S.state = GV_SLOT_DONE;
-            } else {
-                const bool last = S.width + 1 == D.q.k;
-                const bool tail = S.width + 2 == D.q.k && D.q.step[S.width + 1].nback == 1;
-                mode = last ? 2 : (tail ? 3 : 1);
-                if (D.has_leaves && S.width == D.leaves.width) mode = 4;
-                S.rmode = mode == 1 || mode == 4 ? -1
-                                    : count_rows_mode(&D.q.step[S.width],
-                                                      mode == 3 ? &D.q.step[S.width + 1] : nullptr,
-                                                      S.width);
-                rows = n;
-                if (n > a.row_cap) {  // can never fit one iteration
-                    S.state = GV_SLOT_OVERFLOW;
-                    mode = 0;
-                    rows = 0;
-                }
-            }
+            if (S.wait) waiting = true;
+            else mode = plan_running(a, S, a.desc[s], rows, rmode);
         }
     }
-    // which slots run: those that waited kAgeLimit iterations first, then the
+    __syncthreads();   // every leader's state of this iteration is settled
+    // a waiting follower: share its leader's table if the leader holds it now;
+    // finish as the leader did if its table ran empty or overflowed before
+    // that width (the follower's identical table would have too); restart on
+    // its own if the leader went past that width or its slot changed hands
+    uint32_t leader = kNoSlot;
+    if (waiting) {
+        BState& S = a.st[s];
+        const BDesc& D = a.desc[s];
+        const BState& P = a.st[D.leader];
+        const uint32_t d = D.fork_width;
+        int act = 2;   // 0 wait, 1 share, 2 restart, 3 empty, 4 overflow
+        if (P.serial == D.leader_serial) {
+            if (P.state == GV_SLOT_RUNNING) act = P.wait || P.width < d ? 0 : (P.width == d ? 1 : 2);
+            else if (P.state == GV_SLOT_DONE) act = P.width <= d ? 3 : 2;
+            else if (P.state == GV_SLOT_OVERFLOW) act = P.width <= d ? 4 : 2;
+        }
+        if (act == 1) {
+            leader = D.leader;
+            atomicAdd(&nfollow[leader], 1u);
+        } else if (act == 3) {
+            S.wait = 0;
+            S.bytes = P.bytes;
+            S.state = GV_SLOT_DONE;
+        } else if (act == 4) {
+            S.wait = 0;
+            S.state = GV_SLOT_OVERFLOW;
+        } else if (act == 2) {
+            S.wait = 0;
+            S.width = D.init_width;
+            S.rows = D.init_n;
+            S.bytes = D.init_bytes;
+            S.sel = 0;
+            mode = plan_running(a, S, D, rows, rmode);
+            atomicAdd(&a.glob[5], 1ull);
+        }
+    }
+    __syncthreads();
+    // a leader and the followers sharing its table run as one unit of
+    // (1 + followers) x its rows; a unit larger than row_cap never fits, so
+    // its followers restart on their own
+    bool share = false;
+    if (leader != kNoSlot) {
+        const uint64_t pn = a.st[leader].rows;
+        if (pn * (1 + nfollow[leader]) > a.row_cap) {
+            BState& S = a.st[s];
+            const BDesc& D = a.desc[s];
+            S.wait = 0;
+            S.width = D.init_width;
+            S.rows = D.init_n;
+            S.bytes = D.init_bytes;
+            S.sel = 0;
+            mode = plan_running(a, S, D, rows, rmode);
+            atomicAdd(&a.glob[5], 1ull);
+            leader = kNoSlot;
+        } else {
+            share = true;
+        }
+    }
+    uint64_t unit = share ? 0 : rows;
+    if (in && !share && nfollow[s] && rows * (1 + nfollow[s]) <= a.row_cap) unit = rows * (1 + nfollow[s]);
+    // which units run: those that waited kAgeLimit iterations first, then the
     // rest, each group in slot order while the rows fit row_cap; the others
     // wait (and age)
-    const bool aged = s < a.nslots && a.st[s].defer >= kAgeLimit;
+    const bool aged = in && !share && a.st[s].defer >= kAgeLimit;
     uint64_t tot, tot_a;
-    const uint64_t ra = aged ? rows : 0;
+    const uint64_t ra = aged ? unit : 0;
     const uint64_t ex_a = block_scan_1024(ra, wsum, tot);
     const bool run_a = aged && ex_a + ra <= a.row_cap;
-    block_scan_1024(run_a ? rows : 0, wsum, tot_a);
-    const uint64_t rb = aged ? 0 : rows;
+    block_scan_1024(run_a ? unit : 0, wsum, tot_a);
+    const uint64_t rb = aged ? 0 : unit;
     const uint64_t ex_b = block_scan_1024(rb, wsum, tot);
-    const bool run = run_a || (!aged && tot_a + ex_b + rb <= a.row_cap);
-    if (s < a.nslots && mode != 0) a.st[s].defer = run ? 0 : a.st[s].defer + 1;
+    bool run = run_a || (!aged && tot_a + ex_b + rb <= a.row_cap);
+    ran[s] = run;
+    __syncthreads();
+    uint32_t src = s, src_sel = in ? a.st[s].sel : 0;
+    if (share) {
+        // runs with its leader or keeps waiting
+        run = ran[leader];
+        if (run) {
+            BState& S = a.st[s];
+            const BState& P = a.st[leader];
+            const uint32_t d = a.desc[s].fork_width;
+            S.wait = 0;
+            S.width = d;
+            S.rows = P.rows;
+            S.bytes = P.bytes;
+            S.sel = 0;
+            rows = P.rows;
+            mode = slot_mode(a.desc[s], d, rmode);
+            src = leader;
+            src_sel = P.sel;
+            atomicAdd(&a.glob[4], 1ull);
+        }
+    } else if (in && mode != 0) {
+        a.st[s].defer = run ? 0 : a.st[s].defer + 1;
+    }
     if (!run) {
         rows = 0;
         mode = 0;
@@ -207,10 +332,14 @@ __global__ __launch_bounds__(1024) void k_b_prep(BArgs a) {
     const uint64_t nrt = (rows + kBSizeTile - 1) / kBSizeTile;
     uint64_t rtot;
     const uint64_t rex = block_scan_1024(nrt, wsum, rtot);
-    if (s < a.nslots) {
-        a.st[s].mode = mode;
-        a.st[s].row_end = ex + rows;
-        a.st[s].rtile_end = rex + nrt;
+    if (in) {
+        BState& S = a.st[s];
+        S.mode = mode;
+        S.rmode = rmode;
+        S.src = src;
+        S.src_sel = src_sel;
+        S.row_end = ex + rows;
+        S.rtile_end = rex + nrt;
     }
     if (s == 0) {
         a.glob[0] = total_rows;
@@ -578,6 +707,10 @@ struct gv_batch {
     unsigned long long* d_kb = nullptr;   // per-kernel B (profiling), 2 words
     bool kb_on = false;
     bool pair_start = true;               // first steps from the label-pair edge tables
+    bool share = true;                    // followers share a leader's table (prefix sharing)
+    uint64_t next_serial = 1;
+    std::vector<uint32_t> hwidth;         // host view: width at admission or last poll
+    std::vector<uint32_t> hstate;
     // rows sized and candidates walked by each profiled iteration
     std::vector<uint64_t> hist;
 };
@@ -704,6 +837,55 @@ int flush(gv_batch* b) {
     return GV_OK;
 }
 
+bool same_step(const gv_step_t& x, const gv_step_t& y) {
+    if (x.label != y.label || x.col != y.col || x.nback != y.nback || x.restrict_col != y.restrict_col ||
+        x.ncoll != y.ncoll)
+        return false;
+    for (int i = 0; i < x.nback && i < GV_MAXK; ++i)
+        if (x.back_col[i] != y.back_col[i] || x.back_part[i] != y.back_part[i]) return false;
+    for (int i = 0; i < x.ncoll && i < GV_MAXK; ++i)
+        if (x.coll_col[i] != y.coll_col[i]) return false;
+    return true;
+}
+
+// Widest table a slot extends from before it counts: its last step, a
+// single-edge tail counted with it, or its leaf groups' core.
+uint32_t counted_width(const BDesc& d) {
+    uint32_t w = d.q.k - 1;
+    if (d.q.k >= 3 && d.q.step[d.q.k - 1].nback == 1) w = d.q.k - 2;
+    if (d.has_leaves) w = std::min(w, d.leaves.width);
+    return w;
+}
+
+// The live slot whose table of the widest width d >= 3 equals this query's
+// own table of width d (same start label and steps 1 .. d-1, both extending
+// from width d); 0 = none.  A leader that itself waits to share a table at
+// least as wide is replaced by its own leader.
+uint32_t pick_leader(gv_batch* b, uint32_t self, const BDesc& d, uint32_t& leader) {
+    const uint32_t cwf = counted_width(d);
+    uint32_t best = 0;
+    leader = kNoSlot;
+    for (uint32_t p = 0; p < b->nslots; ++p) {
+        if (p == self || !b->used[p] || b->hstate[p] != GV_SLOT_RUNNING) continue;
+        const BDesc& e = b->h_desc[p];
+        if (e.q.label0 != d.q.label0) continue;
+        uint32_t m = 0;
+        const uint32_t kk = std::min(e.q.k, d.q.k);
+        while (m + 1 < kk && same_step(e.q.step[m + 1], d.q.step[m + 1])) ++m;
+        uint32_t w = std::min({m + 1, cwf, counted_width(e)});
+        if (w < 3 || w <= best || b->hwidth[p] > w) continue;
+        best = w;
+        leader = p;
+    }
+    while (leader != kNoSlot) {
+        const BDesc& e = b->h_desc[leader];
+        if (e.leader == kNoSlot || e.fork_width < best) break;
+        if (!b->used[e.leader] || b->h_st[e.leader].serial != e.leader_serial) break;
+        leader = e.leader;
+    }
+    return leader == kNoSlot ? 0 : best;
+}
+
 bool tune_ok(const gv_tune_t* t) {
     return t && (t->ipt == 1 || t->ipt == 2 || t->ipt == 4 || t->ipt == 8) &&
            t->walk_blocks_per_cu >= 1 && t->walk_blocks_per_cu <= 64;
@@ -730,6 +912,8 @@ int gv_batch_create(gv_ctx_t c, gv_graph_t g, uint32_t slots, uint64_t slot_byte
     b->used.assign(slots, 0);
     b->dirty.assign(slots, 0);
     b->flush_epoch.assign(slots, 0);
+    b->hwidth.assign(slots, 0);
+    b->hstate.assign(slots, GV_SLOT_FREE);
     b->gwords = g->ngraphs ? (g->ngraphs + 31) / 32 : 1;
     const uint64_t ntiles = (row_cap + kScanTile - 1) / kScanTile;
     bool ok = hipMalloc(&b->d_desc, sizeof(BDesc) * slots) == hipSuccess &&
@@ -741,7 +925,7 @@ int gv_batch_create(gv_ctx_t c, gv_graph_t g, uint32_t slots, uint64_t slot_byte
               hipMalloc(&b->d_anc_i, row_cap) == hipSuccess &&
               hipMalloc(&b->d_csum, 8 * row_cap) == hipSuccess &&
               hipMalloc(&b->d_tiles, 8 * (ntiles + 1)) == hipSuccess &&
-              hipMalloc(&b->d_glob, 8 * 4) == hipSuccess &&
+              hipMalloc(&b->d_glob, 8 * 8) == hipSuccess &&
               hipMalloc(&b->d_gbits, 4ull * b->gwords * slots) == hipSuccess &&
               hipMalloc(&b->d_sacc, 8ull * kBLines * kAccStride * slots) == hipSuccess &&
               hipHostMalloc((void**)&b->h_desc, sizeof(BDesc) * slots, hipHostMallocDefault) == hipSuccess &&
@@ -754,7 +938,7 @@ int gv_batch_create(gv_ctx_t c, gv_graph_t g, uint32_t slots, uint64_t slot_byte
     memset(b->h_zero, 0, sizeof(BState));
     HIPCHK(hipMemsetAsync(b->d_desc, 0, sizeof(BDesc) * slots, c->stream));
     HIPCHK(hipMemsetAsync(b->d_st, 0, sizeof(BState) * slots, c->stream));
-    HIPCHK(hipMemsetAsync(b->d_glob, 0, 8 * 4, c->stream));
+    HIPCHK(hipMemsetAsync(b->d_glob, 0, 8 * 8, c->stream));
     HIPCHK(hipMemsetAsync(b->d_gbits, 0, 4ull * b->gwords * slots, c->stream));
     HIPCHK(hipMemsetAsync(b->d_sacc, 0, 8ull * kBLines * kAccStride * slots, c->stream));
     int rc = capture(b);
@@ -871,6 +1055,24 @@ int gv_batch_admit_tuned(gv_batch_t b, const gv_query_t* q, const gv_leaves_t* l
         // each row it writes (8)
         st.bytes = 12ull * d.start_n + 4ull * (e1 - e0) + 8ull * (r1 - r0);
     }
+    st.serial = b->next_serial++;
+    d.init_width = st.width;
+    d.init_n = st.rows;
+    d.init_bytes = st.bytes;
+    d.leader = kNoSlot;
+    if (b->share) {
+        uint32_t p;
+        const uint32_t w = pick_leader(b, s, d, p);
+        if (w) {
+            d.leader = p;
+            d.fork_width = w;
+            d.leader_serial = b->h_st[p].serial;
+            st.wait = 1;
+            st.bytes = 0;   // the leader's counters at the shared width are taken over
+        }
+    }
+    b->hwidth[s] = st.width;
+    b->hstate[s] = GV_SLOT_RUNNING;
     b->dirty[s] = 1;
     b->used[s] = 1;
     *slot = s;
@@ -913,6 +1115,10 @@ int gv_batch_poll_bits(gv_batch_t b, gv_slot_info_t* info, uint32_t* bits, uint3
     rc = sync(b);
     if (rc != GV_OK) return rc;
     for (uint32_t s = 0; s < b->nslots; ++s) {
+        if (b->used[s]) {
+            b->hwidth[s] = h[s].width;
+            b->hstate[s] = h[s].state;
+        }
         info[s].tag = h[s].tag;
         info[s].state = b->used[s] ? h[s].state : GV_SLOT_FREE;
         info[s].width = h[s].width;
@@ -932,6 +1138,7 @@ int gv_batch_release(gv_batch_t b, uint32_t s, uint32_t* bits, uint32_t nwords) 
         // never reached the device: nothing to undo there
         b->dirty[s] = 0;
         b->used[s] = 0;
+        b->hstate[s] = GV_SLOT_FREE;
         return GV_OK;
     }
     if (bits && nwords) {
@@ -947,6 +1154,7 @@ int gv_batch_release(gv_batch_t b, uint32_t s, uint32_t* bits, uint32_t nwords) 
         if (rc != GV_OK) return rc;
     }
     b->used[s] = 0;
+    b->hstate[s] = GV_SLOT_FREE;
     return GV_OK;
 }
 
@@ -979,6 +1187,23 @@ int gv_batch_pair_start(gv_batch_t b, int on) {
     return GV_OK;
 }
 
+int gv_batch_share(gv_batch_t b, int on) {
+    if (!b) return GV_ERR_ARG;
+    b->share = on != 0;
+    return GV_OK;
+}
+
+int gv_batch_share_stats(gv_batch_t b, uint64_t* shared, uint64_t* restarted) {
+    if (!b) return GV_ERR_ARG;
+    uint64_t g[8];
+    HIPCHK(hipMemcpyAsync(g, b->d_glob, sizeof(g), hipMemcpyDeviceToHost, b->ctx->stream));
+    int rc = sync(b);
+    if (rc != GV_OK) return rc;
+    if (shared) *shared = g[4];
+    if (restarted) *restarted = g[5];
+    return GV_OK;
+}
+
 int gv_batch_history(gv_batch_t b, uint64_t* rows, uint64_t* candidates, uint32_t cap, uint32_t* n) {
     if (!b || !n) return GV_ERR_ARG;
     *n = (uint32_t)(b->hist.size() / 2);
@@ -1002,6 +1227,7 @@ int gv_batch_release_many(gv_batch_t b, uint32_t n, const uint32_t* slots, uint3
             if (bits) memset(bits + (uint64_t)i * nwords, 0, 4ull * nwords);
             b->dirty[s] = 0;
             b->used[s] = 0;
+        b->hstate[s] = GV_SLOT_FREE;
             continue;
         }
         if (bits) {
@@ -1012,6 +1238,7 @@ int gv_batch_release_many(gv_batch_t b, uint32_t n, const uint32_t* slots, uint3
         HIPCHK(hipMemcpyAsync(b->d_st + s, b->h_zero, sizeof(BState), hipMemcpyHostToDevice,
                               b->ctx->stream));
         b->used[s] = 0;
+        b->hstate[s] = GV_SLOT_FREE;
     }
     if (copied) return sync(b);
     return GV_OK;

@@ -325,6 +325,20 @@ int gv_batch_profile_bytes(gv_batch_t b, uint64_t* size_bytes, uint64_t* walk_by
  * width 2 with that table as its input, the first iteration is saved, and
  * the step's algorithmic bytes are added in closed form.  On by default. */
 int gv_batch_pair_start(gv_batch_t b, int on);
+/* Prefix sharing (on by default): a query admitted while a running slot's
+ * query has the same start label and the same first steps up to a table of
+ * width d >= 3 (both extending from that width) waits for that slot's table
+ * of width d and extends it, in the iteration the leader does, instead of
+ * building its own copy; it takes over the leader's counters at that point
+ * (the shared steps' algorithmic bytes).  A follower whose leader ran empty
+ * or overflowed before width d ends the same way; one whose leader went past
+ * width d or left its slot, or whose shared table cannot fit an iteration
+ * with its leader, starts on its own.  Applies to queries admitted from now
+ * on. */
+int gv_batch_share(gv_batch_t b, int on);
+/* Tables shared and followers restarted on their own since creation
+ * (synchronises). */
+int gv_batch_share_stats(gv_batch_t b, uint64_t* shared, uint64_t* restarted);
 /* Rows sized and candidates walked by each iteration run while the
  * context's profiling was on (oldest first; cleared when read into rows or
  * candidates). */

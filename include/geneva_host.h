@@ -180,6 +180,10 @@ int gh_engine_batch_profile_bytes(gh_engine_t e, uint64_t* size_bytes, uint64_t*
 /* Batch engine: first steps from the label-pair edge tables on or off
  * (gv_batch_pair_start). */
 int gh_engine_batch_pair_start(gh_engine_t e, int on);
+/* Batch engine: prefix sharing between queries on or off (gv_batch_share),
+ * and the tables shared and followers restarted so far. */
+int gh_engine_batch_share(gh_engine_t e, int on);
+int gh_engine_batch_share_stats(gh_engine_t e, uint64_t* shared, uint64_t* restarted);
 /* Rows sized and candidates walked per profiled batch iteration
  * (gv_batch_history). */
 int gh_engine_batch_history(gh_engine_t e, uint64_t* rows, uint64_t* candidates, uint32_t cap, uint32_t* n);

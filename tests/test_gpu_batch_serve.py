@@ -53,6 +53,34 @@ def test_each_result_same_alone_and_in_the_mix(w3):
         assert alone[0].graph_ids == m.graph_ids == plain.graph_ids
 
 
+def test_prefix_sharing_equals_unshared_tables(w3):
+    """Queries with the same first steps share one table: every result equals
+    the fixture with sharing on and off, tables were shared, and under a row
+    cap too small for some leaders with their followers those followers
+    restart on their own, with the same results."""
+    ds, eng, cases = w3
+    want = {c["query"]: c for c in cases}
+    qs = [c["query"] for c in cases]
+    qs = qs + qs[:12]   # admitted again: the same steps as the originals
+    for row_cap in (1 << 22, 1 << 14):
+        eng.batch_config(slots=64, slot_bytes=8 << 20, row_cap=row_cap)
+        s0 = eng.batch_share_stats()
+        on, _ = eng.mine_batch(qs)
+        s1 = eng.batch_share_stats()
+        eng.batch_share(False)
+        off, _ = eng.mine_batch(qs)
+        s2 = eng.batch_share_stats()
+        print("row_cap", row_cap, "shared/restarted", s0, s1, s2, flush=True)
+        assert s2 == s1
+        assert s1[0] > s0[0]
+        for q, a, b in zip(qs, on, off):
+            c = want[q]
+            assert a.count == b.count == c["count"]
+            assert a.alg_bytes == b.alg_bytes == c["alg_bytes"]
+            assert sorted(a.graph_ids) == sorted(b.graph_ids) == c["graph_ids"]
+    eng.batch_config(slots=64, slot_bytes=8 << 20, row_cap=1 << 22)
+
+
 def test_overflow_falls_back_to_plain_path(require_gpu):
     import oracle as O
     ds = configs.workload4(scale=0.005, nqueries=8)
