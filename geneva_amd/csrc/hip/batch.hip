@@ -8,7 +8,9 @@
 //              row space, deferral when the row space is full;
 //   k_b_size   per row of every running slot: shortest back-neighbour list,
 //              restriction cut, candidate count;
-//   k_b_scan_* grid-level scan of the counts (size read from device memory);
+//              each sizing tile's candidate prefix (within the tile) and total;
+//   k_b_tscan  one workgroup: the tiles' prefix, each slot's candidate range
+//              and walk tiles;
 //   k_b_walk   load-balanced walk over the candidates of all slots at once
 //              (a wave's tile may straddle slots; per-slot groups are formed
 //              with ballots), writing each slot's next table or counting;
@@ -26,6 +28,7 @@ namespace {
 // interleaved beat two here (the plain path's single-query levels use two).
 constexpr int kBSizeRows = 4;
 constexpr uint64_t kBSizeTile = (uint64_t)kBlock * kBSizeRows;
+static_assert(kBSizeTile == (1u << kPTileShift), "a sizing tile is a tile of the candidate prefix");
 constexpr uint32_t kMaxSlots = 1024;
 
 struct BDesc {               // written by the host at admission
@@ -99,12 +102,13 @@ struct BArgs {
     uint32_t* half0;
     uint32_t* half1;
     uint64_t slot_uints;
+    // per row of the iteration's row space (slots start at sizing tiles, so
+    // rows between a slot's last row and the next tile are unused)
     uint32_t* anc_beg;
-    uint32_t* cnt;
+    uint32_t* cnt;              // candidate prefix within the row's sizing tile (inclusive)
     uint8_t* anc_i;
-    uint64_t* csum;
     uint64_t row_cap;
-    uint64_t* tiles;
+    uint64_t* tiles;            // per sizing tile: its total, then the tiles' exclusive prefix
     unsigned long long* glob;   // [0] rows, [1] candidates, [2] tiles, [3] sizing tiles,
                                 // [4] shared tables (forks), [5] followers restarted
     uint64_t tile;              // candidates per walk tile (64 * IPT)
@@ -325,10 +329,11 @@ __global__ __launch_bounds__(1024) void k_b_prep(BArgs a) {
         rows = 0;
         mode = 0;
     }
-    // the running slots' row ranges, in slot order
-    uint64_t ex = block_scan_1024(rows, wsum, tot);
-    const uint64_t total_rows = tot;
-    // sizing tiles: kBSizeTile rows of one slot each
+    // the running slots' row ranges, in slot order, each from the start of a
+    // sizing tile (kBSizeTile rows of one slot), so that row r lies in tile
+    // r / kBSizeTile of the iteration's candidate prefix
+    uint64_t total_rows;
+    block_scan_1024(rows, wsum, total_rows);
     const uint64_t nrt = (rows + kBSizeTile - 1) / kBSizeTile;
     uint64_t rtot;
     const uint64_t rex = block_scan_1024(nrt, wsum, rtot);
@@ -338,7 +343,7 @@ __global__ __launch_bounds__(1024) void k_b_prep(BArgs a) {
         S.rmode = rmode;
         S.src = src;
         S.src_sel = src_sel;
-        S.row_end = ex + rows;
+        S.row_end = rex * kBSizeTile + rows;
         S.rtile_end = rex + nrt;
     }
     if (s == 0) {
@@ -348,8 +353,20 @@ __global__ __launch_bounds__(1024) void k_b_prep(BArgs a) {
     }
 }
 
+__device__ __forceinline__ Prefix two_level(const BArgs& a) {
+    Prefix p;
+    p.tpre = a.tiles;
+    p.loc = a.cnt;
+    return p;
+}
+
+// Sizing: one block per sizing tile (kBSizeTile rows of one slot, kBSizeRows
+// consecutive rows per thread).  Besides each row's anchor segment it writes
+// the candidate prefix of the tile's rows (two-level form: loc = inclusive
+// within the tile, in `cnt`) and the tile's total in tiles[tile].
 __global__ __launch_bounds__(kBlock) void k_b_size(BArgs a) {
     __shared__ uint64_t rtile_end[kMaxSlots];
+    __shared__ uint64_t wsum[kWavesPerBlock];
     for (uint32_t i = threadIdx.x; i < a.nslots; i += blockDim.x) rtile_end[i] = a.st[i].rtile_end;
     __syncthreads();
     const uint64_t ntiles = a.glob[3];
@@ -379,29 +396,27 @@ __global__ __launch_bounds__(kBlock) void k_b_size(BArgs a) {
         const uint64_t rows = L == 1 ? D.start_n : S.rows;
         const uint64_t nrt = (rows + kBSizeTile - 1) / kBSizeTile;
         const uint64_t tile0 = (RT - (rtile_end[s] - nrt)) * kBSizeTile;
-        const uint64_t rbase = S.row_end - rows;   // slot's first row in the iteration space
         const gv_step_t& st = D.q.step[L];
         const uint32_t* trows = slot_rows(a, s, S, D);
         uint64_t li[kBSizeRows];
         bool ok[kBSizeRows];
 #pragma unroll
         for (int k = 0; k < kBSizeRows; ++k) {
-            li[k] = tile0 + (uint64_t)k * kBlock + threadIdx.x;
+            li[k] = tile0 + (uint64_t)threadIdx.x * kBSizeRows + k;
             ok[k] = li[k] < rows;
         }
+        uint32_t beg[kBSizeRows], len[kBSizeRows], cnt[kBSizeRows], ai[kBSizeRows];
+#pragma unroll
+        for (int k = 0; k < kBSizeRows; ++k) beg[k] = cnt[k] = ai[k] = 0;
         if (S.mode == 4) {
             // leaf groups on these core rows: counted here, no candidates (a
             // row's own read is part of the leaf bytes)
 #pragma unroll
             for (int k = 0; k < kBSizeRows; ++k) {
                 if (!ok[k]) continue;
-                const uint64_t r = rbase + li[k];
                 const uint32_t* row = trows + li[k] * L;
                 uint64_t by = 0;
                 const uint64_t n = leaf_row_count(a.g, D.leaves, row, by, rq);
-                a.cnt[r] = 0;
-                a.anc_beg[r] = 0;
-                a.anc_i[r] = 0;
                 cacc += n;
                 acc += by;
                 if (n && a.g.gof) {
@@ -412,155 +427,105 @@ __global__ __launch_bounds__(kBlock) void k_b_size(BArgs a) {
                     }
                 }
             }
-            continue;
-        }
-        uint32_t beg[kBSizeRows], len[kBSizeRows], cnt[kBSizeRows], ai[kBSizeRows];
-        uint64_t by = 0;
-        size_rows<kBSizeRows>(a.g, st, trows, L, li, ok, beg, len, cnt, ai, by, rq,
-                             L == 1 ? (int64_t)D.start_off : (int64_t)-1);
-        acc += by;
-#pragma unroll
-        for (int k = 0; k < kBSizeRows; ++k) {
-            if (!ok[k]) continue;
-            const uint64_t r = rbase + li[k];
-            a.anc_beg[r] = beg[k];
-            a.anc_i[r] = (uint8_t)ai[k];
+        } else {
+            uint64_t by = 0;
+            size_rows<kBSizeRows>(a.g, st, trows, L, li, ok, beg, len, cnt, ai, by, rq,
+                                 L == 1 ? (int64_t)D.start_off : (int64_t)-1);
+            acc += by;
             if (S.rmode >= 0) {
                 // counted here, per row: no candidates are left for the walk
                 const gv_step_t& tl = D.q.step[L + 1 < GV_MAXK ? L + 1 : GV_MAXK - 1];
-                uint64_t tby = 0;
-                const uint64_t n = row_count(a.g, st, tl, S.rmode, L, trows + li[k] * L, beg[k],
-                                             beg[k] + len[k], tby, rq);
-                acc += tby;
-                a.cnt[r] = 0;
-                cacc += n;
-                if (n && a.g.gof) {
-                    const uint32_t gid = a.g.gof[trows[li[k] * L]];
-                    if (gid != last_gid) {
-                        atomicOr(&a.gbits[(uint64_t)s * a.gwords + (gid >> 5)], 1u << (gid & 31));
-                        last_gid = gid;
+#pragma unroll
+                for (int k = 0; k < kBSizeRows; ++k) {
+                    if (!ok[k]) continue;
+                    uint64_t tby = 0;
+                    const uint64_t n = row_count(a.g, st, tl, S.rmode, L, trows + li[k] * L, beg[k],
+                                                 beg[k] + len[k], tby, rq);
+                    acc += tby;
+                    cnt[k] = 0;
+                    cacc += n;
+                    if (n && a.g.gof) {
+                        const uint32_t gid = a.g.gof[trows[li[k] * L]];
+                        if (gid != last_gid) {
+                            atomicOr(&a.gbits[(uint64_t)s * a.gwords + (gid >> 5)], 1u << (gid & 31));
+                            last_gid = gid;
+                        }
                     }
                 }
-            } else {
-                a.cnt[r] = cnt[k];
             }
+#pragma unroll
+            for (int k = 0; k < kBSizeRows; ++k)
+                if (!ok[k]) cnt[k] = 0;
         }
+        // the tile's candidate prefix in row order (below 2^32: a row has at
+        // most max_deg < 2^32 / kBSizeTile candidates)
+        uint32_t incl[kBSizeRows];
+        uint32_t run = 0;
+#pragma unroll
+        for (int k = 0; k < kBSizeRows; ++k) {
+            run += cnt[k];
+            incl[k] = run;
+        }
+        uint64_t btot;
+        const uint32_t bex = (uint32_t)block_scan_1024(run, wsum, btot);
+        const uint64_t r0 = RT * kBSizeTile + (uint64_t)threadIdx.x * kBSizeRows;
+        static_assert(kBSizeRows == 4, "one 16-byte store per thread");
+        *reinterpret_cast<uint4*>(a.cnt + r0) =
+            make_uint4(bex + incl[0], bex + incl[1], bex + incl[2], bex + incl[3]);
+        *reinterpret_cast<uint4*>(a.anc_beg + r0) = make_uint4(beg[0], beg[1], beg[2], beg[3]);
+        *reinterpret_cast<uint32_t*>(a.anc_i + r0) =
+            (ai[0] & 0xffu) | (ai[1] & 0xffu) << 8 | (ai[2] & 0xffu) << 16 | (ai[3] & 0xffu) << 24;
+        if (threadIdx.x == 0) a.tiles[RT] = btot;
     }
     flush();
 }
 
-// ---- scan with the extent in device memory --------------------------------
-
-__global__ __launch_bounds__(kBlock) void k_b_scan_reduce(BArgs a) {
-    __shared__ uint64_t wsum[kWavesPerBlock];
-    const uint64_t n = a.glob[0];
-    const uint64_t ntiles = (n + kScanTile - 1) / kScanTile;
-    for (uint64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
-        const uint64_t base = t * kScanTile;
-        uint64_t s = 0;
-        for (int i = 0; i < kScanItems; ++i) {
-            const uint64_t idx = base + (uint64_t)i * kBlock + threadIdx.x;
-            if (idx < n) s += a.cnt[idx];
-        }
-        s = wave_sum_u64(s);
-        if (__lane_id() == 0) wsum[threadIdx.x >> 6] = s;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            uint64_t x = 0;
-            for (int i = 0; i < kWavesPerBlock; ++i) x += wsum[i];
-            a.tiles[t] = x;
-        }
-        __syncthreads();
+// The candidate prefix of the tiles (exclusive, in place in `tiles`), then
+// each running slot's candidate range and its walk tiles (tiles never
+// straddle slots).  One workgroup.
+__global__ __launch_bounds__(1024) void k_b_tscan(BArgs a) {
+    __shared__ uint64_t wsum[16];
+    const uint64_t nt = a.glob[3];
+    const uint64_t per = (nt + blockDim.x - 1) / blockDim.x;
+    const uint64_t b0 = min((uint64_t)threadIdx.x * per, nt), b1 = min(b0 + per, nt);
+    uint64_t sum = 0;
+    for (uint64_t i = b0; i < b1; ++i) sum += a.tiles[i];
+    uint64_t cand;
+    uint64_t ex = block_scan_1024(sum, wsum, cand);
+    for (uint64_t i = b0; i < b1; ++i) {
+        const uint64_t v = a.tiles[i];
+        a.tiles[i] = ex;
+        ex += v;
     }
-}
-
-__global__ __launch_bounds__(kBlock) void k_b_scan_tiles(BArgs a) {
-    __shared__ uint64_t wsum[kWavesPerBlock];
-    const uint64_t n = a.glob[0];
-    const uint64_t ntiles = (n + kScanTile - 1) / kScanTile;
-    uint64_t carry = 0;
-    for (uint64_t b = 0; b < ntiles; b += kBlock) {
-        const uint64_t idx = b + threadIdx.x;
-        const uint64_t v = idx < ntiles ? a.tiles[idx] : 0;
-        uint64_t tot;
-        const uint64_t ex = block_scan_1024(v, wsum, tot);
-        if (idx < ntiles) a.tiles[idx] = carry + ex;
-        carry += tot;
-        __syncthreads();
+    __syncthreads();
+    const uint32_t s = threadIdx.x;
+    uint64_t nwt = 0, cb = 0, ce = 0;
+    if (s < a.nslots) {
+        const BState& S = a.st[s];
+        if (S.mode != 0) {
+            const Prefix pf = two_level(a);
+            const uint64_t rows = S.width == 1 ? a.desc[s].start_n : S.rows;
+            const uint64_t re = S.row_end, rb = re - rows;
+            cb = pf.begin(rb);
+            ce = pf.end(re - 1);
+            const uint64_t tl = a.desc[s].tile ? a.desc[s].tile : a.tile;
+            nwt = (ce - cb + tl - 1) / tl;
+        }
     }
-    if (threadIdx.x == 0) a.glob[1] = carry;
-}
-
-__global__ __launch_bounds__(kBlock) void k_b_scan_final(BArgs a) {
-    // the counts and the sums share one padded tile: the counts are all read
-    // (and the block scan has synchronised) before the first sum is written
-    __shared__ uint64_t outv[kScanTilePadded / 2];
-    uint32_t* vals = reinterpret_cast<uint32_t*>(outv);
-    __shared__ uint64_t wsum[kWavesPerBlock];
-    const uint64_t n = a.glob[0];
-    const uint64_t ntiles = (n + kScanTile - 1) / kScanTile;
-    for (uint64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
-        const uint64_t base = t * kScanTile;
-        for (int i = 0; i < kScanItems; ++i) {
-            const uint64_t idx = base + (uint64_t)i * kBlock + threadIdx.x;
-            vals[lds_pad16(i * kBlock + threadIdx.x)] = idx < n ? a.cnt[idx] : 0u;
-        }
-        __syncthreads();
-        uint64_t loc[kScanItems];
-        uint64_t s = 0;
-#pragma unroll
-        for (int i = 0; i < kScanItems; ++i) {
-            s += vals[lds_pad16(threadIdx.x * kScanItems + i)];
-            loc[i] = s;
-        }
-        uint64_t tot;
-        const uint64_t ex = block_scan_1024(s, wsum, tot) + a.tiles[t];
-        for (int half = 0; half < 2; ++half) {
-            __syncthreads();
-            const int lo = half * (kScanTile / 2);
-#pragma unroll
-            for (int i = 0; i < kScanItems; ++i) {
-                const int p = threadIdx.x * kScanItems + i - lo;
-                if (p >= 0 && p < kScanTile / 2) outv[lds_pad16(p)] = ex + loc[i];
-            }
-            __syncthreads();
-            for (int i = threadIdx.x; i < kScanTile / 2; i += kBlock) {
-                const uint64_t idx = base + lo + i;
-                if (idx < n) a.csum[idx] = outv[lds_pad16(i)];
-            }
-        }
-        __syncthreads();
+    uint64_t tot;
+    const uint64_t wex = block_scan_1024(nwt, wsum, tot);
+    if (s < a.nslots) {
+        a.st[s].cand_base = cb;
+        a.st[s].cand_end = ce;
+        a.st[s].tile_end = wex + nwt;
+    }
+    if (s == 0) {
+        a.glob[1] = cand;
+        a.glob[2] = tot;
     }
 }
 
 // ---- walk -------------------------------------------------------------------
-
-// Lay the walk out in tiles that never straddle two slots: per slot, its
-// candidate range from the scan and ceil(range / tile) tiles.
-__global__ __launch_bounds__(1024) void k_b_tiles(BArgs a) {
-    __shared__ uint64_t wsum[16];
-    const uint32_t s = threadIdx.x;
-    uint64_t nt = 0, cb = 0, ce = 0;
-    if (s < a.nslots) {
-        const BState& S = a.st[s];
-        if (S.mode != 0) {
-            const uint64_t rows = S.width == 1 ? a.desc[s].start_n : S.rows;
-            const uint64_t re = S.row_end, rb = re - rows;
-            cb = rb ? a.csum[rb - 1] : 0;
-            ce = a.csum[re - 1];
-            const uint64_t tl = a.desc[s].tile ? a.desc[s].tile : a.tile;
-            nt = (ce - cb + tl - 1) / tl;
-        }
-    }
-    uint64_t tot;
-    const uint64_t ex = block_scan_1024(nt, wsum, tot);
-    if (s < a.nslots) {
-        a.st[s].cand_base = cb;
-        a.st[s].cand_end = ce;
-        a.st[s].tile_end = ex + nt;
-    }
-    if (s == 0) a.glob[2] = tot;
-}
 
 template <int IPT>
 __global__ __launch_bounds__(kBlock) void k_b_walk(BArgs a) {
@@ -614,8 +579,7 @@ __global__ __launch_bounds__(kBlock) void k_b_walk(BArgs a) {
         in.anc = a.g.nbr;
         in.row_base = S.row_end - rows;
         in.L = L;
-        in.csum = a.csum;
-        in.cnt = a.cnt;
+        in.pf = two_level(a);
         in.anc_beg = a.anc_beg;
         in.anc_i = a.anc_i;
         in.row_lo = S.row_end - rows;
@@ -685,7 +649,6 @@ struct gv_batch {
     uint32_t* d_anc_beg = nullptr;
     uint32_t* d_cnt = nullptr;
     uint8_t* d_anc_i = nullptr;
-    uint64_t* d_csum = nullptr;
     uint64_t* d_tiles = nullptr;
     unsigned long long* d_glob = nullptr;
     uint32_t* d_gbits = nullptr;
@@ -731,7 +694,6 @@ BArgs bargs(gv_batch* b) {
     a.anc_beg = b->d_anc_beg;
     a.cnt = b->d_cnt;
     a.anc_i = b->d_anc_i;
-    a.csum = b->d_csum;
     a.row_cap = b->row_cap;
     a.tiles = b->d_tiles;
     a.glob = b->d_glob;
@@ -750,7 +712,6 @@ void launch_iteration(gv_batch* b, bool prof) {
     gv_ctx* c = b->ctx;
     BArgs a = bargs(b);
     const int size_blocks = c->num_cu * 8;
-    const int scan_blocks = c->num_cu * 4;
     const int walk_blocks = c->num_cu * (int)b->tune.walk_blocks_per_cu;
     // events only when profiling (never inside a capture)
     std::optional<ProfScope> ps;
@@ -763,11 +724,7 @@ void launch_iteration(gv_batch* b, bool prof) {
     timed(kKindSize);
     hipLaunchKernelGGL(k_b_size, dim3(size_blocks), dim3(kBlock), 0, c->stream, a);
     timed(kKindScan);
-    hipLaunchKernelGGL(k_b_scan_reduce, dim3(scan_blocks), dim3(kBlock), 0, c->stream, a);
-    hipLaunchKernelGGL(k_b_scan_tiles, dim3(1), dim3(kBlock), 0, c->stream, a);
-    hipLaunchKernelGGL(k_b_scan_final, dim3(scan_blocks), dim3(kBlock), 0, c->stream, a);
-    timed(kKindBatch);
-    hipLaunchKernelGGL(k_b_tiles, dim3(1), dim3(1024), 0, c->stream, a);
+    hipLaunchKernelGGL(k_b_tscan, dim3(1), dim3(1024), 0, c->stream, a);
     timed(kKindExt);
     switch (b->tune.ipt) {
         case 1: hipLaunchKernelGGL(k_b_walk<1>, dim3(walk_blocks), dim3(kBlock), 0, c->stream, a); break;
@@ -900,6 +857,8 @@ int gv_batch_create(gv_ctx_t c, gv_graph_t g, uint32_t slots, uint64_t slot_byte
     if (!c || !g || !out || slots == 0 || slots > kMaxSlots || slot_bytes < 4096 || row_cap == 0)
         return GV_ERR_ARG;
     if (tune && !tune_ok(tune)) return GV_ERR_ARG;
+    // a sizing tile's candidate prefix is kept in 32 bits
+    if ((uint64_t)g->max_deg * kBSizeTile >= (1ull << 32)) return GV_ERR_ARG;
     *out = nullptr;
     HIPCHK(hipSetDevice(c->device));
     gv_batch* b = new gv_batch();
@@ -915,15 +874,17 @@ int gv_batch_create(gv_ctx_t c, gv_graph_t g, uint32_t slots, uint64_t slot_byte
     b->hwidth.assign(slots, 0);
     b->hstate.assign(slots, GV_SLOT_FREE);
     b->gwords = g->ngraphs ? (g->ngraphs + 31) / 32 : 1;
-    const uint64_t ntiles = (row_cap + kScanTile - 1) / kScanTile;
+    // the row space: row_cap rows and up to kBSizeTile - 1 unused rows after
+    // each slot's
+    const uint64_t space = row_cap + kBSizeTile * (uint64_t)slots;
+    const uint64_t ntiles = space / kBSizeTile;
     bool ok = hipMalloc(&b->d_desc, sizeof(BDesc) * slots) == hipSuccess &&
               hipMalloc(&b->d_st, sizeof(BState) * slots) == hipSuccess &&
               hipMalloc(&b->d_half[0], b->slot_bytes * slots) == hipSuccess &&
               hipMalloc(&b->d_half[1], b->slot_bytes * slots) == hipSuccess &&
-              hipMalloc(&b->d_anc_beg, 4 * row_cap) == hipSuccess &&
-              hipMalloc(&b->d_cnt, 4 * row_cap) == hipSuccess &&
-              hipMalloc(&b->d_anc_i, row_cap) == hipSuccess &&
-              hipMalloc(&b->d_csum, 8 * row_cap) == hipSuccess &&
+              hipMalloc(&b->d_anc_beg, 4 * space) == hipSuccess &&
+              hipMalloc(&b->d_cnt, 4 * space) == hipSuccess &&
+              hipMalloc(&b->d_anc_i, space) == hipSuccess &&
               hipMalloc(&b->d_tiles, 8 * (ntiles + 1)) == hipSuccess &&
               hipMalloc(&b->d_glob, 8 * 8) == hipSuccess &&
               hipMalloc(&b->d_gbits, 4ull * b->gwords * slots) == hipSuccess &&
@@ -959,7 +920,7 @@ int gv_batch_destroy(gv_batch_t b) {
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
     void* ptrs[] = {b->d_desc, b->d_st, b->d_half[0], b->d_half[1], b->d_anc_beg, b->d_cnt,
-                    b->d_anc_i, b->d_csum, b->d_tiles, b->d_glob, b->d_gbits, b->d_sacc, b->d_kb};
+                    b->d_anc_i, b->d_tiles, b->d_glob, b->d_gbits, b->d_sacc, b->d_kb};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (b->h_desc) (void)hipHostFree(b->h_desc);

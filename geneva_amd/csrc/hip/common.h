@@ -634,6 +634,64 @@ __device__ __forceinline__ void block_acc_add2(unsigned long long* p0, uint64_t 
 // non-anchor neighbour segments) and the lanes stream consecutive anchor
 // entries; otherwise each lane finds its row by binary search in the scan.
 
+// Candidate prefix of a table's rows: row r's candidates are [begin(r),
+// end(r)) of the step's candidate space.  Flat: csum = inclusive prefix per
+// row (cnt = the row's count).  Two-level (csum null): the rows form tiles of
+// 2^kPTileShift rows, tpre = exclusive prefix of the tiles' totals and loc =
+// inclusive prefix of each row within its tile (32 bits).
+constexpr uint32_t kPTileShift = 10;
+struct Prefix {
+    const uint64_t* csum = nullptr;
+    const uint32_t* cnt = nullptr;
+    const uint64_t* tpre = nullptr;
+    const uint32_t* loc = nullptr;
+    __device__ __forceinline__ uint64_t end(uint64_t r) const {
+        return csum ? csum[r] : tpre[r >> kPTileShift] + loc[r];
+    }
+    __device__ __forceinline__ uint64_t begin(uint64_t r) const {
+        if (csum) return csum[r] - cnt[r];
+        const uint64_t t = tpre[r >> kPTileShift];
+        return (r & ((1u << kPTileShift) - 1)) ? t + loc[r - 1] : t;
+    }
+};
+
+// first r in [lo, hi) with end(r) > c
+__device__ __forceinline__ uint64_t find_row(const Prefix& p, uint64_t lo, uint64_t hi, uint64_t c) {
+    uint64_t n = hi - lo;
+    while (n > 0) {
+        uint64_t h = n >> 1;
+        if (p.end(lo + h) <= c) {
+            lo += h + 1;
+            n -= h + 1;
+        } else {
+            n = h;
+        }
+    }
+    return lo;
+}
+
+// wave_find_row over a Prefix (see wave_find_row above).
+__device__ __forceinline__ uint64_t wave_find_row(const Prefix& pf, uint64_t lo, uint64_t hi, uint64_t c) {
+    const unsigned lane = __lane_id();
+    while (hi - lo > 1) {
+        const uint64_t n = hi - lo;
+        const uint64_t step = (n + 63) >> 6;
+        const uint64_t x = lo + (uint64_t)lane * step;
+        const bool p = x < hi && pf.end(x) > c;
+        const uint64_t m = __ballot(p);
+        if (m == 0) {
+            const uint64_t last = (n - 1) / step < 63 ? (n - 1) / step : 63;
+            lo = lo + last * step + 1;
+            continue;
+        }
+        const unsigned j = __builtin_ctzll(m);
+        if (j == 0) return lo;
+        hi = lo + (uint64_t)j * step + 1;
+        lo = lo + (uint64_t)(j - 1) * step + 1;
+    }
+    return lo;
+}
+
 struct TileIn {
     const uint32_t* rows;    // table of the step's input rows (row r at rows + (r - row_base) * L)
     // anchor entries: a row's candidates are anc[anc_beg[r] ..]; the graph's
@@ -641,8 +699,7 @@ struct TileIn {
     const uint32_t* anc;
     uint64_t row_base;
     uint32_t L;
-    const uint64_t* csum;    // inclusive candidate prefix per row
-    const uint32_t* cnt;
+    Prefix pf;               // the rows' candidate ranges
     const uint32_t* anc_beg;
     const uint8_t* anc_i;
     uint64_t row_lo, row_hi; // rows searched for the tile's candidates
@@ -728,15 +785,15 @@ __device__ __forceinline__ void walk_tile(const DevGraph& g, const gv_step_t& st
                                           const BlockReserve& reserve) {
     const unsigned lane = __lane_id();
     const uint32_t L = in.L, W = L + 1;
-    const uint64_t r_lo = wave_find_row(in.csum, in.row_lo, in.row_hi, t0);
+    const uint64_t r_lo = wave_find_row(in.pf, in.row_lo, in.row_hi, t0);
     // the tile's last row is mostly within 64 rows of its first: one round of
     // 64 consecutive rows, then the wide search past them (when none of them
     // ends after t1 - 1, r_lo + 64 < row_hi, since csum[row_hi - 1] does)
     uint64_t r_hi;
     {
         const uint64_t x = r_lo + lane;
-        const uint64_t m = __ballot(x < in.row_hi && in.csum[x] > t1 - 1);
-        r_hi = m ? r_lo + (uint64_t)__builtin_ctzll(m) : wave_find_row(in.csum, r_lo + 64, in.row_hi, t1 - 1);
+        const uint64_t m = __ballot(x < in.row_hi && in.pf.end(x) > t1 - 1);
+        r_hi = m ? r_lo + (uint64_t)__builtin_ctzll(m) : wave_find_row(in.pf, r_lo + 64, in.row_hi, t1 - 1);
     }
     auto rowp = [&](uint64_t r) { return in.rows + (r - in.row_base) * L; };
 
@@ -759,7 +816,7 @@ __device__ __forceinline__ void walk_tile(const DevGraph& g, const gv_step_t& st
         // one row: its data is loaded once and kept in scalars; ids are below
         // 0xffffffff, so an unused same-label slot never matches
         const uint64_t r = r_lo;
-        const uint64_t first = in.csum[r] - in.cnt[r];
+        const uint64_t first = in.pf.begin(r);
         const uint32_t abeg = in.anc_beg[r];
         const uint32_t skip = in.anc_i[r];
         const uint32_t* row = rowp(r);
@@ -872,7 +929,7 @@ __device__ __forceinline__ void walk_tile(const DevGraph& g, const gv_step_t& st
         // own start is first0.
         const uint64_t nr = r_hi - r_lo + 1;
         const bool staged = nr <= 64ull * IPT;
-        const uint64_t first0 = in.csum[r_lo] - in.cnt[r_lo];
+        const uint64_t first0 = in.pf.begin(r_lo);
         // up to kRowStage rows: each row's data is also staged once (in the
         // window space, kRowWords words a row): anchor start, the other
         // back lists' segments, the same-label values and whether the row's
@@ -881,7 +938,7 @@ __device__ __forceinline__ void walk_tile(const DevGraph& g, const gv_step_t& st
         const uint32_t nob = st.nback - 1;
         if (staged) {
             for (uint32_t i = lane; i <= nr; i += 64) {
-                const uint64_t x = i < nr ? in.csum[r_lo + i] - in.cnt[r_lo + i] : in.csum[r_hi];
+                const uint64_t x = i < nr ? in.pf.begin(r_lo + i) : in.pf.end(r_hi);
                 lx[i] = x <= t0 ? 0u : (x - t0 > 0xffffffffull ? 0xffffffffu : (uint32_t)(x - t0));
             }
             if (rstaged && lane < nr) {
@@ -972,8 +1029,8 @@ __device__ __forceinline__ void walk_tile(const DevGraph& g, const gv_step_t& st
                     r = r_lo + lo;
                     first = lo == 0 ? first0 : t0 + lx[lo];
                 } else {
-                    r = find_row(in.csum, r_lo, r_hi + 1, c);
-                    first = in.csum[r] - in.cnt[r];
+                    r = find_row(in.pf, r_lo, r_hi + 1, c);
+                    first = in.pf.begin(r);
                 }
                 v = in.anc[in.anc_beg[r] + (uint32_t)(c - first)];
                 acc.rq += 4;

@@ -859,8 +859,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) 
     in.anc = a.anc_base ? a.anc_base : a.g.nbr;
     in.row_base = 0;
     in.L = a.L;
-    in.csum = a.csum;
-    in.cnt = a.cnt;
+    in.pf.csum = a.csum;
+    in.pf.cnt = a.cnt;
     in.anc_beg = a.anc_beg;
     in.anc_i = a.anc_i;
     in.row_lo = 0;

@@ -274,7 +274,8 @@ typedef struct {
 } gv_slot_info_t;
 
 /* row_cap: rows the whole batch may size in one iteration (queries beyond it
- * wait for the next iteration). */
+ * wait for the next iteration).  GV_ERR_ARG for a graph with a neighbour list
+ * of 2^22 entries or more. */
 int gv_batch_create(gv_ctx_t ctx, gv_graph_t g, uint32_t slots, uint64_t slot_bytes,
                     uint64_t row_cap, const gv_tune_t* tune, gv_batch_t* out);
 int gv_batch_destroy(gv_batch_t b);
