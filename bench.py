@@ -280,7 +280,10 @@ def section_w3(args):
            "plain_embeddings_per_s": round(emb / (tp / 1e3), 1),
            "speedup_vs_plain": round(tp / ms_step, 3),
            "overflowed_to_plain": int(st["overflowed"]), "graph_launches": int(st["graph_launches"]),
-           "batch_device_ms": round(st["device_ms"], 3), "tune": tune, "parity": parity,
+           "batch_device_ms": round(st["device_ms"], 3), "batch_wall_ms": round(st["wall_ms"], 3),
+           "planning_ms": round(st["compile_ms"], 3), "plain_rerun_ms": round(st["fallback_ms"], 3),
+           "pieces": int(st["pieces"]),
+           "tune": tune, "parity": parity,
            "shared_tables": shared,
            "roofline": roofline, "kernels": kern,
            "plain_device_ms_sum": round(sum(r.ms for r in plain), 3)}

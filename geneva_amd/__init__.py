@@ -368,7 +368,7 @@ class Engine:
         ab = (C.c_uint64 * max(n, 1))()
         nwords = (self.ngraphs + 31) // 32 if self.ngraphs > 1 else 0
         bits = (C.c_uint32 * max(n * nwords, 1))()
-        st = (C.c_double * 5)()
+        st = (C.c_double * 8)()
         rc = lib().gh_engine_batch_mine(self._h, self._cstrs(queries), n, int(single_v), counts, ab,
                                         bits, nwords, st)
         if rc:
@@ -378,7 +378,8 @@ class Engine:
         for i in range(n):
             w = words[i * nwords:(i + 1) * nwords]
             res.append(MineResult(counts[i], ab[i], 0.0, w, 0, self.ngraphs))
-        stats = dict(zip(("batched", "overflowed", "graph_launches", "device_ms", "wall_ms"), list(st)))
+        stats = dict(zip(("batched", "overflowed", "graph_launches", "device_ms", "wall_ms", "compile_ms",
+                          "fallback_ms", "pieces"), list(st)))
         return res, stats
 
     def autotune(self, queries):

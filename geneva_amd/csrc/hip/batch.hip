@@ -27,7 +27,8 @@ namespace {
 // slots at once and are large, so four rows per thread with their loads
 // interleaved beat two here (the plain path's single-query levels use two).
 constexpr int kBSizeRows = 4;
-constexpr uint64_t kBSizeTile = (uint64_t)kBlock * kBSizeRows;
+constexpr int kBSizeThreads = 256;
+constexpr uint64_t kBSizeTile = (uint64_t)kBSizeThreads * kBSizeRows;
 static_assert(kBSizeTile == (1u << kPTileShift), "a sizing tile is a tile of the candidate prefix");
 constexpr uint32_t kMaxSlots = 1024;
 
@@ -55,7 +56,7 @@ struct BDesc {               // written by the host at admission
     // where the slot starts (width, rows, bytes of a pair start): a follower
     // that cannot share restarts from here
     uint32_t init_width;
-    uint32_t pad_;
+    uint32_t piece;          // one of several pieces of the query's first table: never shares
     uint64_t init_n;
     uint64_t init_bytes;
 };
@@ -174,6 +175,16 @@ __device__ uint64_t block_scan_1024(uint64_t v, uint64_t* wsum, uint64_t& total)
     }
     __syncthreads();
     return off + x - v;
+}
+
+// Sizing tiles of a slot with `rows` rows, rounded up to a multiple of the
+// GPU's XCD count.  Workgroup b runs on XCD b mod kXcds and the sizing grid
+// is a multiple of kXcds, so tile k of two adjacent slots runs on one XCD at
+// about the same time: slots that start from the same table (admitted next
+// to each other) then find each other's label-offset lines in that XCD's L2.
+constexpr uint64_t kXcds = 8;
+__device__ __forceinline__ uint64_t slot_tiles(uint64_t rows) {
+    return ((rows + kBSizeTile - 1) / kBSizeTile + kXcds - 1) / kXcds * kXcds;
 }
 
 // Mode of a slot whose input table has width w: 1 extend, 2 count the last
@@ -331,10 +342,11 @@ __global__ __launch_bounds__(1024) void k_b_prep(BArgs a) {
     }
     // the running slots' row ranges, in slot order, each from the start of a
     // sizing tile (kBSizeTile rows of one slot), so that row r lies in tile
-    // r / kBSizeTile of the iteration's candidate prefix
+    // r / kBSizeTile of the iteration's candidate prefix; a slot's tiles are
+    // a multiple of kXcds (see slot_tiles)
     uint64_t total_rows;
     block_scan_1024(rows, wsum, total_rows);
-    const uint64_t nrt = (rows + kBSizeTile - 1) / kBSizeTile;
+    const uint64_t nrt = slot_tiles(rows);
     uint64_t rtot;
     const uint64_t rex = block_scan_1024(nrt, wsum, rtot);
     if (in) {
@@ -353,6 +365,32 @@ __global__ __launch_bounds__(1024) void k_b_prep(BArgs a) {
     }
 }
 
+// A thread's R consecutive rows of a per-row array in one store (R words
+// from an R-aligned row).
+template <int R>
+__device__ __forceinline__ void store_rows(uint32_t* p, const uint32_t (&v)[R]) {
+    if constexpr (R == 4) {
+        *reinterpret_cast<uint4*>(p) = make_uint4(v[0], v[1], v[2], v[3]);
+    } else if constexpr (R == 2) {
+        *reinterpret_cast<uint2*>(p) = make_uint2(v[0], v[1]);
+    } else {
+#pragma unroll
+        for (int k = 0; k < R; ++k) p[k] = v[k];
+    }
+}
+template <int R>
+__device__ __forceinline__ void store_bytes(uint8_t* p, const uint32_t (&v)[R]) {
+    if constexpr (R == 4) {
+        *reinterpret_cast<uint32_t*>(p) =
+            (v[0] & 0xffu) | (v[1] & 0xffu) << 8 | (v[2] & 0xffu) << 16 | (v[3] & 0xffu) << 24;
+    } else if constexpr (R == 2) {
+        *reinterpret_cast<uint16_t*>(p) = (uint16_t)((v[0] & 0xffu) | (v[1] & 0xffu) << 8);
+    } else {
+#pragma unroll
+        for (int k = 0; k < R; ++k) p[k] = (uint8_t)v[k];
+    }
+}
+
 __device__ __forceinline__ Prefix two_level(const BArgs& a) {
     Prefix p;
     p.tpre = a.tiles;
@@ -364,9 +402,9 @@ __device__ __forceinline__ Prefix two_level(const BArgs& a) {
 // consecutive rows per thread).  Besides each row's anchor segment it writes
 // the candidate prefix of the tile's rows (two-level form: loc = inclusive
 // within the tile, in `cnt`) and the tile's total in tiles[tile].
-__global__ __launch_bounds__(kBlock) void k_b_size(BArgs a) {
+__global__ __launch_bounds__(kBSizeThreads) void k_b_size(BArgs a) {
     __shared__ uint64_t rtile_end[kMaxSlots];
-    __shared__ uint64_t wsum[kWavesPerBlock];
+    __shared__ uint64_t wsum[kBSizeThreads / 64];
     for (uint32_t i = threadIdx.x; i < a.nslots; i += blockDim.x) rtile_end[i] = a.st[i].rtile_end;
     __syncthreads();
     const uint64_t ntiles = a.glob[3];
@@ -394,8 +432,7 @@ __global__ __launch_bounds__(kBlock) void k_b_size(BArgs a) {
         const BDesc& D = a.desc[s];
         const uint32_t L = S.width;
         const uint64_t rows = L == 1 ? D.start_n : S.rows;
-        const uint64_t nrt = (rows + kBSizeTile - 1) / kBSizeTile;
-        const uint64_t tile0 = (RT - (rtile_end[s] - nrt)) * kBSizeTile;
+        const uint64_t tile0 = (RT - (rtile_end[s] - slot_tiles(rows))) * kBSizeTile;
         const gv_step_t& st = D.q.step[L];
         const uint32_t* trows = slot_rows(a, s, S, D);
         uint64_t li[kBSizeRows];
@@ -469,12 +506,11 @@ __global__ __launch_bounds__(kBlock) void k_b_size(BArgs a) {
         uint64_t btot;
         const uint32_t bex = (uint32_t)block_scan_1024(run, wsum, btot);
         const uint64_t r0 = RT * kBSizeTile + (uint64_t)threadIdx.x * kBSizeRows;
-        static_assert(kBSizeRows == 4, "one 16-byte store per thread");
-        *reinterpret_cast<uint4*>(a.cnt + r0) =
-            make_uint4(bex + incl[0], bex + incl[1], bex + incl[2], bex + incl[3]);
-        *reinterpret_cast<uint4*>(a.anc_beg + r0) = make_uint4(beg[0], beg[1], beg[2], beg[3]);
-        *reinterpret_cast<uint32_t*>(a.anc_i + r0) =
-            (ai[0] & 0xffu) | (ai[1] & 0xffu) << 8 | (ai[2] & 0xffu) << 16 | (ai[3] & 0xffu) << 24;
+#pragma unroll
+        for (int k = 0; k < kBSizeRows; ++k) incl[k] += bex;
+        store_rows(a.cnt + r0, incl);
+        store_rows(a.anc_beg + r0, beg);
+        store_bytes(a.anc_i + r0, ai);
         if (threadIdx.x == 0) a.tiles[RT] = btot;
     }
     flush();
@@ -711,7 +747,7 @@ BArgs bargs(gv_batch* b) {
 void launch_iteration(gv_batch* b, bool prof) {
     gv_ctx* c = b->ctx;
     BArgs a = bargs(b);
-    const int size_blocks = c->num_cu * 8;
+    const int size_blocks = c->num_cu * (2048 / kBSizeThreads);
     const int walk_blocks = c->num_cu * (int)b->tune.walk_blocks_per_cu;
     // events only when profiling (never inside a capture)
     std::optional<ProfScope> ps;
@@ -722,7 +758,7 @@ void launch_iteration(gv_batch* b, bool prof) {
     timed(kKindBatch);
     hipLaunchKernelGGL(k_b_prep, dim3(1), dim3(1024), 0, c->stream, a);
     timed(kKindSize);
-    hipLaunchKernelGGL(k_b_size, dim3(size_blocks), dim3(kBlock), 0, c->stream, a);
+    hipLaunchKernelGGL(k_b_size, dim3(size_blocks), dim3(kBSizeThreads), 0, c->stream, a);
     timed(kKindScan);
     hipLaunchKernelGGL(k_b_tscan, dim3(1), dim3(1024), 0, c->stream, a);
     timed(kKindExt);
@@ -825,6 +861,7 @@ uint32_t pick_leader(gv_batch* b, uint32_t self, const BDesc& d, uint32_t& leade
     for (uint32_t p = 0; p < b->nslots; ++p) {
         if (p == self || !b->used[p] || b->hstate[p] != GV_SLOT_RUNNING) continue;
         const BDesc& e = b->h_desc[p];
+        if (e.piece) continue;
         if (e.q.label0 != d.q.label0) continue;
         uint32_t m = 0;
         const uint32_t kk = std::min(e.q.k, d.q.k);
@@ -874,9 +911,9 @@ int gv_batch_create(gv_ctx_t c, gv_graph_t g, uint32_t slots, uint64_t slot_byte
     b->hwidth.assign(slots, 0);
     b->hstate.assign(slots, GV_SLOT_FREE);
     b->gwords = g->ngraphs ? (g->ngraphs + 31) / 32 : 1;
-    // the row space: row_cap rows and up to kBSizeTile - 1 unused rows after
-    // each slot's
-    const uint64_t space = row_cap + kBSizeTile * (uint64_t)slots;
+    // the row space: row_cap rows and fewer than kXcds tiles of unused rows
+    // after each slot's
+    const uint64_t space = row_cap + kXcds * kBSizeTile * (uint64_t)slots;
     const uint64_t ntiles = space / kBSizeTile;
     bool ok = hipMalloc(&b->d_desc, sizeof(BDesc) * slots) == hipSuccess &&
               hipMalloc(&b->d_st, sizeof(BState) * slots) == hipSuccess &&
@@ -949,6 +986,12 @@ int gv_batch_admit_leaves(gv_batch_t b, const gv_query_t* q, const gv_leaves_t* 
 
 int gv_batch_admit_tuned(gv_batch_t b, const gv_query_t* q, const gv_leaves_t* leaves, uint32_t ipt,
                          uint64_t tag, uint32_t* slot) {
+    return gv_batch_admit_part(b, q, leaves, ipt, tag, 0, 1, slot);
+}
+
+int gv_batch_admit_part(gv_batch_t b, const gv_query_t* q, const gv_leaves_t* leaves, uint32_t ipt,
+                        uint64_t tag, uint32_t part, uint32_t nparts, uint32_t* slot) {
+    if (nparts == 0 || part >= nparts) return GV_ERR_ARG;
     if (b && ipt && (ipt > b->tune.ipt || (ipt & (ipt - 1)))) return GV_ERR_ARG;
     if (!b || !q || !slot || q->k < 2 || q->k > GV_MAXK || q->label0 >= b->g->nlabels)
         return GV_ERR_ARG;
@@ -1016,12 +1059,28 @@ int gv_batch_admit_tuned(gv_batch_t b, const gv_query_t* q, const gv_leaves_t* l
         // each row it writes (8)
         st.bytes = 12ull * d.start_n + 4ull * (e1 - e0) + 8ull * (r1 - r0);
     }
+    if (nparts > 1) {
+        // rows [n * part / nparts, n * (part + 1) / nparts) of the first
+        // table; the closed-form first step's bytes go with piece 0
+        const uint64_t n = st.rows;
+        const uint64_t lo = (uint64_t)((unsigned __int128)n * part / nparts);
+        const uint64_t hi = (uint64_t)((unsigned __int128)n * (part + 1) / nparts);
+        if (st.width == 1) {
+            d.start_off += (uint32_t)lo;
+            d.start_n = (uint32_t)(hi - lo);
+        } else {
+            d.init_rows += 2 * lo;
+        }
+        st.rows = hi - lo;
+        if (part != 0) st.bytes = 0;
+        d.piece = 1;
+    }
     st.serial = b->next_serial++;
     d.init_width = st.width;
     d.init_n = st.rows;
     d.init_bytes = st.bytes;
     d.leader = kNoSlot;
-    if (b->share) {
+    if (b->share && !d.piece) {
         uint32_t p;
         const uint32_t w = pick_leader(b, s, d, p);
         if (w) {

@@ -5,6 +5,7 @@
 #include <chrono>
 #include <deque>
 #include <mutex>
+#include <tuple>
 
 namespace geneva {
 
@@ -76,13 +77,15 @@ std::string BatchMiner::classKey(const CompiledQuery& cq) {
     return "k" + std::to_string(cq.q.k) + "c" + std::to_string(cyc) + (cq.hasLeaves ? "l" : "");
 }
 
-int BatchMiner::admit(const CompiledQuery& cq, uint64_t tag, uint32_t* slot) {
+int BatchMiner::admit(const CompiledQuery& cq, uint64_t tag, uint32_t* slot, uint32_t part,
+                      uint32_t nparts) {
     uint32_t ipt = 0;
     if (!classIpt_.empty()) {
         auto it = classIpt_.find(classKey(cq));
         if (it != classIpt_.end() && it->second <= opt_.tune.ipt) ipt = it->second;
     }
-    return gv_batch_admit_tuned(b_, &cq.q, cq.hasLeaves ? &cq.leaves : nullptr, ipt, tag, slot);
+    return gv_batch_admit_part(b_, &cq.q, cq.hasLeaves ? &cq.leaves : nullptr, ipt, tag, part, nparts,
+                               slot);
 }
 
 int BatchMiner::run(const std::vector<CompiledQuery>& qs, std::vector<MineResult>& out,
@@ -91,16 +94,64 @@ int BatchMiner::run(const std::vector<CompiledQuery>& qs, std::vector<MineResult
     if (rc != GV_OK) return rc;
     auto w0 = std::chrono::steady_clock::now();
     out.assign(qs.size(), MineResult());
-    std::deque<size_t> pending;
+    std::vector<size_t> order;
     for (size_t i = 0; i < qs.size(); ++i) {
         if (qs[i].empty) {
             out[i].graphBits.assign(nwords_, 0);
             continue;
         }
-        pending.push_back(i);
+        order.push_back(i);
     }
-    std::vector<size_t> slotQuery(opt_.slots, SIZE_MAX);
-    std::vector<size_t> overflow;
+    // queries that start from the same table (start label and first step)
+    // take adjacent slots, whose sizing tiles the engine runs side by side
+    auto startKey = [&](size_t i) {
+        const gv_query_t& q = qs[i].q;
+        const gv_step_t& s1 = q.step[1];
+        return std::make_tuple(q.label0, s1.label, s1.nback ? s1.back_part[0] : 0u, s1.restrict_col, s1.nback);
+    };
+    std::stable_sort(order.begin(), order.end(),
+                     [&](size_t a, size_t b) { return startKey(a) < startKey(b); });
+    // a query whose table outgrows its slot comes back as two pieces of its
+    // first table (each piece that overflows splits again), up to kMaxPieces;
+    // beyond that it is mined on the plain path
+    struct Piece {
+        size_t q;
+        uint32_t part, nparts;
+    };
+    constexpr uint32_t kMaxPieces = 64;
+    auto queryKey = [&](size_t i) {
+        const CompiledQuery& cq = qs[i];
+        std::string k;
+        auto put = [&](uint32_t v) { k.append(reinterpret_cast<const char*>(&v), sizeof(v)); };
+        put(cq.q.k);
+        put(cq.q.label0);
+        for (uint32_t t = 1; t < cq.q.k; ++t) {
+            const gv_step_t& st = cq.q.step[t];
+            put(st.label);
+            put(st.nback);
+            for (uint32_t j = 0; j < st.nback; ++j) put(st.back_col[j] | st.back_part[j] << 8);
+            put(st.restrict_col);
+            put(st.ncoll);
+            for (uint32_t j = 0; j < st.ncoll; ++j) put(st.coll_col[j]);
+        }
+        put(cq.hasLeaves ? cq.leaves.width : 0);
+        return k;
+    };
+    std::vector<uint32_t> maxParts(qs.size(), 1);
+    std::deque<Piece> pending;
+    for (size_t i : order) {
+        auto it = pieceHint_.find(queryKey(i));
+        const uint32_t n = it == pieceHint_.end() ? 1 : it->second;
+        for (uint32_t p = 0; p < n; ++p) pending.push_back({i, p, n});
+        maxParts[i] = n;
+    }
+    std::vector<uint32_t> open(qs.size(), 0);   // pieces not finished
+    std::vector<uint8_t> toPlain(qs.size(), 0);
+    for (size_t i : order) {
+        open[i] = maxParts[i];
+        out[i].graphBits.assign(nwords_, 0);
+    }
+    std::vector<Piece> slotPiece(opt_.slots, Piece{SIZE_MAX, 0, 0});
     std::vector<gv_slot_info_t> info(opt_.slots);
     std::vector<uint32_t> bits;
     BatchStats local;
@@ -110,12 +161,16 @@ int BatchMiner::run(const std::vector<CompiledQuery>& qs, std::vector<MineResult
         uint32_t maxk = 0;
         while (!pending.empty()) {
             uint32_t slot = 0;
-            const CompiledQuery& cq = qs[pending.front()];
-            rc = admit(cq, pending.front(), &slot);
+            const Piece pc = pending.front();
+            if (toPlain[pc.q]) {   // an earlier piece gave up: the plain path mines it whole
+                pending.pop_front();
+                continue;
+            }
+            rc = admit(qs[pc.q], pc.q, &slot, pc.part, pc.nparts);
             if (rc == GV_ERR_CAPACITY) break;
             if (rc != GV_OK) return rc;
-            slotQuery[slot] = pending.front();
-            maxk = std::max(maxk, qs[pending.front()].q.k);
+            slotPiece[slot] = pc;
+            maxk = std::max(maxk, qs[pc.q].q.k);
             pending.pop_front();
             ++running;
         }
@@ -129,7 +184,7 @@ int BatchMiner::run(const std::vector<CompiledQuery>& qs, std::vector<MineResult
         // finished slots: their graph bits in one synchronised copy, then free
         std::vector<uint32_t> done, over;
         for (uint32_t s = 0; s < opt_.slots; ++s) {
-            if (slotQuery[s] == SIZE_MAX) continue;
+            if (slotPiece[s].q == SIZE_MAX) continue;
             if (info[s].state == GV_SLOT_DONE) done.push_back(s);
             else if (info[s].state == GV_SLOT_OVERFLOW) over.push_back(s);
         }
@@ -139,28 +194,46 @@ int BatchMiner::run(const std::vector<CompiledQuery>& qs, std::vector<MineResult
         if (rc != GV_OK) return rc;
         for (size_t i = 0; i < done.size(); ++i) {
             const uint32_t s = done[i];
-            MineResult& r = out[slotQuery[s]];
-            r.count = info[s].count;
-            r.algBytes = info[s].alg_bytes;
-            r.graphBits.assign(bits.begin() + i * nwords_, bits.begin() + (i + 1) * nwords_);
-            slotQuery[s] = SIZE_MAX;
+            const size_t q = slotPiece[s].q;
+            MineResult& r = out[q];
+            r.count += info[s].count;
+            r.algBytes += info[s].alg_bytes;
+            for (uint32_t w = 0; w < nwords_; ++w) r.graphBits[w] |= bits[i * nwords_ + w];
+            slotPiece[s].q = SIZE_MAX;
             --running;
-            local.batched++;
+            if (--open[q] == 0 && !toPlain[q]) local.batched++;
         }
         rc = gv_batch_release_many(b_, (uint32_t)over.size(), over.data(), nullptr, 0);
         if (rc != GV_OK) return rc;
         for (uint32_t s : over) {
-            overflow.push_back(slotQuery[s]);
-            slotQuery[s] = SIZE_MAX;
+            const Piece pc = slotPiece[s];
+            slotPiece[s].q = SIZE_MAX;
             --running;
+            if (!toPlain[pc.q] && pc.nparts < kMaxPieces) {
+                pending.push_front({pc.q, 2 * pc.part + 1, 2 * pc.nparts});
+                pending.push_front({pc.q, 2 * pc.part, 2 * pc.nparts});
+                ++open[pc.q];
+                maxParts[pc.q] = std::max(maxParts[pc.q], 2 * pc.nparts);
+                local.pieces++;
+            } else {
+                toPlain[pc.q] = 1;
+                --open[pc.q];
+            }
         }
+    }
+    std::vector<size_t> overflow;
+    for (size_t i : order) {
+        if (toPlain[i]) overflow.push_back(i);
+        else if (maxParts[i] > 1) pieceHint_[queryKey(i)] = maxParts[i];
     }
     gv_mark(ctx_, 3);
     float ms = 0;
     gv_elapsed_ms(ctx_, 2, 3, &ms);
     local.deviceMs = ms;
+    const auto f0 = std::chrono::steady_clock::now();
     for (size_t qi : overflow) {
         local.overflowed++;
+        out[qi] = MineResult();
         if (!fallback_) {
             out[qi].status = GV_ERR_CAPACITY;
             continue;
@@ -168,6 +241,7 @@ int BatchMiner::run(const std::vector<CompiledQuery>& qs, std::vector<MineResult
         rc = fallback_->mine(qs[qi], out[qi]);
         if (rc != GV_OK) return rc;
     }
+    local.fallbackMs = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - f0).count();
     local.wallMs = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - w0).count();
     if (st) *st = local;
     return GV_OK;

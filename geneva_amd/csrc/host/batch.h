@@ -39,6 +39,8 @@ struct BatchStats {
     uint64_t graphLaunches = 0;
     double deviceMs = 0;        // batch iterations, device time
     double wallMs = 0;
+    double fallbackMs = 0;      // wall time of the plain-path re-runs
+    uint64_t pieces = 0;        // splits of overflowed queries into pieces
 };
 
 class BatchMiner {
@@ -63,7 +65,9 @@ public:
 
     // Place a query in a free slot with its class's candidates per lane
     // (classTune); GV_ERR_CAPACITY when no slot is free.
-    int admit(const CompiledQuery& cq, uint64_t tag, uint32_t* slot);
+    // Admit cq (or piece `part` of `nparts` of its first table) to a free slot.
+    int admit(const CompiledQuery& cq, uint64_t tag, uint32_t* slot, uint32_t part = 0,
+              uint32_t nparts = 1);
 
     // Query class of the per-class tuning: vertices, steps with more than
     // one back edge, trailing leaf groups.
@@ -80,6 +84,9 @@ private:
     gv_batch_t b_ = nullptr;
     uint32_t nwords_ = 0;
     std::map<std::string, uint32_t> classIpt_;
+    // pieces a query needed the last time it outgrew its slot (keyed by its
+    // compiled steps): later runs admit it in that many pieces at once
+    std::map<std::string, uint32_t> pieceHint_;
 };
 
 // Autotuner: times the batch engine under each candidate tuning on a sample of

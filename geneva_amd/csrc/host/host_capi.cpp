@@ -1,6 +1,7 @@
 // extern "C" surface of the host library (include/geneva_host.h).
 #include "../../../include/geneva_host.h"
 
+#include <chrono>
 #include <cstring>
 #include <memory>
 #include <sstream>
@@ -460,8 +461,11 @@ int gh_engine_batch_mine(gh_engine_t e, const char* const* queries, uint32_t n, 
                          double* stats) {
     if (!e || (n && !queries)) return GV_ERR_ARG;
     std::vector<geneva::CompiledQuery> qs;
+    const auto c0 = std::chrono::steady_clock::now();
     int rc = compileAll(e, queries, n, single_v, qs);
     if (rc != GV_OK) return rc;
+    const double compileMs =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - c0).count();
     rc = ensureBatch(e);
     if (rc != GV_OK) {
         setErr(std::string("batch engine: ") + gv_status_string(rc) + " " + gv_last_error());
@@ -491,6 +495,9 @@ int gh_engine_batch_mine(gh_engine_t e, const char* const* queries, uint32_t n, 
         stats[2] = (double)st.graphLaunches;
         stats[3] = st.deviceMs;
         stats[4] = st.wallMs;
+        stats[5] = compileMs;
+        stats[6] = st.fallbackMs;
+        stats[7] = (double)st.pieces;
     }
     return GV_OK;
 }

@@ -293,6 +293,14 @@ int gv_batch_admit_leaves(gv_batch_t b, const gv_query_t* q, const gv_leaves_t* 
  * the per-query-class tuning of the wavefront split. */
 int gv_batch_admit_tuned(gv_batch_t b, const gv_query_t* q, const gv_leaves_t* leaves, uint32_t ipt,
                          uint64_t tag, uint32_t* slot);
+/* As gv_batch_admit_tuned, for piece `part` of `nparts` of the query: the
+ * slot starts from rows [n * part / nparts, n * (part + 1) / nparts) of the
+ * query's first table (its start vertices, or its label-pair edge rows), so
+ * the counts, algorithmic bytes and graph bitsets of the nparts pieces add up
+ * (bitsets by union) to the whole query's.  Pieces never share tables.  For
+ * a query too large for one slot. */
+int gv_batch_admit_part(gv_batch_t b, const gv_query_t* q, const gv_leaves_t* leaves, uint32_t ipt,
+                        uint64_t tag, uint32_t part, uint32_t nparts, uint32_t* slot);
 /* Launch `iterations` iterations (asynchronous). */
 int gv_batch_run(gv_batch_t b, uint32_t iterations);
 /* Copy every slot's state to info[0..slots) (synchronises). */

@@ -120,8 +120,11 @@ int gh_engine_count_by_rows(gh_engine_t e, int on);
 int gh_engine_batch_config(gh_engine_t e, uint32_t slots, uint64_t slot_bytes, uint64_t row_cap,
                            uint32_t ipt, uint32_t blocks_per_cu);
 /* Mine n queries together.  counts/alg_bytes: n entries; graph_bits: n*nwords
- * (nullable); stats (nullable, 5 doubles): batched, overflowed (re-run on the
- * plain path), graph launches, device ms of the batch iterations, wall ms. */
+ * (nullable); stats (nullable, 8 doubles): batched, overflowed (re-run on the
+ * plain path), graph launches, device ms of the batch iterations, wall ms of
+ * the batch run, ms spent planning the queries, wall ms of the plain-path
+ * re-runs, splits of queries too large for a slot into pieces of their first
+ * table (gv_batch_admit_part). */
 int gh_engine_batch_mine(gh_engine_t e, const char* const* queries, uint32_t n, int single_v,
                          uint64_t* counts, uint64_t* alg_bytes, uint32_t* graph_bits,
                          uint32_t nwords, double* stats);

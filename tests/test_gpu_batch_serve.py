@@ -89,9 +89,28 @@ def test_overflow_falls_back_to_plain_path(require_gpu):
     eng.batch_config(slots=8, slot_bytes=1 << 16, row_cap=1 << 20)   # tiny slot tables
     res, st = eng.mine_batch(ds.queries)
     print("stats", st, flush=True)
-    assert st["overflowed"] >= 1
+    assert st["overflowed"] + st["pieces"] >= 1
     for q, r in zip(ds.queries, res):
-        assert r.count == og.mine(q, threads=4).count
+        o = og.mine(q, threads=4)
+        assert r.count == o.count
+        assert r.alg_bytes == o.alg_bytes
+
+
+def test_pieces_of_large_queries_add_up(w3):
+    """Queries whose tables outgrow small slots come back as pieces of their
+    first table; the pieces' counts, bytes and graph ids add up to the
+    fixture's."""
+    ds, eng, cases = w3
+    qs = [c["query"] for c in cases]
+    eng.batch_config(slots=64, slot_bytes=1 << 18, row_cap=1 << 22)
+    res, st = eng.mine_batch(qs)
+    print("stats", st, flush=True)
+    assert st["pieces"] > 0
+    for c, r in zip(cases, res):
+        assert r.count == c["count"]
+        assert r.alg_bytes == c["alg_bytes"]
+        assert sorted(r.graph_ids) == c["graph_ids"]
+    eng.batch_config(slots=64, slot_bytes=8 << 20, row_cap=1 << 22)
 
 
 def test_row_cap_defers_slots_without_changing_results(w3):
