@@ -850,12 +850,22 @@ uint32_t counted_width(const BDesc& d) {
     return w;
 }
 
+// Widest table a follower can take from a leader: as counted_width, except
+// that a single-edge tail counted with the step before it can instead be
+// counted on its own from the table of width k - 1 (the same count and
+// algorithmic bytes).
+uint32_t follow_width(const BDesc& d) {
+    uint32_t w = d.q.k - 1;
+    if (d.has_leaves) w = std::min(w, d.leaves.width);
+    return w;
+}
+
 // The live slot whose table of the widest width d >= 3 equals this query's
-// own table of width d (same start label and steps 1 .. d-1, both extending
-// from width d); 0 = none.  A leader that itself waits to share a table at
-// least as wide is replaced by its own leader.
+// own table of width d (same start label and steps 1 .. d-1; the leader
+// extends from width d); 0 = none.  A leader that itself waits to share a
+// table at least as wide is replaced by its own leader.
 uint32_t pick_leader(gv_batch* b, uint32_t self, const BDesc& d, uint32_t& leader) {
-    const uint32_t cwf = counted_width(d);
+    const uint32_t cwf = follow_width(d);
     uint32_t best = 0;
     leader = kNoSlot;
     for (uint32_t p = 0; p < b->nslots; ++p) {

@@ -336,7 +336,8 @@ int gv_batch_profile_bytes(gv_batch_t b, uint64_t* size_bytes, uint64_t* walk_by
 int gv_batch_pair_start(gv_batch_t b, int on);
 /* Prefix sharing (on by default): a query admitted while a running slot's
  * query has the same start label and the same first steps up to a table of
- * width d >= 3 (both extending from that width) waits for that slot's table
+ * width d >= 3 (the leader extending from that width, the follower extending
+ * from it or counting its last step on it) waits for that slot's table
  * of width d and extends it, in the iteration the leader does, instead of
  * building its own copy; it takes over the leader's counters at that point
  * (the shared steps' algorithmic bytes).  A follower whose leader ran empty
