@@ -103,13 +103,14 @@ PMC_KINDS = [("k_b_walk<1>", "k_b_walk"), ("k_b_walk<2>", "k_b_walk"), ("k_b_wal
              ("k_isect_big", "k_isect")]
 
 
-def pmc_summary(paths, workload, cal_mib):
+def pmc_summary(paths, workload, cal_mib, last=0):
     """HBM bytes per launch per kernel kind from two rocprofv3 --pmc runs of the
     bench (FETCH_SIZE, then WRITE_SIZE; KiB per dispatch) and the same two
     counters over k_copy_back moving a known number of bytes each way
     (--copy-calibration cal_mib): each counter is scaled by bytes moved / KiB
-    read for the copy.  Writes profiles/traffic_workload<N>.json and prints the
-    table."""
+    read for the copy.  last > 0: only the last `last` dispatches of each
+    kernel (the bench's accounting run).  Writes
+    profiles/traffic_workload<N>.json and prints the table."""
     import csv
     import re
     fetch_csv, write_csv, cal_fetch_csv, cal_write_csv = paths
@@ -132,8 +133,21 @@ def pmc_summary(paths, workload, cal_mib):
     fetch_scale = moved / (mean(per_kernel(cal_fetch_csv, "FETCH_SIZE")["k_copy_back"]) * 1024)
     write_scale = moved / (mean(per_kernel(cal_write_csv, "WRITE_SIZE")["k_copy_back"]) * 1024)
     f, w = per_kernel(fetch_csv, "FETCH_SIZE"), per_kernel(write_csv, "WRITE_SIZE")
+    if last:
+        # the accounting run: every dispatch from its first iteration's k_b_prep on
+        def keep_last(acc, path):
+            preps = []
+            for r in csv.DictReader(open(path)):
+                if "k_b_prep" in r["Kernel_Name"]:
+                    preps.append(int(r["Dispatch_Id"]))
+            preps = sorted(set(preps))
+            start = preps[-last] if len(preps) >= last else 0
+            out = {k: {i: v for i, v in d.items() if int(i) >= start} for k, d in acc.items()}
+            return {k: d for k, d in out.items() if d}
+        f, w = keep_last(f, fetch_csv), keep_last(w, write_csv)
     out = {"fetch_scale": round(fetch_scale, 4), "write_scale": round(write_scale, 4),
-           "calibration": f"k_copy_back: {moved} B read + {moved} B written per launch"}
+           "calibration": f"k_copy_back: {moved} B read + {moved} B written per launch",
+           "dispatches": f"from the last {last} batch iterations on" if last else "all"}
     kinds = {}
     for kern, kind in PMC_KINDS:
         if kern not in f or kern not in w:
@@ -524,6 +538,8 @@ def main():
                          "--copy-calibration into profiles/traffic_workload<N>.json (no GPU)")
     ap.add_argument("--workload", type=int, default=3, help="workload of --pmc-summary")
     ap.add_argument("--cal-mib", type=int, default=512, help="MIB of the calibration run (--pmc-summary)")
+    ap.add_argument("--pmc-last", type=int, default=0,
+                    help="--pmc-summary over the dispatches of the last N batch iterations (the accounting run)")
     ap.add_argument("--pmc-table", metavar="CSV", help="only print per-kernel means of a rocprofv3 --pmc CSV")
     ap.add_argument("--only", choices=("w2", "w4"),
                     help="only run the workload-2 or workload-4 section and print its JSON (profiling runs)")
@@ -531,7 +547,7 @@ def main():
     if args.copy_calibration:
         return copy_calibration(args.copy_calibration)
     if args.pmc_summary:
-        return pmc_summary(args.pmc_summary, args.workload, args.cal_mib)
+        return pmc_summary(args.pmc_summary, args.workload, args.cal_mib, args.pmc_last)
     if args.pmc_table:
         return pmc_table(args.pmc_table)
     if args.gpus != 1:
