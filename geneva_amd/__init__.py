@@ -94,6 +94,7 @@ def lib():
         "gh_engine_batch_profile": (i32, [vp, i32]),
         "gh_engine_batch_profile_bytes": (i32, [vp, C.POINTER(u64), C.POINTER(u64)]),
         "gh_engine_batch_pair_start": (i32, [vp, i32]),
+        "gh_engine_batch_row_cap": (i32, [vp, C.POINTER(u64)]),
         "gh_engine_batch_share": (i32, [vp, i32]),
         "gh_engine_batch_share_stats": (i32, [vp, C.POINTER(u64), C.POINTER(u64)]),
         "gh_engine_batch_history": (i32, [vp, C.POINTER(u64), C.POINTER(u64), u32, C.POINTER(u32)]),
@@ -396,7 +397,10 @@ class Engine:
             if ent:
                 k, i, m = ent.split(":")
                 classes[k] = {"ipt": int(i), "device_ms": round(float(m), 4)}
-        return {"ipt": ipt.value, "blocks_per_cu": bpc.value, "device_ms": ms.value, "classes": classes}
+        cap = C.c_uint64(0)
+        lib().gh_engine_batch_row_cap(self._h, C.byref(cap))
+        return {"ipt": ipt.value, "blocks_per_cu": bpc.value, "row_cap": cap.value, "device_ms": ms.value,
+                "classes": classes}
 
     def serve(self, queries, arrival_s, batched=True, mem_budget=16 << 30, with_graphs=False):
         """Serve requests arriving at arrival_s (seconds).  Returns (latency_ms,

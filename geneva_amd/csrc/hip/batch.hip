@@ -678,6 +678,7 @@ struct gv_batch {
     uint32_t nslots = 0;
     uint64_t slot_bytes = 0;
     uint64_t row_cap = 0;
+    uint64_t row_cap_alloc = 0;         // rows the sizing arrays were made for
     gv_tune_t tune{4, 16};
     BDesc* d_desc = nullptr;
     BState* d_st = nullptr;
@@ -914,6 +915,7 @@ int gv_batch_create(gv_ctx_t c, gv_graph_t g, uint32_t slots, uint64_t slot_byte
     b->nslots = slots;
     b->slot_bytes = slot_bytes & ~uint64_t(63);
     b->row_cap = row_cap;
+    b->row_cap_alloc = row_cap;
     if (tune) b->tune = *tune;
     b->used.assign(slots, 0);
     b->dirty.assign(slots, 0);
@@ -982,6 +984,14 @@ int gv_batch_set_tune(gv_batch_t b, const gv_tune_t* t) {
     int rc = sync(b);
     if (rc != GV_OK) return rc;
     b->tune = *t;
+    return capture(b);
+}
+
+int gv_batch_set_row_cap(gv_batch_t b, uint64_t row_cap) {
+    if (!b || row_cap == 0 || row_cap > b->row_cap_alloc) return GV_ERR_ARG;
+    int rc = sync(b);
+    if (rc != GV_OK) return rc;
+    b->row_cap = row_cap;
     return capture(b);
 }
 

@@ -65,6 +65,12 @@ int BatchMiner::init() {
     return gv_batch_create(ctx_, g_, opt_.slots, opt_.slotBytes, opt_.rowCap, &opt_.tune, &b_);
 }
 
+int BatchMiner::setRowCap(uint64_t rows) {
+    int rc = gv_batch_set_row_cap(b_, rows);
+    if (rc == GV_OK) rowCap_ = rows;
+    return rc;
+}
+
 int BatchMiner::setTune(const gv_tune_t& t) {
     int rc = gv_batch_set_tune(b_, &t);
     if (rc == GV_OK) opt_.tune = t;
@@ -256,6 +262,7 @@ std::mutex& tuneMutex() {
 }
 struct CachedTune {
     gv_tune_t tune;
+    uint64_t rowCap;
     std::map<std::string, uint32_t> classes;
 };
 std::map<std::string, CachedTune>& tuneCache() {
@@ -294,12 +301,16 @@ int autotune(BatchMiner& bm, const std::vector<CompiledQuery>& sample, TuneResul
         auto it = tuneCache().find(key);
         if (it != tuneCache().end()) {
             out.tune = it->second.tune;
+            out.rowCap = it->second.rowCap;
             bm.classTune() = it->second.classes;
             for (auto& kv : it->second.classes) out.classes[kv.first] = {kv.second, 0.0};
-            return bm.setTune(out.tune);
+            int rc = bm.setTune(out.tune);
+            return rc != GV_OK ? rc : bm.setRowCap(out.rowCap);
         }
     }
     bm.classTune().clear();
+    int rc0 = bm.setRowCap(bm.options().rowCap);
+    if (rc0 != GV_OK) return rc0;
     const uint32_t ipts[] = {1, 2, 4, 8};
     const uint32_t bpcs[] = {4, 8, 16, 32};
     double best = 1e300;
@@ -324,6 +335,24 @@ int autotune(BatchMiner& bm, const std::vector<CompiledQuery>& sample, TuneResul
     }
     out.bestMs = best;
     int rc = bm.setTune(out.tune);
+    if (rc != GV_OK) return rc;
+    // rows sized per iteration: fewer rows run fewer slots together
+    const uint64_t full = bm.options().rowCap;
+    out.rowCap = full;
+    for (uint64_t cap : {full / 4, full / 16}) {
+        if (cap < (1u << 20)) break;
+        rc = bm.setRowCap(cap);
+        if (rc != GV_OK) return rc;
+        BatchStats st;
+        rc = bm.run(sample, res, nullptr);
+        if (rc == GV_OK) rc = bm.run(sample, res, &st);
+        if (rc != GV_OK) return rc;
+        if (st.deviceMs < out.bestMs) {
+            out.bestMs = st.deviceMs;
+            out.rowCap = cap;
+        }
+    }
+    rc = bm.setRowCap(out.rowCap);
     if (rc != GV_OK) return rc;
     // per class: candidates per lane up to the chosen kernel's
     std::map<std::string, std::vector<CompiledQuery>> byClass;
@@ -350,7 +379,7 @@ int autotune(BatchMiner& bm, const std::vector<CompiledQuery>& sample, TuneResul
     }
     {
         std::lock_guard<std::mutex> l(tuneMutex());
-        tuneCache()[key] = CachedTune{out.tune, bm.classTune()};
+        tuneCache()[key] = CachedTune{out.tune, out.rowCap, bm.classTune()};
     }
     return GV_OK;
 }

@@ -52,6 +52,9 @@ public:
 
     int init();
     int setTune(const gv_tune_t& t);
+    // Rows sized per iteration (at most options().rowCap, the allocation).
+    int setRowCap(uint64_t rows);
+    uint64_t rowCap() const { return rowCap_ ? rowCap_ : opt_.rowCap; }
     const BatchOptions& options() const { return opt_; }
     gv_batch_t handle() const { return b_; }
     gv_graph_t graph() const { return g_; }
@@ -63,9 +66,9 @@ public:
     // get status GV_ERR_ARG; overflowing ones are re-run on the plain path.
     int run(const std::vector<CompiledQuery>& qs, std::vector<MineResult>& out, BatchStats* st);
 
-    // Place a query in a free slot with its class's candidates per lane
-    // (classTune); GV_ERR_CAPACITY when no slot is free.
-    // Admit cq (or piece `part` of `nparts` of its first table) to a free slot.
+    // Place a query (or piece `part` of `nparts` of its first table) in a
+    // free slot with its class's candidates per lane (classTune);
+    // GV_ERR_CAPACITY when no slot is free.
     int admit(const CompiledQuery& cq, uint64_t tag, uint32_t* slot, uint32_t part = 0,
               uint32_t nparts = 1);
 
@@ -83,6 +86,7 @@ private:
     BatchOptions opt_;
     gv_batch_t b_ = nullptr;
     uint32_t nwords_ = 0;
+    uint64_t rowCap_ = 0;
     std::map<std::string, uint32_t> classIpt_;
     // pieces a query needed the last time it outgrew its slot (keyed by its
     // compiled steps): later runs admit it in that many pieces at once
@@ -91,13 +95,15 @@ private:
 
 // Autotuner: times the batch engine under each candidate tuning on a sample of
 // queries and keeps the fastest (candidates per lane of the walk kernel x its
-// blocks per CU), then, for each query class of the sample with at least four
-// queries, times that class alone under each candidates-per-lane value up to
-// the chosen one and keeps the fastest for the class (slots of the class run
-// their walk tiles at that size).  Remembered per graph, engine options and
-// sample shape.
+// blocks per CU), then the rows sized per iteration (the allocation, a
+// quarter or a sixteenth of it: how many slots run together), then, for each
+// query class of the sample with at least four queries, times that class
+// alone under each candidates-per-lane value up to the chosen one and keeps
+// the fastest for the class (slots of the class run their walk tiles at that
+// size).  Remembered per graph, engine options and sample shape.
 struct TuneResult {
     gv_tune_t tune{4, 16};
+    uint64_t rowCap = 0;
     double bestMs = 0;
     std::vector<std::pair<gv_tune_t, double>> tried;
     // per query class: the candidates per lane chosen and its device ms

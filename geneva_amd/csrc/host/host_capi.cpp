@@ -524,6 +524,12 @@ int gh_engine_autotune(gh_engine_t e, const char* const* queries, uint32_t n, ui
     return GV_OK;
 }
 
+int gh_engine_batch_row_cap(gh_engine_t e, uint64_t* rows) {
+    if (!e || !rows) return GV_ERR_ARG;
+    *rows = e->batch ? e->batch->rowCap() : e->batchOpt.rowCap;
+    return GV_OK;
+}
+
 uint64_t gh_engine_autotune_classes(gh_engine_t e, char* out, uint64_t cap) {
     if (!e) return 0;
     const std::string& t = e->tuneTable;

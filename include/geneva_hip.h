@@ -279,6 +279,10 @@ typedef struct {
 int gv_batch_create(gv_ctx_t ctx, gv_graph_t g, uint32_t slots, uint64_t slot_bytes,
                     uint64_t row_cap, const gv_tune_t* tune, gv_batch_t* out);
 int gv_batch_destroy(gv_batch_t b);
+/* Rows sized per iteration from now on (at most the row_cap the batch was
+ * created with; re-captures the iteration).  Slots whose table alone exceeds
+ * it stop with GV_SLOT_OVERFLOW. */
+int gv_batch_set_row_cap(gv_batch_t b, uint64_t row_cap);
 /* Re-capture the iteration with other tuning parameters. */
 int gv_batch_set_tune(gv_batch_t b, const gv_tune_t* tune);
 /* Place a compiled query in a free slot (stream-ordered; effective at the next

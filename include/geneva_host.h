@@ -132,6 +132,8 @@ int gh_engine_batch_mine(gh_engine_t e, const char* const* queries, uint32_t n, 
  * lane and blocks per CU and its device ms. */
 int gh_engine_autotune(gh_engine_t e, const char* const* queries, uint32_t n, uint32_t* ipt,
                        uint32_t* blocks_per_cu, double* best_ms);
+/* Rows the batch engine sizes per iteration (after autotune: its choice). */
+int gh_engine_batch_row_cap(gh_engine_t e, uint64_t* rows);
 /* The per-query-class choices of the last autotune as text, one
  * "class:candidates per lane:device ms;" entry per class (class = k<vertices>
  * c<steps with several back edges>[l if leaf groups]).  Returns its length. */
