@@ -2,8 +2,8 @@
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
 
-Headline (BASELINE metric "embeddings/s and queries/s (p50/p99 latency) vs
-plain GPU"): workload 3, a database of 100,000 labelled graphs (n ~ U[20,200],
+Headline (metric "embeddings/s and queries/s (p50/p99 latency) vs plain
+GPU"): workload 3, a database of 100,000 labelled graphs (n ~ U[20,200],
 Erdos-Renyi / Barabasi-Albert, 32 labels Zipf(1.1)) resident in HBM, with 256
 random-walk queries of 4-8 vertices.  A step is one run of the batched engine
 over all 256 queries (captured iteration graph, device work queue, tuning
@@ -417,6 +417,21 @@ def section_w4(args):
             rec["whole_path_device_ms"] = round(whole.ms, 3)
         per.append(rec)
         log(f"workload4 query {i}: {rec}")
+    # the configuration's walk queries as they come (pendant vertices on hubs
+    # included): the same generator without the two-neighbour condition
+    uq = ds.add_walk_queries(64, 6, 8, induced=True, min_extra=2, seed=configs.seed(4) + 100,
+                             min_deg=0).queries[-64:]
+    uper = []
+    udeadline = time.perf_counter() + args.w4u_budget
+    for i, q in enumerate(uq):
+        eng.time_limit(min(args.w4u_cap, max(0.2, udeadline - time.perf_counter())))
+        r = eng.mine(q)
+        fin, frac = eng.last_progress()
+        uper.append({"query": i, "k": sum(1 for l in q.splitlines() if l.startswith("v ")), "finished": fin,
+                     "fraction_done": round(frac, 6), "count": r.count, "device_ms": round(r.ms, 3)})
+    eng.time_limit(0)
+    ufin = [p for p in uper if p["finished"]]
+    log(f"workload 4 unconstrained: {len(ufin)} of {len(uper)} finished")
     # the oracle's full-size shards (8 start vertices each)
     spath = os.path.join(ROOT, "tests", "golden", "workload4_shards.json")
     shard_ok = shard_n = 0
@@ -436,7 +451,16 @@ def section_w4(args):
            "finished_device_ms": round(dev, 3),
            "embeddings_per_s_finished": round(sum(p["count"] for p in fin) / max(dev, 1e-9) * 1e3, 1),
            "oracle_shards": {"checked": shard_n, "bit_exact": shard_ok},
-           "per_query": per}
+           "per_query": per,
+           "unconstrained": {"queries": "the same generator without the two-neighbour condition "
+                                        "(pendant vertices on hubs included)",
+                             "query_cap_s": args.w4u_cap, "budget_s": args.w4u_budget,
+                             "finished": len(ufin), "queries_run": len(uper),
+                             "finished_embeddings": sum(p["count"] for p in ufin),
+                             "finished_device_ms": round(sum(p["device_ms"] for p in ufin), 3),
+                             "parity": "scaled: tests/test_gpu_workload4.py against "
+                                       "workload4_unconstrained_scaled.json",
+                             "per_query": uper}}
     eng.close()
     return ds, out
 
@@ -528,6 +552,9 @@ def main():
     ap.add_argument("--no-w4", action="store_true")
     ap.add_argument("--w4-query-cap", type=float, default=2.0, help="seconds per workload-4 query")
     ap.add_argument("--w4-budget", type=float, default=100.0, help="seconds for all workload-4 queries")
+    ap.add_argument("--w4u-cap", type=float, default=1.0, help="seconds per unconstrained workload-4 query")
+    ap.add_argument("--w4u-budget", type=float, default=40.0,
+                    help="seconds for all unconstrained workload-4 queries")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--cpu-replay", type=int, default=150, help="workload-5 requests replayed on the CPU")
     ap.add_argument("--no-cpu", action="store_true")

@@ -45,12 +45,15 @@ def workload3(scale=1.0, nqueries=256):
     return ds.add_walk_queries(nqueries, 4, 8, seed=seed(3) + 100)
 
 
-def workload4(scale=1.0, nqueries=64):
+def workload4(scale=1.0, nqueries=64, min_deg=2):
+    """min_deg: every query vertex has at least this many query neighbours (2:
+    every vertex on a cycle; 0: the configuration's walk queries as they come,
+    pendant vertices included)."""
     nv = max(2000, int(4_000_000 * scale))
     wmax = max(50.0, 1e5 * scale)
     ds = Dataset.chung_lu(nv, 1.9, 30.0, wmax, 64, 1.0, seed(4))
     return ds.add_walk_queries(nqueries, 6, 8, induced=True, min_extra=2, seed=seed(4) + 100,
-                               min_deg=2)
+                               min_deg=min_deg)
 
 
 def workload5_queries(ds, n, seed_offset=0):

@@ -8,10 +8,12 @@
                                                   # vertices) over the scaled workload-3 database
     python tests/golden/make_golden.py --only-w3full  # workload 3 at full size (100,000 graphs):
                                                       # counts, bytes and matched graph ids
+    python tests/golden/make_golden.py --only-w4u  # workload 4's walk queries without the
+                                                   # two-neighbour condition, scaled
 
 Reference values come from the reference's own cpumine, compiled from its
 sources into oracle/_ref/cpumine (`make -C oracle ref`), run on files written
-by this project's format writers (prefix-free .reorder); oracle
+by this project's format writers (.reorder files without skip-index entries); oracle
 values come from oracle/liboracle.so.  Only data is written: query texts,
 plan dumps, counts, matched graph ids and algorithmic byte totals.
 """
@@ -196,6 +198,21 @@ def workload4_scaled(scale=0.01, nq=24):
               open(os.path.join(HERE, "workload4_scaled.json"), "w"), indent=1)
 
 
+def workload4_unconstrained(scale=0.01, nq=24):
+    """Workload 4's induced walk queries as they come (pendant vertices
+    included, min_deg=0) on the scaled graph: oracle counts and algorithmic
+    bytes."""
+    ds = configs.workload4(scale=scale, nqueries=nq, min_deg=0)
+    ogw = O.OracleGraph.of(ds)
+    cases = []
+    for qt in ds.queries:
+        r = ogw.mine(qt, threads=os.cpu_count())
+        cases.append({"query": qt, "count": r.count, "alg_bytes": r.alg_bytes})
+        print("workload4 unconstrained:", r.count, round(r.seconds, 2), flush=True)
+    json.dump({"workload": 4, "scale": scale, "kw": {"nqueries": nq, "min_deg": 0}, "cases": cases},
+              open(os.path.join(HERE, "workload4_unconstrained_scaled.json"), "w"), indent=1)
+
+
 def workload5(scale=0.01, nq=64):
     """The serving stream's queries (random walks of 3-10 vertices, the
     workload-5 generator) over workload 3's database at `scale`: oracle counts,
@@ -246,6 +263,8 @@ def main():
         return workload3_full()
     if "--only-w5" in sys.argv:
         return workload5()
+    if "--only-w4u" in sys.argv:
+        return workload4_unconstrained()
     if "--only-full2" in sys.argv:
         return full_workload2()
     if "--only-w4" in sys.argv:

@@ -35,6 +35,28 @@ def test_workload4_scaled_whole_queries(require_gpu):
     eng.close()
 
 
+def test_workload4_unconstrained_scaled_queries(require_gpu):
+    """The configuration's walk queries as they come (pendant vertices on
+    hubs included, no two-neighbour condition), scaled: plain path and batch
+    engine against the oracle's counts and bytes."""
+    gold = _load("workload4_unconstrained_scaled.json")
+    ds = configs.workload4(scale=gold["scale"], **gold["kw"])
+    eng = G.Engine(ds)
+    qs = [c["query"] for c in gold["cases"]]
+    for c in gold["cases"]:
+        print("workload4 unconstrained query", c["query"].replace("\n", " "), flush=True)
+        r = eng.mine(c["query"])
+        assert r.count == c["count"]
+        assert r.alg_bytes == c["alg_bytes"]
+    print("workload4 unconstrained batch", flush=True)
+    res, st = eng.mine_batch(qs)
+    print("stats", st, flush=True)
+    for c, r in zip(gold["cases"], res):
+        assert r.count == c["count"]
+        assert r.alg_bytes == c["alg_bytes"]
+    eng.close()
+
+
 def test_workload4_shards_partition_the_count(require_gpu):
     gold = _load("workload4_scaled.json")
     ds = configs.workload4(scale=gold["scale"], **gold["kw"])
