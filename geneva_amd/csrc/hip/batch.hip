@@ -665,9 +665,41 @@ __global__ __launch_bounds__(1024) void k_b_post(BArgs a) {
     S.mode = 0;
 }
 
-__global__ void k_b_fill(uint32_t* p, uint64_t n) {
+// Slots named to a kernel by value (up to kMaxSlots), so no host staging
+// buffer is read by a copy in flight.
+struct SlotList {
+    uint32_t n;
+    uint16_t s[kMaxSlots];
+};
+constexpr uint32_t kStateWords = sizeof(BState) / 4;
+static_assert(sizeof(BState) % 4 == 0, "BState is copied word by word");
+
+unsigned list_blocks(uint64_t work) { return (unsigned)std::max<uint64_t>(1, std::min<uint64_t>((work + 255) / 256, 1024)); }
+
+// Admitted slots: their staged states copied in and their graph bitsets
+// cleared.
+__global__ void k_b_admit(BState* st, const BState* stage, uint32_t* gbits, uint32_t gwords, SlotList L) {
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = 0;
+    const uint64_t t0 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (uint64_t i = t0; i < (uint64_t)L.n * gwords; i += stride)
+        gbits[(uint64_t)L.s[i / gwords] * gwords + i % gwords] = 0;
+    for (uint64_t i = t0; i < (uint64_t)L.n * kStateWords; i += stride) {
+        const uint32_t s = L.s[i / kStateWords];
+        reinterpret_cast<uint32_t*>(st + s)[i % kStateWords] =
+            reinterpret_cast<const uint32_t*>(stage + s)[i % kStateWords];
+    }
+}
+
+// Released slots: their graph bitsets gathered into out[k * gwords ..] (k =
+// position in the list; out nullable) and their states zeroed (free).
+__global__ void k_b_release(BState* st, const uint32_t* gbits, uint32_t gwords, uint32_t* out, SlotList L) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    const uint64_t t0 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (out)
+        for (uint64_t i = t0; i < (uint64_t)L.n * gwords; i += stride)
+            out[i] = gbits[(uint64_t)L.s[i / gwords] * gwords + i % gwords];
+    for (uint64_t i = t0; i < (uint64_t)L.n * kStateWords; i += stride)
+        reinterpret_cast<uint32_t*>(st + L.s[i / kStateWords])[i % kStateWords] = 0u;
 }
 
 }  // namespace
@@ -701,7 +733,8 @@ struct gv_batch {
     uint64_t epoch = 1;                 // advanced by every synchronisation here
     BDesc* h_desc = nullptr;            // pinned staging, one entry per slot
     BState* h_st = nullptr;
-    BState* h_zero = nullptr;           // pinned, all zero: the state of a released slot
+    BState* d_st_stage = nullptr;       // staged states on their way to d_st (flush)
+    uint32_t* d_bits_out = nullptr;     // released slots' graph bitsets, gathered
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     unsigned long long* d_kb = nullptr;   // per-kernel B (profiling), 2 words
@@ -801,34 +834,50 @@ int sync(gv_batch* b) {
     return GV_OK;
 }
 
-// Copy the staged descriptors and states of dirty slots, one copy per run of
-// consecutive dirty slots, and clear their graph bitsets.
+// Copy the staged descriptors and states of the dirty slots and clear their
+// graph bitsets: the staging entries of the slots lo..hi spanning them go over
+// in two copies (descriptors straight to the device table: the device never
+// writes it, and a slot's staged descriptor is what it last received; states
+// to a device staging table), then one kernel applies the dirty slots' states.
 int flush(gv_batch* b) {
     gv_ctx* c = b->ctx;
-    uint32_t s = 0;
-    while (s < b->nslots) {
-        if (!b->dirty[s]) {
-            ++s;
-            continue;
-        }
-        uint32_t e = s;
-        while (e < b->nslots && b->dirty[e]) ++e;
-        const uint32_t n = e - s;
-        HIPCHK(hipMemcpyAsync(b->d_desc + s, b->h_desc + s, sizeof(BDesc) * n, hipMemcpyHostToDevice,
-                              c->stream));
-        HIPCHK(hipMemcpyAsync(b->d_st + s, b->h_st + s, sizeof(BState) * n, hipMemcpyHostToDevice,
-                              c->stream));
-        const uint64_t words = (uint64_t)b->gwords * n;
-        hipLaunchKernelGGL(k_b_fill, dim3((unsigned)std::min<uint64_t>((words + 255) / 256, 1024)),
-                           dim3(256), 0, c->stream, b->d_gbits + (uint64_t)s * b->gwords, words);
-        HIPCHK(hipGetLastError());
-        for (uint32_t i = s; i < e; ++i) {
-            b->dirty[i] = 0;
-            b->flush_epoch[i] = b->epoch;
-        }
-        s = e;
+    SlotList L;
+    L.n = 0;
+    uint32_t lo = b->nslots, hi = 0;
+    for (uint32_t s = 0; s < b->nslots; ++s) {
+        if (!b->dirty[s]) continue;
+        L.s[L.n++] = (uint16_t)s;
+        lo = std::min(lo, s);
+        hi = s;
+    }
+    if (L.n == 0) return GV_OK;
+    const uint32_t n = hi - lo + 1;
+    HIPCHK(hipMemcpyAsync(b->d_desc + lo, b->h_desc + lo, sizeof(BDesc) * n, hipMemcpyHostToDevice,
+                          c->stream));
+    HIPCHK(hipMemcpyAsync(b->d_st_stage + lo, b->h_st + lo, sizeof(BState) * n, hipMemcpyHostToDevice,
+                          c->stream));
+    hipLaunchKernelGGL(k_b_admit, dim3(list_blocks((uint64_t)L.n * std::max(b->gwords, kStateWords))),
+                       dim3(256), 0, c->stream, b->d_st, b->d_st_stage, b->d_gbits, b->gwords, L);
+    HIPCHK(hipGetLastError());
+    for (uint32_t s = lo; s <= hi; ++s) {
+        b->dirty[s] = 0;
+        b->flush_epoch[s] = b->epoch;
     }
     return GV_OK;
+}
+
+// Free the listed slots on the device (states zeroed); with bits, their
+// graph bitsets are copied to bits[k * gwords ..] (synchronises).
+int release_list(gv_batch* b, const SlotList& L, uint32_t* bits) {
+    if (L.n == 0) return GV_OK;
+    hipLaunchKernelGGL(k_b_release, dim3(list_blocks((uint64_t)L.n * std::max(b->gwords, kStateWords))),
+                       dim3(256), 0, b->ctx->stream, b->d_st, b->d_gbits, b->gwords,
+                       bits ? b->d_bits_out : nullptr, L);
+    HIPCHK(hipGetLastError());
+    if (!bits) return GV_OK;
+    HIPCHK(hipMemcpyAsync(bits, b->d_bits_out, 4ull * b->gwords * L.n, hipMemcpyDeviceToHost,
+                          b->ctx->stream));
+    return sync(b);
 }
 
 bool same_step(const gv_step_t& x, const gv_step_t& y) {
@@ -940,12 +989,12 @@ int gv_batch_create(gv_ctx_t c, gv_graph_t g, uint32_t slots, uint64_t slot_byte
               hipMalloc(&b->d_sacc, 8ull * kBLines * kAccStride * slots) == hipSuccess &&
               hipHostMalloc((void**)&b->h_desc, sizeof(BDesc) * slots, hipHostMallocDefault) == hipSuccess &&
               hipHostMalloc((void**)&b->h_st, sizeof(BState) * slots, hipHostMallocDefault) == hipSuccess &&
-              hipHostMalloc((void**)&b->h_zero, sizeof(BState), hipHostMallocDefault) == hipSuccess;
+              hipMalloc(&b->d_st_stage, sizeof(BState) * slots) == hipSuccess &&
+              hipMalloc(&b->d_bits_out, 4ull * b->gwords * slots) == hipSuccess;
     if (!ok) {
         gv_batch_destroy(b);
         return GV_ERR_NOMEM;
     }
-    memset(b->h_zero, 0, sizeof(BState));
     HIPCHK(hipMemsetAsync(b->d_desc, 0, sizeof(BDesc) * slots, c->stream));
     HIPCHK(hipMemsetAsync(b->d_st, 0, sizeof(BState) * slots, c->stream));
     HIPCHK(hipMemsetAsync(b->d_glob, 0, 8 * 8, c->stream));
@@ -969,12 +1018,12 @@ int gv_batch_destroy(gv_batch_t b) {
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
     void* ptrs[] = {b->d_desc, b->d_st, b->d_half[0], b->d_half[1], b->d_anc_beg, b->d_cnt,
-                    b->d_anc_i, b->d_tiles, b->d_glob, b->d_gbits, b->d_sacc, b->d_kb};
+                    b->d_anc_i, b->d_tiles, b->d_glob, b->d_gbits, b->d_sacc, b->d_kb,
+                    b->d_st_stage, b->d_bits_out};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (b->h_desc) (void)hipHostFree(b->h_desc);
     if (b->h_st) (void)hipHostFree(b->h_st);
-    if (b->h_zero) (void)hipHostFree(b->h_zero);
     delete b;
     return GV_OK;
 }
@@ -1181,18 +1230,15 @@ int gv_batch_release(gv_batch_t b, uint32_t s, uint32_t* bits, uint32_t nwords) 
         b->hstate[s] = GV_SLOT_FREE;
         return GV_OK;
     }
-    if (bits && nwords) {
-        HIPCHK(hipMemcpyAsync(bits, b->d_gbits + (uint64_t)s * b->gwords, 4ull * nwords,
-                              hipMemcpyDeviceToHost, b->ctx->stream));
-    }
     // the slot stops here whatever its state: its device state becomes free
-    // (from the constant zero entry, so no staging entry is involved)
-    HIPCHK(hipMemcpyAsync(b->d_st + s, b->h_zero, sizeof(BState), hipMemcpyHostToDevice,
-                          b->ctx->stream));
-    if (bits && nwords) {
-        int rc = sync(b);
-        if (rc != GV_OK) return rc;
-    }
+    SlotList L;
+    L.n = 1;
+    L.s[0] = (uint16_t)s;
+    std::vector<uint32_t> all;
+    if (bits && nwords) all.resize(b->gwords);
+    int rc = release_list(b, L, bits && nwords ? all.data() : nullptr);
+    if (rc != GV_OK) return rc;
+    if (bits && nwords) memcpy(bits, all.data(), 4ull * nwords);
     b->used[s] = 0;
     b->hstate[s] = GV_SLOT_FREE;
     return GV_OK;
@@ -1260,27 +1306,31 @@ int gv_batch_release_many(gv_batch_t b, uint32_t n, const uint32_t* slots, uint3
     if (!b || (n && !slots) || (bits && nwords != b->gwords)) return GV_ERR_ARG;
     for (uint32_t i = 0; i < n; ++i)
         if (slots[i] >= b->nslots || !b->used[slots[i]]) return GV_ERR_ARG;
-    bool copied = false;
+    // one kernel frees every listed slot that reached the device and gathers
+    // their bitsets; one copy brings those back
+    SlotList L;
+    L.n = 0;
+    std::vector<uint32_t> pos;   // output position of each listed slot
     for (uint32_t i = 0; i < n; ++i) {
         const uint32_t s = slots[i];
         if (b->dirty[s]) {   // never reached the device: no bits, nothing to undo
             if (bits) memset(bits + (uint64_t)i * nwords, 0, 4ull * nwords);
             b->dirty[s] = 0;
-            b->used[s] = 0;
-        b->hstate[s] = GV_SLOT_FREE;
-            continue;
+        } else {
+            L.s[L.n++] = (uint16_t)s;
+            pos.push_back(i);
         }
-        if (bits) {
-            HIPCHK(hipMemcpyAsync(bits + (uint64_t)i * nwords, b->d_gbits + (uint64_t)s * b->gwords,
-                                  4ull * nwords, hipMemcpyDeviceToHost, b->ctx->stream));
-            copied = true;
-        }
-        HIPCHK(hipMemcpyAsync(b->d_st + s, b->h_zero, sizeof(BState), hipMemcpyHostToDevice,
-                              b->ctx->stream));
         b->used[s] = 0;
         b->hstate[s] = GV_SLOT_FREE;
     }
-    if (copied) return sync(b);
+    const bool direct = L.n == n;   // no dirty slot skipped: positions are 0..n-1
+    std::vector<uint32_t> gathered;
+    if (bits && !direct) gathered.resize((uint64_t)L.n * nwords);
+    int rc = release_list(b, L, bits ? (direct ? bits : gathered.data()) : nullptr);
+    if (rc != GV_OK) return rc;
+    if (bits && !direct)
+        for (uint32_t k = 0; k < L.n; ++k)
+            memcpy(bits + (uint64_t)pos[k] * nwords, gathered.data() + (uint64_t)k * nwords, 4ull * nwords);
     return GV_OK;
 }
 
