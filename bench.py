@@ -542,7 +542,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--db-scale", type=float, default=1.0, help="workload 3 scale")
     ap.add_argument("--slot-mib", type=int, default=256, help="MiB per batch slot table")
-    ap.add_argument("--qps", type=float, nargs="*", default=[100, 200, 500, 1000, 2000, 5000])
+    ap.add_argument("--qps", type=float, nargs="*", default=[100, 200, 500, 1000, 2000, 5000, 10000])
     ap.add_argument("--serve-seconds", type=float, default=1.5, help="seconds of arrivals per rate")
     ap.add_argument("--serve-min", type=int, default=300)
     ap.add_argument("--serve-max", type=int, default=8000)
