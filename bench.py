@@ -91,8 +91,8 @@ def copy_calibration(mib):
 
 
 # kernel (lean variant, as rocprofv3 names it) -> the kind bench.py reports
-PMC_KINDS = [("k_b_walk<1>", "k_b_walk"), ("k_b_walk<2>", "k_b_walk"), ("k_b_walk<4>", "k_b_walk"),
-             ("k_b_walk<8>", "k_b_walk"), ("k_b_size", "k_b_size"), ("k_b_tscan", "k_b_scan"),
+PMC_KINDS = [(f"k_b_walk<{i}, {c}>", "k_b_walk") for i in (1, 2, 4, 8) for c in ("false", "true")] + [
+             ("k_b_size", "k_b_size"), ("k_b_tscan", "k_b_scan"),
              ("k_walk<true, false, 2>", "k_walk<count>"), ("k_walk<true, false, 4>", "k_walk<count>"), ("k_walk<false, false, 4>", "k_walk<extend>"),
              ("k_walk<false, false, 2>", "k_walk<extend>"),
              ("k_size<false>", "k_size"), ("k_count_rows<false>", "k_count_rows"),
@@ -158,9 +158,11 @@ def pmc_summary(paths, workload, cal_mib, last=0):
         k["write_kib"] += sum(w[kern].values())
         # a launch of the scan kind is its two kernels; the row counters are
         # counted by their counting kernel (the chain's marking and weights
-        # belong to the chain count)
+        # belong to the chain count); a batch walk launch is its extension and
+        # its counting kernel
         if kern not in ("k_scan_tiles", "k_mark_parents", "k_isect", "k_isect_big") \
-                and not kern.startswith("k_chain_"):
+                and not kern.startswith("k_chain_") \
+                and not (kern.startswith("k_b_walk<") and kern.endswith("true>")):
             k["launches"] += len(f[kern])
     print("| Kind | Kernels (dispatches) | FETCH_SIZE KiB / launch | WRITE_SIZE KiB / launch | HBM bytes / launch |")
     print("|---|---|---|---|---|")

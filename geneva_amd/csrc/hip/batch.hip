@@ -110,6 +110,10 @@ struct BArgs {
     uint8_t* anc_i;
     uint64_t row_cap;
     uint64_t* tiles;            // per sizing tile: its total, then the tiles' exclusive prefix
+    // walk tiles: the extension slots' first, then the counted slots'; slot
+    // wslot[i] owns the tiles up to wend[i] (ascending)
+    uint64_t* wend;
+    uint16_t* wslot;
     unsigned long long* glob;   // [0] rows, [1] candidates, [2] tiles, [3] sizing tiles,
                                 // [4] shared tables (forks), [5] followers restarted
     uint64_t tile;              // candidates per walk tile (64 * IPT)
@@ -548,28 +552,46 @@ __global__ __launch_bounds__(1024) void k_b_tscan(BArgs a) {
             nwt = (ce - cb + tl - 1) / tl;
         }
     }
-    uint64_t tot;
-    const uint64_t wex = block_scan_1024(nwt, wsum, tot);
-    if (s < a.nslots) {
+    // the extension slots' tiles first, then the counted ones' (two kernels)
+    const bool in = s < a.nslots;
+    const bool ext = in && a.st[s].mode == 1;
+    uint64_t t0, t1, n0, n1;
+    const uint64_t wex0 = block_scan_1024(ext ? nwt : 0, wsum, t0);
+    const uint64_t wex1 = block_scan_1024(in && !ext ? nwt : 0, wsum, t1);
+    const uint64_t rk0 = block_scan_1024(ext ? 1 : 0, wsum, n0);
+    const uint64_t rk1 = block_scan_1024(in && !ext ? 1 : 0, wsum, n1);
+    if (in) {
+        const uint64_t end = ext ? wex0 + nwt : t0 + wex1 + nwt;
+        const uint64_t rank = ext ? rk0 : n0 + rk1;
         a.st[s].cand_base = cb;
         a.st[s].cand_end = ce;
-        a.st[s].tile_end = wex + nwt;
+        a.st[s].tile_end = end;
+        a.wend[rank] = end;
+        a.wslot[rank] = (uint16_t)s;
     }
     if (s == 0) {
         a.glob[1] = cand;
-        a.glob[2] = tot;
+        a.glob[2] = t0 + t1;
+        a.glob[6] = t0;
     }
 }
 
 // ---- walk -------------------------------------------------------------------
 
-template <int IPT>
+// The walk over the extension slots' tiles (kCount false) or the counted
+// slots' (true): two kernels, so that each holds only its own path's
+// registers (one more wave per SIMD each).
+template <int IPT, bool kCount>
 __global__ __launch_bounds__(kBlock) void k_b_walk(BArgs a) {
-    __shared__ uint64_t tile_end[kMaxSlots];
+    __shared__ uint64_t wend[kMaxSlots];
+    __shared__ uint16_t wslot[kMaxSlots];
     __shared__ uint32_t wl[kWavesPerBlock][wave_lds_words<IPT>()];
-    for (uint32_t i = threadIdx.x; i < a.nslots; i += blockDim.x) tile_end[i] = a.st[i].tile_end;
+    for (uint32_t i = threadIdx.x; i < a.nslots; i += blockDim.x) {
+        wend[i] = a.wend[i];
+        wslot[i] = a.wslot[i];
+    }
     __syncthreads();
-    const uint64_t ntiles = a.glob[2];
+    const uint64_t tlo = kCount ? a.glob[6] : 0, ntiles = kCount ? a.glob[2] : a.glob[6];
     const unsigned lane = __lane_id(), wib = threadIdx.x >> 6;
     uint32_t cur = 0xffffffffu;
     TileAcc acc;
@@ -588,7 +610,7 @@ __global__ __launch_bounds__(kBlock) void k_b_walk(BArgs a) {
     };
     // block rounds: wave wib of block b takes tile (round * blocks + b) * 4 + wib,
     // and every wave of the block runs every round (the reservation is shared)
-    for (uint64_t T0 = (uint64_t)blockIdx.x * kWavesPerBlock; T0 < ntiles;
+    for (uint64_t T0 = tlo + (uint64_t)blockIdx.x * kWavesPerBlock; T0 < ntiles;
          T0 += (uint64_t)gridDim.x * kWavesPerBlock) {
         const uint64_t T = T0 + wib;
         if (T >= ntiles) {
@@ -596,7 +618,8 @@ __global__ __launch_bounds__(kBlock) void k_b_walk(BArgs a) {
             continue;
         }
         // the tile's slot, made wave-uniform so its descriptor is read once
-        const uint32_t s = __builtin_amdgcn_readfirstlane(slot_of(tile_end, a.nslots, T));
+        const uint32_t pos = __builtin_amdgcn_readfirstlane(slot_of(wend, a.nslots, T));
+        const uint32_t s = wslot[pos];
         if (s != cur) {
             flush();
             cur = s;
@@ -607,7 +630,7 @@ __global__ __launch_bounds__(kBlock) void k_b_walk(BArgs a) {
         const uint64_t rows = L == 1 ? D.start_n : S.rows;
         const uint64_t tl = D.tile ? D.tile : a.tile;
         const uint64_t ntile_s = (S.cand_end - S.cand_base + tl - 1) / tl;
-        const uint64_t k = T - (tile_end[s] - ntile_s);
+        const uint64_t k = T - (wend[pos] - ntile_s);
         const uint64_t t0 = S.cand_base + k * tl;
         const uint64_t t1 = t0 + tl < S.cand_end ? t0 + tl : S.cand_end;
         TileIn in;
@@ -622,7 +645,7 @@ __global__ __launch_bounds__(kBlock) void k_b_walk(BArgs a) {
         in.row_hi = S.row_end;
         uint32_t* gb = a.g.gof ? a.gbits + (uint64_t)s * a.gwords : nullptr;
         const gv_step_t& st = D.q.step[L];
-        if (S.mode == 1) {
+        if constexpr (!kCount) {
             TileOut o{slot_out(a, s, S), a.slot_uints / (L + 1), &a.st[s].out_rows, &a.st[s].overflow};
             walk_tile<false, IPT>(a.g, st, st, false, in, t0, t1, o, wl[wib], wl[wib], wl[wib] + 64 * IPT + 1, gb,
                                   acc, reserve);
@@ -733,6 +756,8 @@ struct gv_batch {
     uint64_t epoch = 1;                 // advanced by every synchronisation here
     BDesc* h_desc = nullptr;            // pinned staging, one entry per slot
     BState* h_st = nullptr;
+    uint64_t* d_wend = nullptr;
+    uint16_t* d_wslot = nullptr;
     BState* d_st_stage = nullptr;       // staged states on their way to d_st (flush)
     uint32_t* d_bits_out = nullptr;     // released slots' graph bitsets, gathered
     hipGraph_t graph = nullptr;
@@ -772,7 +797,15 @@ BArgs bargs(gv_batch* b) {
     a.sacc = b->d_sacc;
     a.tile = 64ull * b->tune.ipt;
     a.kb = b->kb_on ? b->d_kb : nullptr;
+    a.wend = b->d_wend;
+    a.wslot = b->d_wslot;
     return a;
+}
+
+template <int IPT>
+void launch_walk(const BArgs& a, int blocks, hipStream_t st) {
+    hipLaunchKernelGGL((k_b_walk<IPT, false>), dim3(blocks), dim3(kBlock), 0, st, a);
+    hipLaunchKernelGGL((k_b_walk<IPT, true>), dim3(blocks), dim3(kBlock), 0, st, a);
 }
 
 // The kernels of one iteration on the context's stream; with `prof` each is
@@ -797,10 +830,10 @@ void launch_iteration(gv_batch* b, bool prof) {
     hipLaunchKernelGGL(k_b_tscan, dim3(1), dim3(1024), 0, c->stream, a);
     timed(kKindExt);
     switch (b->tune.ipt) {
-        case 1: hipLaunchKernelGGL(k_b_walk<1>, dim3(walk_blocks), dim3(kBlock), 0, c->stream, a); break;
-        case 2: hipLaunchKernelGGL(k_b_walk<2>, dim3(walk_blocks), dim3(kBlock), 0, c->stream, a); break;
-        case 8: hipLaunchKernelGGL(k_b_walk<8>, dim3(walk_blocks), dim3(kBlock), 0, c->stream, a); break;
-        default: hipLaunchKernelGGL(k_b_walk<4>, dim3(walk_blocks), dim3(kBlock), 0, c->stream, a); break;
+        case 1: launch_walk<1>(a, walk_blocks, c->stream); break;
+        case 2: launch_walk<2>(a, walk_blocks, c->stream); break;
+        case 8: launch_walk<8>(a, walk_blocks, c->stream); break;
+        default: launch_walk<4>(a, walk_blocks, c->stream); break;
     }
     timed(kKindBatch);
     hipLaunchKernelGGL(k_b_post, dim3(1), dim3(1024), 0, c->stream, a);
@@ -990,6 +1023,8 @@ int gv_batch_create(gv_ctx_t c, gv_graph_t g, uint32_t slots, uint64_t slot_byte
               hipHostMalloc((void**)&b->h_desc, sizeof(BDesc) * slots, hipHostMallocDefault) == hipSuccess &&
               hipHostMalloc((void**)&b->h_st, sizeof(BState) * slots, hipHostMallocDefault) == hipSuccess &&
               hipMalloc(&b->d_st_stage, sizeof(BState) * slots) == hipSuccess &&
+              hipMalloc(&b->d_wend, 8ull * slots) == hipSuccess &&
+              hipMalloc(&b->d_wslot, 2ull * slots) == hipSuccess &&
               hipMalloc(&b->d_bits_out, 4ull * b->gwords * slots) == hipSuccess;
     if (!ok) {
         gv_batch_destroy(b);
@@ -1019,7 +1054,7 @@ int gv_batch_destroy(gv_batch_t b) {
     if (b->graph) (void)hipGraphDestroy(b->graph);
     void* ptrs[] = {b->d_desc, b->d_st, b->d_half[0], b->d_half[1], b->d_anc_beg, b->d_cnt,
                     b->d_anc_i, b->d_tiles, b->d_glob, b->d_gbits, b->d_sacc, b->d_kb,
-                    b->d_st_stage, b->d_bits_out};
+                    b->d_st_stage, b->d_bits_out, b->d_wend, b->d_wslot};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (b->h_desc) (void)hipHostFree(b->h_desc);
