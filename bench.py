@@ -214,8 +214,9 @@ def section_w3(args):
     t0 = time.perf_counter()
     ds = configs.workload3(scale=args.db_scale)
     eng = G.Engine(ds, device=0, level_bytes=1 << 30)
-    # 256 slots, two tables of slot_mib each
-    eng.batch_config(slots=256, slot_bytes=args.slot_mib << 20, row_cap=1 << 28)
+    # 264 slots (the 256 queries and the pieces of any too large for one
+    # slot), two tables of slot_mib each
+    eng.batch_config(slots=264, slot_bytes=args.slot_mib << 20, row_cap=1 << 28)
     qs = ds.queries
     log(f"workload 3 scale {args.db_scale}: nv={ds.nv} graphs={ds.ngraphs} queries={len(qs)} "
         f"graph {eng.graph_bytes / 2**20:.0f} MiB ready in {time.perf_counter() - t0:.1f}s")

@@ -1369,6 +1369,16 @@ int gv_batch_release_many(gv_batch_t b, uint32_t n, const uint32_t* slots, uint3
     return GV_OK;
 }
 
+int gv_batch_slot_iterations(gv_batch_t b, uint32_t slot, uint32_t* iterations) {
+    if (!b || slot >= b->nslots || !b->used[slot] || !iterations) return GV_ERR_ARG;
+    const BDesc& d = b->h_desc[slot];
+    uint32_t last = counted_width(d);
+    if (d.leader != kNoSlot) last = std::max(last, d.fork_width);
+    const uint32_t w = b->hwidth[slot];
+    *iterations = last >= w ? last - w + 1 : 1;
+    return GV_OK;
+}
+
 int gv_batch_info(gv_batch_t b, uint32_t* slots, uint64_t* slot_bytes, uint32_t* free_slots) {
     if (!b) return GV_ERR_ARG;
     if (slots) *slots = b->nslots;

@@ -164,7 +164,6 @@ int BatchMiner::run(const std::vector<CompiledQuery>& qs, std::vector<MineResult
     uint32_t running = 0;
     gv_mark(ctx_, 2);
     while (!pending.empty() || running > 0) {
-        uint32_t maxk = 0;
         while (!pending.empty()) {
             uint32_t slot = 0;
             const Piece pc = pending.front();
@@ -176,12 +175,19 @@ int BatchMiner::run(const std::vector<CompiledQuery>& qs, std::vector<MineResult
             if (rc == GV_ERR_CAPACITY) break;
             if (rc != GV_OK) return rc;
             slotPiece[slot] = pc;
-            maxk = std::max(maxk, qs[pc.q].q.k);
             pending.pop_front();
             ++running;
         }
-        // every admitted query needs at most k-1 iterations; deferred ones loop again
-        const uint32_t iters = std::max<uint32_t>(1, maxk ? maxk - 1 : 1);
+        // as many iterations as the running slot furthest from its end needs
+        // (deferred slots need more: they come round again)
+        uint32_t iters = 1;
+        for (uint32_t s = 0; s < opt_.slots; ++s) {
+            if (slotPiece[s].q == SIZE_MAX) continue;
+            uint32_t n = 1;
+            rc = gv_batch_slot_iterations(b_, s, &n);
+            if (rc != GV_OK) return rc;
+            iters = std::max(iters, n);
+        }
         rc = gv_batch_run(b_, iters);
         if (rc != GV_OK) return rc;
         local.graphLaunches += iters;

@@ -357,6 +357,12 @@ int gv_batch_share_stats(gv_batch_t b, uint64_t* shared, uint64_t* restarted);
  * context's profiling was on (oldest first; cleared when read into rows or
  * candidates). */
 int gv_batch_history(gv_batch_t b, uint64_t* rows, uint64_t* candidates, uint32_t cap, uint32_t* n);
+/* Iterations a running slot needs from the width it had at admission or at
+ * the last poll until it finishes, when it is never deferred: its counted
+ * width (the width of the table its last step, a fused tail or its leaf
+ * groups are counted on; for a follower at least the shared width) minus
+ * that width, plus one. */
+int gv_batch_slot_iterations(gv_batch_t b, uint32_t slot, uint32_t* iterations);
 /* Slots, rows a slot table holds at a given width, free slots. */
 int gv_batch_info(gv_batch_t b, uint32_t* slots, uint64_t* slot_bytes, uint32_t* free_slots);
 
