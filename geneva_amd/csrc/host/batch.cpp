@@ -342,24 +342,10 @@ int autotune(BatchMiner& bm, const std::vector<CompiledQuery>& sample, TuneResul
     out.bestMs = best;
     int rc = bm.setTune(out.tune);
     if (rc != GV_OK) return rc;
-    // rows sized per iteration: fewer rows run fewer slots together
-    const uint64_t full = bm.options().rowCap;
-    out.rowCap = full;
-    for (uint64_t cap : {full / 4, full / 16}) {
-        if (cap < (1u << 20)) break;
-        rc = bm.setRowCap(cap);
-        if (rc != GV_OK) return rc;
-        BatchStats st;
-        rc = bm.run(sample, res, nullptr);
-        if (rc == GV_OK) rc = bm.run(sample, res, &st);
-        if (rc != GV_OK) return rc;
-        if (st.deviceMs < out.bestMs) {
-            out.bestMs = st.deviceMs;
-            out.rowCap = cap;
-        }
-    }
-    rc = bm.setRowCap(out.rowCap);
-    if (rc != GV_OK) return rc;
+    // rows sized per iteration: the allocation's.  A smaller cap timed on
+    // the sample does not carry over to a larger batch, whose first
+    // iterations it would defer.
+    out.rowCap = bm.options().rowCap;
     // per class: candidates per lane up to the chosen kernel's
     std::map<std::string, std::vector<CompiledQuery>> byClass;
     for (auto& q : sample)

@@ -95,9 +95,8 @@ private:
 
 // Autotuner: times the batch engine under each candidate tuning on a sample of
 // queries and keeps the fastest (candidates per lane of the walk kernel x its
-// blocks per CU), then the rows sized per iteration (the allocation, a
-// quarter or a sixteenth of it: how many slots run together), then, for each
-// query class of the sample with at least four queries, times that class
+// blocks per CU; the rows sized per iteration stay the allocation's), then,
+// for each query class of the sample with at least four queries, times that class
 // alone under each candidates-per-lane value up to the chosen one and keeps
 // the fastest for the class (slots of the class run their walk tiles at that
 // size).  Remembered per graph, engine options and sample shape.

@@ -1,9 +1,13 @@
 // extern "C" surface of the host library (include/geneva_host.h).
 #include "../../../include/geneva_host.h"
 
+#include <atomic>
 #include <chrono>
+#include <condition_variable>
 #include <cstring>
+#include <functional>
 #include <memory>
+#include <mutex>
 #include <sstream>
 #include <string>
 #include <thread>
@@ -25,6 +29,71 @@ struct gh_dataset {
     geneva::Dataset ds;
 };
 
+namespace {
+// Host threads that plan a list of queries: idle between calls, each call's
+// queries taken one at a time from a shared counter.
+class PlanPool {
+public:
+    explicit PlanPool(uint32_t n) {
+        for (uint32_t t = 0; t < n; ++t) th_.emplace_back([this] { loop(); });
+    }
+    ~PlanPool() {
+        {
+            std::lock_guard<std::mutex> l(m_);
+            stop_ = true;
+        }
+        cv_.notify_all();
+        for (auto& t : th_) t.join();
+    }
+    // Run f(i) for i in [0, n) on the pool and the calling thread.
+    void run(uint32_t n, const std::function<void(uint32_t)>& f) {
+        {
+            std::lock_guard<std::mutex> l(m_);
+            f_ = &f;
+            n_ = n;
+            next_.store(0);
+            busy_ = (uint32_t)th_.size();
+            ++gen_;
+        }
+        cv_.notify_all();
+        drain();
+        std::unique_lock<std::mutex> l(m_);
+        done_.wait(l, [this] { return busy_ == 0; });
+        f_ = nullptr;
+    }
+
+private:
+    void drain() {
+        for (uint32_t i = next_.fetch_add(1); i < n_; i = next_.fetch_add(1)) (*f_)(i);
+    }
+    void loop() {
+        uint64_t seen = 0;
+        for (;;) {
+            {
+                std::unique_lock<std::mutex> l(m_);
+                cv_.wait(l, [&] { return stop_ || gen_ != seen; });
+                if (stop_) return;
+                seen = gen_;
+            }
+            drain();
+            {
+                std::lock_guard<std::mutex> l(m_);
+                if (--busy_ == 0) done_.notify_all();
+            }
+        }
+    }
+    std::vector<std::thread> th_;
+    std::mutex m_;
+    std::condition_variable cv_, done_;
+    const std::function<void(uint32_t)>* f_ = nullptr;
+    uint32_t n_ = 0;
+    std::atomic<uint32_t> next_{0};
+    uint32_t busy_ = 0;
+    uint64_t gen_ = 0;
+    bool stop_ = false;
+};
+}  // namespace
+
 struct gh_engine {
     gv_ctx_t ctx = nullptr;
     gv_graph_t graph = nullptr;
@@ -43,6 +112,7 @@ struct gh_engine {
     double lastFraction = 1.0;
     // per-class tuning of the last autotune: "class:ipt:ms;" entries
     std::string tuneTable;
+    std::unique_ptr<PlanPool> planPool;   // made on the first list of queries
 };
 
 namespace {
@@ -64,19 +134,15 @@ int compileAll(gh_engine* e, const char* const* qs, uint32_t n, int single_v,
     out.assign(n, geneva::CompiledQuery());
     std::vector<std::string> err(n);
     std::vector<char> ok(n, 0);
-    auto work = [&](uint32_t lo, uint32_t hi) {
-        for (uint32_t i = lo; i < hi; ++i)
-            ok[i] = qs[i] && geneva::planAndCompile(qs[i], single_v != 0, e->graph, out[i], &err[i]);
+    const std::function<void(uint32_t)> one = [&](uint32_t i) {
+        ok[i] = qs[i] && geneva::planAndCompile(qs[i], single_v != 0, e->graph, out[i], &err[i]);
     };
-    const uint32_t nt = std::min<uint32_t>(std::max(1u, std::thread::hardware_concurrency()),
-                                           std::min<uint32_t>(16, (n + 7) / 8));
-    if (nt <= 1) {
-        work(0, n);
+    if (n < 16) {
+        for (uint32_t i = 0; i < n; ++i) one(i);
     } else {
-        std::vector<std::thread> th;
-        const uint32_t per = (n + nt - 1) / nt;
-        for (uint32_t t = 0; t < nt; ++t) th.emplace_back(work, std::min(n, t * per), std::min(n, (t + 1) * per));
-        for (auto& t : th) t.join();
+        if (!e->planPool)
+            e->planPool.reset(new PlanPool(std::min<uint32_t>(15, std::max(1u, std::thread::hardware_concurrency()) - 1)));
+        e->planPool->run(n, one);
     }
     for (uint32_t i = 0; i < n; ++i)
         if (!ok[i]) {

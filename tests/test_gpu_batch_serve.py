@@ -148,8 +148,8 @@ def test_tunings_agree(w3):
     # nothing but speed
     assert tuned["classes"], tuned
     assert all(c["ipt"] <= tuned["ipt"] for c in tuned["classes"].values())
-    # rows per iteration: the allocation or a quarter of it
-    assert tuned["row_cap"] in (1 << 22, 1 << 20), tuned
+    # rows per iteration: the allocation's
+    assert tuned["row_cap"] == 1 << 22, tuned
     # first steps from the start tables instead of the label-pair edge tables
     eng.batch_pair_start(False)
     res2, _ = eng.mine_batch(qs)
