@@ -90,6 +90,34 @@ def copy_calibration(mib):
     hip.gv_ctx_destroy(ctx)
 
 
+def gather_ceiling(n=100_000_000, sizes_mib=(16, 64, 256, 800, 1450, 2900)):
+    """Rate of independent random 4-byte reads the GPU sustains from tables
+    of several sizes (torch's index_select: n random positions, each read
+    once), in reads per second: the ceiling for kernels whose time goes to
+    scattered single-word reads (label offsets, first list entries)."""
+    import torch
+    out = []
+    for mib in sizes_mib:
+        words = (mib << 20) // 4
+        x = torch.ones(words, dtype=torch.int32, device="cuda")
+        idx = torch.randint(0, words, (n,), device="cuda", dtype=torch.int64)
+        for _ in range(2):
+            torch.index_select(x, 0, idx)
+        torch.cuda.synchronize()
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+        ev[0].record()
+        for _ in range(5):
+            torch.index_select(x, 0, idx)
+        ev[1].record()
+        torch.cuda.synchronize()
+        ms = ev[0].elapsed_time(ev[1]) / 5
+        out.append({"table_mib": mib, "reads": n, "ms": round(ms, 3), "reads_per_s": round(n / ms * 1e3)})
+        log(f"gather from {mib} MiB: {n / ms / 1e6:.1f} G reads/s")
+        del x, idx
+        torch.cuda.empty_cache()
+    print(json.dumps({"gather_ceiling": out}), flush=True)
+
+
 # kernel (lean variant, as rocprofv3 names it) -> the kind bench.py reports
 PMC_KINDS = [(f"k_b_walk<{i}, {c}>", "k_b_walk") for i in (1, 2, 4, 8) for c in ("false", "true")] + [
              ("k_b_size", "k_b_size"), ("k_b_tscan", "k_b_scan"),
@@ -571,6 +599,8 @@ def main():
     ap.add_argument("--pmc-last", type=int, default=0,
                     help="--pmc-summary over the dispatches of the last N batch iterations (the accounting run)")
     ap.add_argument("--pmc-table", metavar="CSV", help="only print per-kernel means of a rocprofv3 --pmc CSV")
+    ap.add_argument("--gather-ceiling", action="store_true",
+                    help="only measure the GPU's rate of random 4-byte reads from tables of 16 MiB - 2.9 GiB")
     ap.add_argument("--only", choices=("w2", "w4"),
                     help="only run the workload-2 or workload-4 section and print its JSON (profiling runs)")
     args = ap.parse_args()
@@ -580,6 +610,8 @@ def main():
         return pmc_summary(args.pmc_summary, args.workload, args.cal_mib, args.pmc_last)
     if args.pmc_table:
         return pmc_table(args.pmc_table)
+    if args.gather_ceiling:
+        return gather_ceiling()
     if args.gpus != 1:
         log("bench.py measures one GPU: this build runs and measures the embedding-extension path on a "
             "single MI355X; sharding over several GPUs is a design note, not part of the build.")

@@ -97,6 +97,8 @@ def lib():
         "gh_engine_batch_row_cap": (i32, [vp, C.POINTER(u64)]),
         "gh_engine_batch_share": (i32, [vp, i32]),
         "gh_engine_batch_share_stats": (i32, [vp, C.POINTER(u64), C.POINTER(u64)]),
+        "gh_engine_batch_fuse_sizing": (i32, [vp, i32]),
+        "gh_engine_batch_fused_rows": (i32, [vp, C.POINTER(u64)]),
         "gh_engine_batch_history": (i32, [vp, C.POINTER(u64), C.POINTER(u64), u32, C.POINTER(u32)]),
         "gh_engine_profile_records": (i32, [vp, C.POINTER(i32), C.POINTER(C.c_float), u64, C.POINTER(u64)]),
         "gh_engine_profile": (i32, [vp, i32]),
@@ -477,6 +479,22 @@ class Engine:
         if rc:
             raise _err("batch_share_stats", rc)
         return a.value, b.value
+
+    def batch_fuse_sizing(self, on=True):
+        """Size batch slots that read one input table together (on by
+        default; the results are the same either way)."""
+        rc = lib().gh_engine_batch_fuse_sizing(self._h, int(on))
+        if rc:
+            raise _err("batch_fuse_sizing", rc)
+
+    def batch_fused_rows(self):
+        """Rows sized as part of another slot's sizing group since the batch
+        engine was made."""
+        a = C.c_uint64(0)
+        rc = lib().gh_engine_batch_fused_rows(self._h, C.byref(a))
+        if rc:
+            raise _err("batch_fused_rows", rc)
+        return a.value
 
     def batch_history(self):
         """[(rows sized, candidates walked)] of each batch iteration run while

@@ -87,6 +87,12 @@ struct BState {              // dynamic, polled by the host (mirrors gv_slot_inf
     uint64_t serial;         // admission number (never 0 for an admitted slot)
     uint32_t src;            // slot and half holding this iteration's input table
     uint32_t src_sel;
+    // slots sizing the same input table this iteration are sized together:
+    // the group's head (szl) runs each of its tiles for the others too, so
+    // their row and neighbour-list lines are read while still in the cache;
+    // sznx = the next slot of the group, kNoSlot after the last
+    uint32_t szl;
+    uint32_t sznx;
 };
 
 // A slot that waited this many iterations in a row for row space is placed
@@ -115,8 +121,11 @@ struct BArgs {
     uint64_t* wend;
     uint16_t* wslot;
     unsigned long long* glob;   // [0] rows, [1] candidates, [2] tiles, [3] sizing tiles,
-                                // [4] shared tables (forks), [5] followers restarted
+                                // [4] shared tables (forks), [5] followers restarted,
+                                // [6] extension tiles, [7] rows sized with another slot's
+                                // tiles (cumulative)
     uint64_t tile;              // candidates per walk tile (64 * IPT)
+    uint32_t fuse;              // size slots with the same input table together
     uint32_t* gbits;            // nslots * gwords
     uint32_t gwords;
     // spread count / bytes accumulators: kBLines lines (kAccStride words) per
@@ -183,9 +192,7 @@ __device__ uint64_t block_scan_1024(uint64_t v, uint64_t* wsum, uint64_t& total)
 
 // Sizing tiles of a slot with `rows` rows, rounded up to a multiple of the
 // GPU's XCD count.  Workgroup b runs on XCD b mod kXcds and the sizing grid
-// is a multiple of kXcds, so tile k of two adjacent slots runs on one XCD at
-// about the same time: slots that start from the same table (admitted next
-// to each other) then find each other's label-offset lines in that XCD's L2.
+// is a multiple of kXcds, so tile k of every slot lands on the same XCD.
 constexpr uint64_t kXcds = 8;
 __device__ __forceinline__ uint64_t slot_tiles(uint64_t rows) {
     return ((rows + kBSizeTile - 1) / kBSizeTile + kXcds - 1) / kXcds * kXcds;
@@ -228,6 +235,11 @@ __global__ __launch_bounds__(1024) void k_b_prep(BArgs a) {
     __shared__ uint64_t wsum[16];
     __shared__ uint32_t nfollow[kMaxSlots];   // followers sharing each leader's table now
     __shared__ uint8_t ran[kMaxSlots];
+    __shared__ uint64_t gkey[kMaxSlots];      // input table address (0: not running)
+    __shared__ uint64_t grows[kMaxSlots];
+    __shared__ uint32_t gwid[kMaxSlots];
+    __shared__ uint32_t hslot[2 * kMaxSlots];  // sizing groups: bucket -> head slot
+    __shared__ uint32_t lnext[kMaxSlots];      // a head's last linked member
     const uint32_t s = threadIdx.x;
     const bool in = s < a.nslots;
     nfollow[s] = 0;
@@ -344,6 +356,50 @@ __global__ __launch_bounds__(1024) void k_b_prep(BArgs a) {
         rows = 0;
         mode = 0;
     }
+    // sizing groups: running slots whose input tables are one table (same
+    // rows at the same address: slots starting from one start list or
+    // label-pair range, a leader and the followers reading its table)
+    uint64_t key = 0;
+    uint32_t kw = 0;
+    if (in && mode != 0 && a.fuse) {
+        const BState& S = a.st[s];
+        const BDesc& D = a.desc[s];
+        kw = S.width;
+        const uint32_t* p = kw == 1                    ? a.lverts + D.start_off
+                            : kw == 2 && D.init_rows ? D.init_rows
+                                                     : (src_sel ? a.half1 : a.half0) + (uint64_t)src * a.slot_uints;
+        key = (uint64_t)p;
+    }
+    gkey[s] = key;
+    grows[s] = rows;
+    gwid[s] = kw;
+    hslot[s] = kNoSlot;
+    hslot[s + kMaxSlots] = kNoSlot;
+    lnext[s] = kNoSlot;
+    __syncthreads();
+    // the group's head is the slot that claims the table's bucket of an
+    // open-addressing set in LDS (twice as many buckets as slots, so a probe
+    // always ends); the others link themselves after it
+    uint32_t szl = s, linked = kNoSlot;
+    if (key) {
+        uint32_t h = (uint32_t)((key >> 2) * 0x9E3779B97F4A7C15ull >> 40 ^ rows * 0x85EBCA6Bu ^ kw) &
+                     (2 * kMaxSlots - 1);
+        for (;;) {
+            const uint32_t old = atomicCAS(&hslot[h], kNoSlot, s);
+            if (old == kNoSlot) break;
+            if (gkey[old] == key && grows[old] == rows && gwid[old] == kw) {
+                szl = old;
+                break;
+            }
+            h = (h + 1) & (2 * kMaxSlots - 1);
+        }
+        if (szl != s) {
+            linked = atomicExch(&lnext[szl], s);
+            atomicAdd(&a.glob[7], (unsigned long long)rows);
+        }
+    }
+    __syncthreads();
+    const uint32_t sznx = szl == s ? lnext[s] : linked;
     // the running slots' row ranges, in slot order, each from the start of a
     // sizing tile (kBSizeTile rows of one slot), so that row r lies in tile
     // r / kBSizeTile of the iteration's candidate prefix; a slot's tiles are
@@ -359,6 +415,8 @@ __global__ __launch_bounds__(1024) void k_b_prep(BArgs a) {
         S.rmode = rmode;
         S.src = src;
         S.src_sel = src_sel;
+        S.szl = szl;
+        S.sznx = sznx;
         S.row_end = rex * kBSizeTile + rows;
         S.rtile_end = rex + nrt;
     }
@@ -408,8 +466,13 @@ __device__ __forceinline__ Prefix two_level(const BArgs& a) {
 // within the tile, in `cnt`) and the tile's total in tiles[tile].
 __global__ __launch_bounds__(kBSizeThreads) void k_b_size(BArgs a) {
     __shared__ uint64_t rtile_end[kMaxSlots];
+    __shared__ uint16_t szl[kMaxSlots], sznx[kMaxSlots];
     __shared__ uint64_t wsum[kBSizeThreads / 64];
-    for (uint32_t i = threadIdx.x; i < a.nslots; i += blockDim.x) rtile_end[i] = a.st[i].rtile_end;
+    for (uint32_t i = threadIdx.x; i < a.nslots; i += blockDim.x) {
+        rtile_end[i] = a.st[i].rtile_end;
+        szl[i] = (uint16_t)a.st[i].szl;
+        sznx[i] = (uint16_t)a.st[i].sznx;
+    }
     __syncthreads();
     const uint64_t ntiles = a.glob[3];
     const unsigned lane = __lane_id();
@@ -424,9 +487,15 @@ __global__ __launch_bounds__(kBSizeThreads) void k_b_size(BArgs a) {
         if (a.kb && lane == 0 && w) atomicAdd(a.kb, (unsigned long long)w);
         acc = cacc = 0;
     };
-    for (uint64_t RT = blockIdx.x; RT < ntiles; RT += gridDim.x) {
+    for (uint64_t RT0 = blockIdx.x; RT0 < ntiles; RT0 += gridDim.x) {
         // every row of this tile belongs to one slot: its step is read once
-        const uint32_t s = __builtin_amdgcn_readfirstlane(slot_of(rtile_end, a.nslots, RT));
+        const uint32_t s0 = __builtin_amdgcn_readfirstlane(slot_of(rtile_end, a.nslots, RT0));
+        // a slot of a sizing group is sized with the group's head slot:
+        // the same tile of each slot of the group, one after the other
+        if (szl[s0] != s0) continue;
+        const uint64_t back = rtile_end[s0] - RT0;
+        for (uint32_t s = s0; s != (kNoSlot & 0xffffu); s = __builtin_amdgcn_readfirstlane(sznx[s])) {
+        const uint64_t RT = rtile_end[s] - back;
         if (s != cur) {
             flush();
             cur = s;
@@ -516,9 +585,12 @@ __global__ __launch_bounds__(kBSizeThreads) void k_b_size(BArgs a) {
         store_rows(a.anc_beg + r0, beg);
         store_bytes(a.anc_i + r0, ai);
         if (threadIdx.x == 0) a.tiles[RT] = btot;
+        }
     }
     flush();
 }
+
+constexpr int kTScanPer = 16;
 
 // The candidate prefix of the tiles (exclusive, in place in `tiles`), then
 // each running slot's candidate range and its walk tiles (tiles never
@@ -526,16 +598,46 @@ __global__ __launch_bounds__(kBSizeThreads) void k_b_size(BArgs a) {
 __global__ __launch_bounds__(1024) void k_b_tscan(BArgs a) {
     __shared__ uint64_t wsum[16];
     const uint64_t nt = a.glob[3];
-    const uint64_t per = (nt + blockDim.x - 1) / blockDim.x;
-    const uint64_t b0 = min((uint64_t)threadIdx.x * per, nt), b1 = min(b0 + per, nt);
-    uint64_t sum = 0;
-    for (uint64_t i = b0; i < b1; ++i) sum += a.tiles[i];
-    uint64_t cand;
-    uint64_t ex = block_scan_1024(sum, wsum, cand);
-    for (uint64_t i = b0; i < b1; ++i) {
-        const uint64_t v = a.tiles[i];
-        a.tiles[i] = ex;
-        ex += v;
+    // rounds of kTScanPer consecutive tiles per thread (16-byte loads and
+    // stores, all of a round's loads in flight at once), carried over rounds
+    uint64_t cand = 0;
+    for (uint64_t r0 = 0; r0 < nt; r0 += (uint64_t)kTScanPer * blockDim.x) {
+        const uint64_t b0 = r0 + (uint64_t)threadIdx.x * kTScanPer;
+        uint64_t v[kTScanPer];
+        const bool full = b0 + kTScanPer <= nt;
+        if (full) {
+            const ulonglong2* p = reinterpret_cast<const ulonglong2*>(a.tiles + b0);
+#pragma unroll
+            for (int k = 0; k < kTScanPer / 2; ++k) {
+                const ulonglong2 x = p[k];
+                v[2 * k] = x.x;
+                v[2 * k + 1] = x.y;
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < kTScanPer; ++k) v[k] = b0 + k < nt ? a.tiles[b0 + k] : 0;
+        }
+        uint64_t sum = 0;
+#pragma unroll
+        for (int k = 0; k < kTScanPer; ++k) sum += v[k];
+        uint64_t tot;
+        uint64_t ex = cand + block_scan_1024(sum, wsum, tot);
+#pragma unroll
+        for (int k = 0; k < kTScanPer; ++k) {
+            const uint64_t x = v[k];
+            v[k] = ex;
+            ex += x;
+        }
+        if (full) {
+            ulonglong2* p = reinterpret_cast<ulonglong2*>(a.tiles + b0);
+#pragma unroll
+            for (int k = 0; k < kTScanPer / 2; ++k) p[k] = make_ulonglong2(v[2 * k], v[2 * k + 1]);
+        } else {
+#pragma unroll
+            for (int k = 0; k < kTScanPer; ++k)
+                if (b0 + k < nt) a.tiles[b0 + k] = v[k];
+        }
+        cand += tot;
     }
     __syncthreads();
     const uint32_t s = threadIdx.x;
@@ -766,6 +868,7 @@ struct gv_batch {
     bool kb_on = false;
     bool pair_start = true;               // first steps from the label-pair edge tables
     bool share = true;                    // followers share a leader's table (prefix sharing)
+    bool fuse = true;                     // slots with one input table are sized together
     uint64_t next_serial = 1;
     std::vector<uint32_t> hwidth;         // host view: width at admission or last poll
     std::vector<uint32_t> hstate;
@@ -796,6 +899,7 @@ BArgs bargs(gv_batch* b) {
     a.gwords = b->gwords;
     a.sacc = b->d_sacc;
     a.tile = 64ull * b->tune.ipt;
+    a.fuse = b->fuse ? 1u : 0u;
     a.kb = b->kb_on ? b->d_kb : nullptr;
     a.wend = b->d_wend;
     a.wslot = b->d_wslot;
@@ -1311,6 +1415,24 @@ int gv_batch_pair_start(gv_batch_t b, int on) {
 int gv_batch_share(gv_batch_t b, int on) {
     if (!b) return GV_ERR_ARG;
     b->share = on != 0;
+    return GV_OK;
+}
+
+int gv_batch_fuse_sizing(gv_batch_t b, int on) {
+    if (!b) return GV_ERR_ARG;
+    int rc = sync(b);
+    if (rc != GV_OK) return rc;
+    b->fuse = on != 0;
+    return capture(b);
+}
+
+int gv_batch_fused_rows(gv_batch_t b, uint64_t* rows) {
+    if (!b || !rows) return GV_ERR_ARG;
+    uint64_t g[8];
+    HIPCHK(hipMemcpyAsync(g, b->d_glob, sizeof(g), hipMemcpyDeviceToHost, b->ctx->stream));
+    int rc = sync(b);
+    if (rc != GV_OK) return rc;
+    *rows = g[7];
     return GV_OK;
 }
 

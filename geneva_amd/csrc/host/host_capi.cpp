@@ -688,6 +688,20 @@ int gh_engine_batch_share_stats(gh_engine_t e, uint64_t* shared, uint64_t* resta
     return gv_batch_share_stats(e->batch->handle(), shared, restarted);
 }
 
+int gh_engine_batch_fuse_sizing(gh_engine_t e, int on) {
+    if (!e) return GV_ERR_ARG;
+    int rc = ensureBatch(e);
+    if (rc != GV_OK) return rc;
+    return gv_batch_fuse_sizing(e->batch->handle(), on);
+}
+
+int gh_engine_batch_fused_rows(gh_engine_t e, uint64_t* rows) {
+    if (!e) return GV_ERR_ARG;
+    int rc = ensureBatch(e);
+    if (rc != GV_OK) return rc;
+    return gv_batch_fused_rows(e->batch->handle(), rows);
+}
+
 int gh_engine_batch_history(gh_engine_t e, uint64_t* rows, uint64_t* candidates, uint32_t cap, uint32_t* n) {
     if (!e || !e->batch) return GV_ERR_ARG;
     return gv_batch_history(e->batch->handle(), rows, candidates, cap, n);

@@ -353,6 +353,16 @@ int gv_batch_share(gv_batch_t b, int on);
 /* Tables shared and followers restarted on their own since creation
  * (synchronises). */
 int gv_batch_share_stats(gv_batch_t b, uint64_t* shared, uint64_t* restarted);
+/* Sizing groups (on by default): slots that size the same input table in an
+ * iteration (the same start list or label-pair range, a leader and the
+ * followers reading its table) are sized tile by tile together, each tile of
+ * the group's slots by one workgroup in a row, so the rows and neighbour
+ * lists one slot reads are still cached for the next.  Results are the same
+ * either way.  Synchronises and captures the iteration again. */
+int gv_batch_fuse_sizing(gv_batch_t b, int on);
+/* Rows sized as part of another slot's sizing group since creation
+ * (synchronises). */
+int gv_batch_fused_rows(gv_batch_t b, uint64_t* rows);
 /* Rows sized and candidates walked by each iteration run while the
  * context's profiling was on (oldest first; cleared when read into rows or
  * candidates). */

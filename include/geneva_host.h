@@ -189,6 +189,10 @@ int gh_engine_batch_pair_start(gh_engine_t e, int on);
  * and the tables shared and followers restarted so far. */
 int gh_engine_batch_share(gh_engine_t e, int on);
 int gh_engine_batch_share_stats(gh_engine_t e, uint64_t* shared, uint64_t* restarted);
+/* Batch engine: sizing groups on or off (gv_batch_fuse_sizing), and the rows
+ * sized with another slot's group so far (gv_batch_fused_rows). */
+int gh_engine_batch_fuse_sizing(gh_engine_t e, int on);
+int gh_engine_batch_fused_rows(gh_engine_t e, uint64_t* rows);
 /* Rows sized and candidates walked per profiled batch iteration
  * (gv_batch_history). */
 int gh_engine_batch_history(gh_engine_t e, uint64_t* rows, uint64_t* candidates, uint32_t cap, uint32_t* n);

@@ -81,6 +81,34 @@ def test_prefix_sharing_equals_unshared_tables(w3):
     eng.batch_config(slots=64, slot_bytes=8 << 20, row_cap=1 << 22)
 
 
+def test_sizing_groups_equal_separate_sizing(w3):
+    """Slots that size one input table in an iteration (one start list or
+    label-pair range; a leader and its followers) are sized together: every
+    result equals the fixture with the groups on and off, with prefix sharing
+    on and off, and rows were sized in groups only when they were on."""
+    ds, eng, cases = w3
+    want = {c["query"]: c for c in cases}
+    qs = [c["query"] for c in cases]
+    qs = qs + qs[:12]
+    for share in (True, False):
+        eng.batch_config(slots=64, slot_bytes=8 << 20, row_cap=1 << 22)
+        eng.batch_share(share)
+        f0 = eng.batch_fused_rows()
+        on, _ = eng.mine_batch(qs)
+        f1 = eng.batch_fused_rows()
+        eng.batch_fuse_sizing(False)
+        off, _ = eng.mine_batch(qs)
+        f2 = eng.batch_fused_rows()
+        print("share", share, "fused rows", f0, f1, f2, flush=True)
+        assert f1 > f0 and f2 == f1
+        for q, a, b in zip(qs, on, off):
+            c = want[q]
+            assert a.count == b.count == c["count"]
+            assert a.alg_bytes == b.alg_bytes == c["alg_bytes"]
+            assert sorted(a.graph_ids) == sorted(b.graph_ids) == c["graph_ids"]
+    eng.batch_config(slots=64, slot_bytes=8 << 20, row_cap=1 << 22)
+
+
 def test_overflow_falls_back_to_plain_path(require_gpu):
     import oracle as O
     ds = configs.workload4(scale=0.005, nqueries=8)
