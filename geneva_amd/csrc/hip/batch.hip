@@ -132,15 +132,23 @@ struct BArgs {
     // slot; blocks add to line blockIdx % kBLines of their slot and k_b_post
     // folds the lines into the slot's state
     unsigned long long* sacc;
-    // algorithmic bytes B charged per kernel (nullable; profiling): [0] k_b_size
+    // algorithmic bytes B charged per kernel (nullable; profiling), spread
+    // like sacc over kBLines lines: line[0] k_b_size, line[1] k_b_walk
     // (rows read, back lists read, rows counted there), [1] k_b_walk
     unsigned long long* kb;
 };
 
 constexpr uint32_t kBLines = 16;
+constexpr uint64_t kKbBytes = 8ull * kBLines * kAccStride;
 
 __device__ __forceinline__ unsigned long long* slot_line(const BArgs& a, uint32_t s) {
     return a.sacc + ((uint64_t)s * kBLines + blockIdx.x % kBLines) * kAccStride;
+}
+
+// the per-kernel B line of this block (one address for all blocks made the
+// profiled kernels wait on that one address)
+__device__ __forceinline__ unsigned long long* kb_line(const BArgs& a) {
+    return a.kb + (uint64_t)(blockIdx.x % kBLines) * kAccStride;
 }
 
 __device__ __forceinline__ const uint32_t* slot_rows(const BArgs& a, uint32_t s, const BState& S,
@@ -484,7 +492,7 @@ __global__ __launch_bounds__(kBSizeThreads) void k_b_size(BArgs a) {
         unsigned long long* line = slot_line(a, cur);
         if (lane == 0 && c) atomicAdd(line, (unsigned long long)c);
         if (lane == 0 && w) atomicAdd(line + 1, (unsigned long long)w);
-        if (a.kb && lane == 0 && w) atomicAdd(a.kb, (unsigned long long)w);
+        if (a.kb && lane == 0 && w) atomicAdd(kb_line(a), (unsigned long long)w);
         acc = cacc = 0;
     };
     for (uint64_t RT0 = blockIdx.x; RT0 < ntiles; RT0 += gridDim.x) {
@@ -706,7 +714,7 @@ __global__ __launch_bounds__(kBlock) void k_b_walk(BArgs a) {
         if (lane == 0) {
             if (c) atomicAdd(line, (unsigned long long)c);
             if (b) atomicAdd(line + 1, (unsigned long long)b);
-            if (a.kb && b) atomicAdd(a.kb + 1, (unsigned long long)b);
+            if (a.kb && b) atomicAdd(kb_line(a) + 1, (unsigned long long)b);
         }
         acc = TileAcc();
     };
@@ -864,7 +872,7 @@ struct gv_batch {
     uint32_t* d_bits_out = nullptr;     // released slots' graph bitsets, gathered
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
-    unsigned long long* d_kb = nullptr;   // per-kernel B (profiling), 2 words
+    unsigned long long* d_kb = nullptr;   // per-kernel B (profiling), kBLines lines
     bool kb_on = false;
     bool pair_start = true;               // first steps from the label-pair edge tables
     bool share = true;                    // followers share a leader's table (prefix sharing)
@@ -1387,8 +1395,8 @@ int gv_batch_profile(gv_batch_t b, int on) {
     if (!b) return GV_ERR_ARG;
     int rc = sync(b);
     if (rc != GV_OK) return rc;
-    if (on && !b->d_kb && hipMalloc(&b->d_kb, 16) != hipSuccess) return GV_ERR_NOMEM;
-    if (on) HIPCHK(hipMemsetAsync(b->d_kb, 0, 16, b->ctx->stream));
+    if (on && !b->d_kb && hipMalloc(&b->d_kb, kKbBytes) != hipSuccess) return GV_ERR_NOMEM;
+    if (on) HIPCHK(hipMemsetAsync(b->d_kb, 0, kKbBytes, b->ctx->stream));
     b->kb_on = on != 0;
     return capture(b);
 }
@@ -1397,9 +1405,14 @@ int gv_batch_profile_bytes(gv_batch_t b, uint64_t* size_bytes, uint64_t* walk_by
     if (!b) return GV_ERR_ARG;
     uint64_t v[2] = {0, 0};
     if (b->d_kb) {
-        HIPCHK(hipMemcpyAsync(v, b->d_kb, 16, hipMemcpyDeviceToHost, b->ctx->stream));
+        std::vector<uint64_t> h(kKbBytes / 8);
+        HIPCHK(hipMemcpyAsync(h.data(), b->d_kb, kKbBytes, hipMemcpyDeviceToHost, b->ctx->stream));
         int rc = sync(b);
         if (rc != GV_OK) return rc;
+        for (uint32_t i = 0; i < kBLines; ++i) {
+            v[0] += h[i * kAccStride];
+            v[1] += h[i * kAccStride + 1];
+        }
     }
     if (size_bytes) *size_bytes = v[0];
     if (walk_bytes) *walk_bytes = v[1];
