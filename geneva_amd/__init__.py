@@ -367,20 +367,23 @@ class Engine:
     def mine_batch(self, queries, single_v=False):
         """Mine all queries together on the batch engine.  Returns (results, stats)."""
         n = len(queries)
-        counts = (C.c_uint64 * max(n, 1))()
-        ab = (C.c_uint64 * max(n, 1))()
+        # numpy buffers (calloc'd): a ctypes array of n * nwords words is
+        # zeroed element by element, about 2 ms for 256 queries on 100,000 graphs
+        counts = np.zeros(max(n, 1), dtype=np.uint64)
+        ab = np.zeros(max(n, 1), dtype=np.uint64)
         nwords = (self.ngraphs + 31) // 32 if self.ngraphs > 1 else 0
-        bits = (C.c_uint32 * max(n * nwords, 1))()
+        words = np.zeros(max(n * nwords, 1), dtype=np.uint32)
         st = (C.c_double * 8)()
-        rc = lib().gh_engine_batch_mine(self._h, self._cstrs(queries), n, int(single_v), counts, ab,
-                                        bits, nwords, st)
+        u64p = C.POINTER(C.c_uint64)
+        rc = lib().gh_engine_batch_mine(self._h, self._cstrs(queries), n, int(single_v),
+                                        counts.ctypes.data_as(u64p), ab.ctypes.data_as(u64p),
+                                        words.ctypes.data_as(_u32p()), nwords, st)
         if rc:
             raise _err("mine_batch", rc)
-        words = np.ctypeslib.as_array(bits)
-        res = []
-        for i in range(n):
-            w = words[i * nwords:(i + 1) * nwords]
-            res.append(MineResult(counts[i], ab[i], 0.0, w, 0, self.ngraphs))
+        rows = words[:n * nwords].reshape(n, nwords) if nwords else [None] * n
+        cl, al = counts.tolist(), ab.tolist()
+        res = [MineResult(cl[i], al[i], 0.0, rows[i] if nwords else None, 0, self.ngraphs)
+               for i in range(n)]
         stats = dict(zip(("batched", "overflowed", "graph_launches", "device_ms", "wall_ms", "compile_ms",
                           "fallback_ms", "pieces"), list(st)))
         return res, stats
@@ -554,12 +557,12 @@ class Engine:
         count, ab, launches = C.c_uint64(0), C.c_uint64(0), C.c_uint64(0)
         ms = C.c_double(0)
         nwords = (self.ngraphs + 31) // 32 if self.ngraphs > 1 else 0
-        bits = (C.c_uint32 * max(nwords, 1))()
+        bits = np.zeros(max(nwords, 1), dtype=np.uint32)
         rc = lib().gh_engine_mine_shard(self._h, query_text.encode(), int(single_v), start_begin,
                                         start_end, stride, C.byref(count), C.byref(ab),
-                                        C.byref(ms), bits, nwords, C.byref(launches))
+                                        C.byref(ms), bits.ctypes.data_as(_u32p()), nwords,
+                                        C.byref(launches))
         if rc:
             raise _err("mine", rc)
-        return MineResult(count.value, ab.value, ms.value,
-                          np.ctypeslib.as_array(bits)[:nwords] if nwords else None,
+        return MineResult(count.value, ab.value, ms.value, bits[:nwords] if nwords else None,
                           launches.value, self.ngraphs)
