@@ -688,6 +688,13 @@ __global__ __launch_bounds__(1024) void k_b_tscan(BArgs a) {
 
 // ---- walk -------------------------------------------------------------------
 
+// Rows of a multi-row walk tile whose candidate starts a wave stages in LDS:
+// twice the candidates of a tile, since a batch's tables hold many rows with
+// no candidate (the rows of workload 3's first iteration average 0.43
+// candidates), so its tiles span more rows than candidates.
+template <int IPT>
+constexpr uint32_t kBLxRows = 128u * IPT;
+
 // The walk over the extension slots' tiles (kCount false) or the counted
 // slots' (true): two kernels, so that each holds only its own path's
 // registers (one more wave per SIMD each).
@@ -695,7 +702,7 @@ template <int IPT, bool kCount>
 __global__ __launch_bounds__(kBlock) void k_b_walk(BArgs a) {
     __shared__ uint64_t wend[kMaxSlots];
     __shared__ uint16_t wslot[kMaxSlots];
-    __shared__ uint32_t wl[kWavesPerBlock][wave_lds_words<IPT>()];
+    __shared__ uint32_t wl[kWavesPerBlock][wave_lds_words<IPT, kBLxRows<IPT>>()];
     for (uint32_t i = threadIdx.x; i < a.nslots; i += blockDim.x) {
         wend[i] = a.wend[i];
         wslot[i] = a.wslot[i];
@@ -757,12 +764,12 @@ __global__ __launch_bounds__(kBlock) void k_b_walk(BArgs a) {
         const gv_step_t& st = D.q.step[L];
         if constexpr (!kCount) {
             TileOut o{slot_out(a, s, S), a.slot_uints / (L + 1), &a.st[s].out_rows, &a.st[s].overflow};
-            walk_tile<false, IPT>(a.g, st, st, false, in, t0, t1, o, wl[wib], wl[wib], wl[wib] + 64 * IPT + 1, gb,
+            walk_tile<false, IPT, kBLxRows<IPT>>(a.g, st, st, false, in, t0, t1, o, wl[wib], wl[wib], wl[wib] + kBLxRows<IPT> + 1, gb,
                                   acc, reserve);
         } else {
             TileOut o{nullptr, 0, nullptr, nullptr};
             const gv_step_t& tl = D.q.step[L + 1 < GV_MAXK ? L + 1 : GV_MAXK - 1];
-            walk_tile<true, IPT>(a.g, st, tl, S.mode == 3, in, t0, t1, o, nullptr, wl[wib], wl[wib] + 64 * IPT + 1, gb,
+            walk_tile<true, IPT, kBLxRows<IPT>>(a.g, st, tl, S.mode == 3, in, t0, t1, o, nullptr, wl[wib], wl[wib] + kBLxRows<IPT> + 1, gb,
                                  acc, reserve);
             reserve(nullptr, 0);
         }

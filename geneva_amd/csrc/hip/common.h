@@ -762,21 +762,23 @@ struct BlockReserve {
     }
 };
 
-// LDS words per wave of a walk with IPT candidates per lane: stage placed over
-// lx and win (see walk_tile).
-template <int IPT>
+// LDS words per wave of a walk with IPT candidates per lane that stages the
+// candidate starts of up to LX rows of a multi-row tile (a tile over more rows
+// searches its rows in global memory): stage placed over lx and win (see
+// walk_tile).
+template <int IPT, uint32_t LX = 64u * IPT>
 constexpr uint32_t wave_lds_words() {
-    return 64u * IPT + 1u + 2u * kWin > 64u * GV_MAXK ? 64u * IPT + 1u + 2u * kWin : 64u * GV_MAXK;
+    return LX + 1u + 2u * kWin > 64u * GV_MAXK ? LX + 1u + 2u * kWin : 64u * GV_MAXK;
 }
 
-// LDS of a wave: lx (64 * IPT + 1 words), the candidate starts of the tile's
+// LDS of a wave: lx (LX + 1 words), the candidate starts of the tile's
 // rows relative to t0, so that a lane finds its row without global reads; win
 // (2 * kWin words), windows of the other back lists; stage (64 * (L + 1)
 // words), survivor rows on their way out.  stage is used only after lx and win
 // are done with, so a kernel may place it over them (wave_lds_words).
 // reserve: the BlockReserve of the block (extension only; every wave of the
 // block reaches it once per call of walk_tile).
-template <bool kCount, int IPT>
+template <bool kCount, int IPT, uint32_t LX = 64u * IPT>
 __device__ __forceinline__ void walk_tile(const DevGraph& g, const gv_step_t& st,
                                           const gv_step_t& tail, bool has_tail, const TileIn& in,
                                           uint64_t t0, uint64_t t1, const TileOut& o,
@@ -928,7 +930,7 @@ __device__ __forceinline__ void walk_tile(const DevGraph& g, const gv_step_t& st
         // which the search over c - t0 < 64 * IPT does not see; row r_lo's
         // own start is first0.
         const uint64_t nr = r_hi - r_lo + 1;
-        const bool staged = nr <= 64ull * IPT;
+        const bool staged = nr <= LX;
         const uint64_t first0 = in.pf.begin(r_lo);
         // up to kRowStage rows: each row's data is also staged once (in the
         // window space, kRowWords words a row): anchor start, the other
@@ -937,6 +939,7 @@ __device__ __forceinline__ void walk_tile(const DevGraph& g, const gv_step_t& st
         const bool rstaged = staged && nr <= kRowStage && st.nback <= 3 && st.ncoll <= 3;
         const uint32_t nob = st.nback - 1;
         if (staged) {
+#pragma unroll 4
             for (uint32_t i = lane; i <= nr; i += 64) {
                 const uint64_t x = i < nr ? in.pf.begin(r_lo + i) : in.pf.end(r_hi);
                 lx[i] = x <= t0 ? 0u : (x - t0 > 0xffffffffull ? 0xffffffffu : (uint32_t)(x - t0));
