@@ -762,6 +762,10 @@ struct BlockReserve {
     }
 };
 
+// A multi-row tile over more than LX rows stages every sr-th row's candidate
+// start, up to sr = kLxStride (beyond, its rows are searched in global memory).
+constexpr uint32_t kLxStride = 16;
+
 // LDS words per wave of a walk with IPT candidates per lane that stages the
 // candidate starts of up to LX rows of a multi-row tile (a tile over more rows
 // searches its rows in global memory): stage placed over lx and win (see
@@ -923,14 +927,18 @@ __device__ __forceinline__ void walk_tile(const DevGraph& g, const gv_step_t& st
         // the windows are rewritten by the next tile
         if (nob > 0) __builtin_amdgcn_wave_barrier();
     } else {
-        // rows r_lo..r_hi hold the tile: when they fit, stage their candidate
-        // starts in LDS relative to t0 (lx[i] = start of row r_lo + i, lx[nr]
-        // = end of row r_hi).  Rows after r_lo start inside the tile, so only
-        // lx[0] (clamped to 0) and lx[nr] (clamped to 2^32 - 1) are not exact,
-        // which the search over c - t0 < 64 * IPT does not see; row r_lo's
-        // own start is first0.
+        // rows r_lo..r_hi hold the tile: stage the candidate starts of every
+        // sr-th of them in LDS relative to t0 (lx[i] = start of row
+        // r_lo + i * sr, lx[ns] = end of row r_hi; sr = 1 when the rows fit
+        // LX).  Rows after r_lo start inside the tile, so only lx[0] (clamped
+        // to 0) and lx[ns] (clamped to 2^32 - 1) are not exact, which the
+        // search over c - t0 < 64 * IPT does not see; row r_lo's own start is
+        // first0.  A lane finds its run of sr rows in LDS and, for sr > 1,
+        // its row among them in global memory.
         const uint64_t nr = r_hi - r_lo + 1;
-        const bool staged = nr <= LX;
+        const uint32_t sr = nr <= LX ? 1u : (uint32_t)((nr + LX - 1) / LX);
+        const bool staged = sr <= kLxStride;
+        const uint32_t ns = staged ? (uint32_t)((nr + sr - 1) / sr) : 0u;
         const uint64_t first0 = in.pf.begin(r_lo);
         // up to kRowStage rows: each row's data is also staged once (in the
         // window space, kRowWords words a row): anchor start, the other
@@ -940,8 +948,8 @@ __device__ __forceinline__ void walk_tile(const DevGraph& g, const gv_step_t& st
         const uint32_t nob = st.nback - 1;
         if (staged) {
 #pragma unroll 4
-            for (uint32_t i = lane; i <= nr; i += 64) {
-                const uint64_t x = i < nr ? in.pf.begin(r_lo + i) : in.pf.end(r_hi);
+            for (uint32_t i = lane; i <= ns; i += 64) {
+                const uint64_t x = i < ns ? in.pf.begin(r_lo + (uint64_t)i * sr) : in.pf.end(r_hi);
                 lx[i] = x <= t0 ? 0u : (x - t0 > 0xffffffffull ? 0xffffffffu : (uint32_t)(x - t0));
             }
             if (rstaged && lane < nr) {
@@ -1019,7 +1027,7 @@ __device__ __forceinline__ void walk_tile(const DevGraph& g, const gv_step_t& st
                 if (staged) {
                     // first i with lx[i + 1] > c - t0
                     const uint32_t cr = (uint32_t)(c - t0);
-                    uint32_t lo = 0, n = (uint32_t)nr;
+                    uint32_t lo = 0, n = ns;
                     while (n > 0) {
                         const uint32_t h = n >> 1;
                         if (lx[lo + h + 1] <= cr) {
@@ -1029,8 +1037,15 @@ __device__ __forceinline__ void walk_tile(const DevGraph& g, const gv_step_t& st
                             n = h;
                         }
                     }
-                    r = r_lo + lo;
-                    first = lo == 0 ? first0 : t0 + lx[lo];
+                    if (sr == 1) {
+                        r = r_lo + lo;
+                        first = lo == 0 ? first0 : t0 + lx[lo];
+                    } else {
+                        const uint64_t b0 = r_lo + (uint64_t)lo * sr;
+                        const uint64_t b1 = b0 + sr < r_hi + 1 ? b0 + sr : r_hi + 1;
+                        r = find_row(in.pf, b0, b1, c);
+                        first = in.pf.begin(r);
+                    }
                 } else {
                     r = find_row(in.pf, r_lo, r_hi + 1, c);
                     first = in.pf.begin(r);
