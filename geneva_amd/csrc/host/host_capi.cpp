@@ -551,9 +551,12 @@ int gh_engine_batch_mine(gh_engine_t e, const char* const* queries, uint32_t n, 
         }
         if (counts) counts[i] = res[i].count;
         if (alg_bytes) alg_bytes[i] = res[i].algBytes;
-        if (bits && nwords)
-            for (uint32_t w = 0; w < nwords; ++w)
-                bits[(uint64_t)i * nwords + w] = w < res[i].graphBits.size() ? res[i].graphBits[w] : 0;
+        if (bits && nwords) {
+            uint32_t* dst = bits + (uint64_t)i * nwords;
+            const size_t m = std::min<size_t>(res[i].graphBits.size(), nwords);
+            if (m) std::memcpy(dst, res[i].graphBits.data(), m * sizeof(uint32_t));
+            if (m < nwords) std::memset(dst + m, 0, (nwords - m) * sizeof(uint32_t));
+        }
     }
     if (stats) {
         stats[0] = (double)st.batched;
