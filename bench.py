@@ -581,7 +581,7 @@ def main():
     ap.add_argument("--qps", type=float, nargs="*", default=[100, 200, 500, 1000, 2000, 5000, 10000])
     ap.add_argument("--serve-seconds", type=float, default=1.5, help="seconds of arrivals per rate")
     ap.add_argument("--serve-min", type=int, default=300)
-    ap.add_argument("--serve-max", type=int, default=8000)
+    ap.add_argument("--serve-max", type=int, default=15000)
     ap.add_argument("--mem-budget", type=int, default=32, help="GiB of estimated tables in flight")
     ap.add_argument("--no-served", action="store_true")
     ap.add_argument("--no-w2", action="store_true")

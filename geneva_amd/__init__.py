@@ -417,16 +417,16 @@ class Engine:
         cnt = (C.c_uint64 * max(n, 1))()
         st = (C.c_double * 6)()
         nwords = (self.ngraphs + 31) // 32 if (with_graphs and self.ngraphs > 1) else 0
-        bits = (C.c_uint32 * max(n * nwords, 1))()
+        words = np.zeros(max(n * nwords, 1), dtype=np.uint32)
         rc = lib().gh_engine_serve(self._h, self._cstrs(queries), arr, n, 0 if batched else 1,
-                                   mem_budget, lat, cnt, bits if nwords else None, nwords, st)
+                                   mem_budget, lat, cnt, words.ctypes.data_as(_u32p()) if nwords else None,
+                                   nwords, st)
         if rc:
             raise _err("serve", rc)
         stats = dict(zip(("wall_s", "iterations", "max_concurrent", "overflowed", "direct",
                           "max_batched"), list(st)))
         if not with_graphs:
             return list(lat)[:n], list(cnt)[:n], stats
-        words = np.ctypeslib.as_array(bits)
         ids = [_bits_to_ids(words[i * nwords:(i + 1) * nwords], self.ngraphs) if nwords else frozenset()
                for i in range(n)]
         return list(lat)[:n], list(cnt)[:n], stats, ids
