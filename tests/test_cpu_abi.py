@@ -78,6 +78,12 @@ def test_argument_checks_without_a_device():
     assert lib.gv_graph_destroy(None) == -1
     assert lib.gv_frontier_create(None, 10, None) == -1
     assert lib.gv_size_rows(None, None, None, None, 1, 1, None, None) == -1
+    # batch engine entries on no batch: rejected, nothing touched
+    rows = C.c_uint64(0)
+    assert lib.gv_batch_fuse_sizing(None, 1) == -1
+    assert lib.gv_batch_fused_rows(None, C.byref(rows)) == -1
+    assert lib.gv_batch_share(None, 1) == -1
+    assert lib.gv_batch_run(None, 1) == -1
     n = C.c_int(-1)
     assert lib.gv_device_count(C.byref(n)) == 0
     assert n.value >= 0
