@@ -110,8 +110,9 @@ private:
 // slot after several iterations and are mined again on the plain path; on
 // workload 5 at 10,000 offered q/s a share of 1/4 instead of the whole table
 // cut such overflows from 75 to 21 in 15,000 requests and raised the batched
-// server's rate by about 1.5 % (1/8 and 1/16 send so many requests to the one
-// side worker that it becomes the bottleneck).
+// server's rate by about 1.5 % (1/3 leaves 53 overflows and less of the
+// gain; 1/6, 1/8 and 1/16 send so many requests to the one side worker that
+// it becomes the bottleneck).
 constexpr uint64_t kDirectShare = 4;
 
 int serveBatched(BatchMiner& bm, const std::vector<CompiledQuery>& reqs,
